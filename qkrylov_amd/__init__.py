@@ -1,0 +1,562 @@
+"""qkrylov_amd — qkrylov's matrix-free H·x and Krylov solvers on one MI355X.
+
+Host-side mirror of the reference Python package (bindings/python/qkrylov:
+basis.py, site.py, operators.py, hamiltonian.py, solvers.py) over the C ABI
+of ``libqkrylov.so`` (include/qkrylov/c_api.h).  Every numerical call runs
+the HIP kernels; vectors cross the boundary as complex128 numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from typing import List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, last_error, load
+
+__all__ = [
+    "Sector", "Basis", "SpinHalfBasis", "FermionBasis", "HubbardBasis", "TJBasis",
+    "Site", "SpinHalfSite", "FermionSite", "HubbardSite", "TJSite",
+    "OpSum", "Sz", "Sp", "Sm", "Sx", "Sy", "CdagUp", "CUp", "CdagDn", "CDn",
+    "Nup", "Ndn", "Nupdn", "N", "C", "Cdag", "Id",
+    "MatrixFreeHamiltonian", "LanczosResult", "lanczos_ground_state",
+    "DavidsonResult", "davidson_lowest", "DynamicsResult", "continued_fraction_coeffs",
+    "continued_fraction_block", "evaluate_spectral_function", "FTLMResult", "ftlm",
+    "ftlm_sweep", "device_info",
+]
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def device_info() -> str:
+    buf = ct.create_string_buffer(256)
+    check(load().qkrylov_device_info(buf, 256), "device_info")
+    return buf.value.decode()
+
+
+# --------------------------------------------------------------------------- sector
+class Sector:
+    """Quantum-number selection (reference include/qkrylov/symmetry/sector.hpp)."""
+
+    def __init__(self):
+        self._h = load().qkrylov_sector_create()
+        if not self._h:
+            raise RuntimeError(last_error())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().qkrylov_sector_destroy(self._h)
+            self._h = None
+
+    def set_sz(self, sz2: int):
+        check(load().qkrylov_sector_set_sz(self._h, int(sz2)), "set_sz")
+        return self
+
+    def set_hubbard_particles(self, nup: int, ndn: int):
+        check(load().qkrylov_sector_set_hubbard_particles(self._h, int(nup), int(ndn)), "set_particles")
+        return self
+
+    def set_n(self, n: int):
+        check(load().qkrylov_sector_set_n(self._h, int(n)), "set_n")
+        return self
+
+    def set_nb(self, nb: int):
+        check(load().qkrylov_sector_set_nb(self._h, int(nb)), "set_nb")
+        return self
+
+    def _get(self, fn, n):
+        vals = [ct.c_int() for _ in range(n)]
+        check(fn(self._h, *[ct.byref(v) for v in vals]), "sector get")
+        return [v.value for v in vals]
+
+    def get_sz(self):
+        sz2, act = self._get(load().qkrylov_sector_get_sz, 2)
+        return sz2 if act else None
+
+    def get_n(self):
+        n, act = self._get(load().qkrylov_sector_get_n, 2)
+        return n if act else None
+
+    def get_nb(self):
+        nb, act = self._get(load().qkrylov_sector_get_nb, 2)
+        return nb if act else None
+
+    def get_hubbard_particles(self):
+        nup, ndn, act = self._get(load().qkrylov_sector_get_hubbard_particles, 3)
+        return (nup, ndn) if act else None
+
+
+# --------------------------------------------------------------------------- bases
+class Basis:
+    """Ascending-integer basis with closed-form ranking (reference src/basis/*)."""
+
+    _create = None
+
+    def __init__(self, N: int, sector: Optional[Sector] = None):
+        sec = sector if sector is not None else Sector()
+        self._sector = sec
+        self._h = getattr(load(), self._create)(int(N), sec._h)
+        if not self._h:
+            raise ValueError(last_error())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().qkrylov_basis_destroy(self._h)
+            self._h = None
+
+    @property
+    def size(self) -> int:
+        return int(load().qkrylov_basis_dimension(self._h))
+
+    @property
+    def nsites(self) -> int:
+        return int(load().qkrylov_basis_nsites(self._h))
+
+    def state(self, i: int) -> int:
+        if not 0 <= i < self.size:
+            raise IndexError("basis state index out of range")
+        return int(load().qkrylov_basis_state(self._h, int(i)))
+
+    def index(self, s: int) -> int:
+        idx = int(load().qkrylov_basis_index(self._h, int(s)))
+        if idx < 0:
+            raise KeyError("State not present in basis")
+        return idx
+
+    def contains(self, s: int) -> bool:
+        return bool(load().qkrylov_basis_contains(self._h, int(s)))
+
+    def states(self, first: int = 0, count: Optional[int] = None) -> np.ndarray:
+        n = self.size - first if count is None else count
+        out = np.empty(n, dtype=np.uint64)
+        check(load().qkrylov_basis_states(self._h, first, n, _ptr(out)), "states")
+        return out
+
+    def indices(self, states: Sequence[int]) -> np.ndarray:
+        st = np.ascontiguousarray(states, dtype=np.uint64)
+        out = np.empty(st.size, dtype=np.int64)
+        check(load().qkrylov_basis_indices(self._h, _ptr(st), st.size, _ptr(out)), "indices")
+        return out
+
+    def __len__(self):
+        return self.size
+
+
+class SpinHalfBasis(Basis):
+    _create = "qkrylov_spinhalf_basis_create"
+    site_kind = 0
+
+    def __init__(self, N: int, conserve_sz: bool = False, sz: Optional[float] = None,
+                 sector: Optional[Sector] = None):
+        if sector is None:
+            sector = Sector()
+            if sz is not None or conserve_sz:
+                sector.set_sz(int(round(2 * (sz or 0))))
+        super().__init__(N, sector)
+
+
+class FermionBasis(Basis):
+    _create = "qkrylov_fermion_basis_create"
+    site_kind = 1
+
+    def __init__(self, N: int, conserve_n: bool = False, n: int = 0,
+                 sector: Optional[Sector] = None):
+        if sector is None:
+            sector = Sector()
+            if conserve_n or n != 0:
+                sector.set_n(n)
+        super().__init__(N, sector)
+
+
+class _TwoSpecies(Basis):
+    def __init__(self, N: int, conserve_nup: bool = False, nup: int = 0,
+                 conserve_ndn: bool = False, ndn: int = 0, sector: Optional[Sector] = None):
+        if sector is None:
+            sector = Sector()
+            lib = load()
+            # the reference python wrapper sets each flag on its own
+            if conserve_nup or nup != 0 or conserve_ndn or ndn != 0:
+                sector.set_hubbard_particles(nup, ndn)
+                if not (conserve_nup or nup != 0) or not (conserve_ndn or ndn != 0):
+                    raise ValueError("conserve both nup and ndn through this wrapper, or pass a Sector")
+        super().__init__(N, sector)
+
+
+class HubbardBasis(_TwoSpecies):
+    _create = "qkrylov_hubbard_basis_create"
+    site_kind = 2
+
+
+class TJBasis(_TwoSpecies):
+    _create = "qkrylov_tj_basis_create"
+    site_kind = 3
+
+
+# --------------------------------------------------------------------------- sites
+class Site:
+    _create = None
+    kind = -1
+
+    def __init__(self):
+        self._h = getattr(load(), self._create)()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().qkrylov_site_destroy(self._h)
+            self._h = None
+
+
+class SpinHalfSite(Site):
+    _create, kind = "qkrylov_spinhalf_site_create", 0
+
+
+class FermionSite(Site):
+    _create, kind = "qkrylov_fermion_site_create", 1
+
+
+class HubbardSite(Site):
+    _create, kind = "qkrylov_hubbard_site_create", 2
+
+
+class TJSite(Site):
+    _create, kind = "qkrylov_tj_site_create", 3
+
+
+def site_for(basis: Basis) -> Site:
+    return [SpinHalfSite, FermionSite, HubbardSite, TJSite][basis.site_kind]()
+
+
+# --------------------------------------------------------------------------- operators
+class LocalOpExpr:
+    """One operator on one site, e.g. Sz(3) (reference operators.py:25-42)."""
+
+    def __init__(self, name: str, site: int):
+        self.name, self.site = name, int(site)
+
+    def __mul__(self, other):
+        return TermExpr(1.0, [self]) * other
+
+    def __rmul__(self, other):
+        return TermExpr(complex(other), [self])
+
+    def __add__(self, other):
+        return TermExpr(1.0, [self]) + other
+
+    def __sub__(self, other):
+        return TermExpr(1.0, [self]) - other
+
+
+class TermExpr:
+    def __init__(self, coeff: complex, ops: List[LocalOpExpr]):
+        self.coeff, self.ops = complex(coeff), list(ops)
+
+    def __mul__(self, other):
+        if isinstance(other, LocalOpExpr):
+            return TermExpr(self.coeff, self.ops + [other])
+        if isinstance(other, TermExpr):
+            return TermExpr(self.coeff * other.coeff, self.ops + other.ops)
+        if isinstance(other, OpSumExpr):
+            return OpSumExpr([self * t for t in other.terms])
+        return TermExpr(self.coeff * complex(other), self.ops)
+
+    def __rmul__(self, other):
+        return TermExpr(self.coeff * complex(other), self.ops)
+
+    def __add__(self, other):
+        return OpSumExpr([self]) + other
+
+    def __sub__(self, other):
+        return OpSumExpr([self]) - other
+
+    def __neg__(self):
+        return TermExpr(-self.coeff, self.ops)
+
+
+class OpSumExpr:
+    def __init__(self, terms: List[TermExpr]):
+        self.terms = list(terms)
+
+    def _as_terms(self, other):
+        if isinstance(other, OpSumExpr):
+            return other.terms
+        if isinstance(other, TermExpr):
+            return [other]
+        if isinstance(other, LocalOpExpr):
+            return [TermExpr(1.0, [other])]
+        raise TypeError(f"cannot combine OpSumExpr with {type(other)}")
+
+    def __add__(self, other):
+        return OpSumExpr(self.terms + self._as_terms(other))
+
+    def __sub__(self, other):
+        return OpSumExpr(self.terms + [-t for t in self._as_terms(other)])
+
+    def __mul__(self, other):
+        return OpSumExpr([t * other for t in self.terms])
+
+    def __rmul__(self, other):
+        return OpSumExpr([complex(other) * t for t in self.terms])
+
+
+def _op(name):
+    return lambda i: LocalOpExpr(name, i)
+
+
+Sz, Sp, Sm, Sx, Sy = (_op(n) for n in ("Sz", "Sp", "Sm", "Sx", "Sy"))
+CdagUp, CUp, CdagDn, CDn = (_op(n) for n in ("CdagUp", "CUp", "CdagDn", "CDn"))
+Nup, Ndn, Nupdn, N, C_, Cdag, Id = (_op(n) for n in ("Nup", "Ndn", "Nupdn", "N", "C", "Cdag", "Id"))
+globals()["C"] = C_  # fermion annihilator, named as the reference site op "C"
+
+
+class OpSum:
+    """Append-only list of (coeff, [(op, site), ...]) (reference opsum.hpp)."""
+
+    def __init__(self):
+        self.terms: List[Tuple[complex, List[Tuple[str, int]]]] = []
+
+    def add_term(self, coeff: complex, *ops):
+        if len(ops) % 2:
+            raise ValueError("add_term(coeff, op1, site1, [op2, site2, ...])")
+        facs = [(str(ops[i]), int(ops[i + 1])) for i in range(0, len(ops), 2)]
+        self.terms.append((complex(coeff), facs))
+        return self
+
+    def __iadd__(self, item):
+        if isinstance(item, tuple):
+            if len(item) < 3 or len(item) % 2 == 0:
+                raise ValueError("OpSum += requires (coeff, op1, site1, [op2, site2, ...]) with odd "
+                                 "tuple length >= 3")
+            return self.add_term(*item)
+        if isinstance(item, LocalOpExpr):
+            item = TermExpr(1.0, [item])
+        if isinstance(item, TermExpr):
+            item = OpSumExpr([item])
+        if isinstance(item, OpSumExpr):
+            for t in item.terms:
+                self.terms.append((t.coeff, [(o.name, o.site) for o in t.ops]))
+            return self
+        raise TypeError(f"cannot add {type(item)} to OpSum")
+
+    def clear(self):
+        self.terms.clear()
+
+    @property
+    def size(self) -> int:
+        return len(self.terms)
+
+    def __len__(self):
+        return len(self.terms)
+
+    def _handle(self):
+        lib = load()
+        h = lib.qkrylov_opsum_create()
+        for coeff, facs in self.terms:
+            n = len(facs)
+            ops = (ct.c_char_p * max(n, 1))(*[f[0].encode() for f in facs])
+            sites = (ct.c_int * max(n, 1))(*[f[1] for f in facs])
+            code = lib.qkrylov_opsum_add_term_f64(h, coeff.real, coeff.imag, n, ops, sites)
+            if code != 0:
+                lib.qkrylov_opsum_destroy(h)
+                check(code, "add_term")
+        return h
+
+    def flat(self):
+        """(coeff_re, coeff_im, nfac, ops, sites) arrays in term order."""
+        cre = np.array([t[0].real for t in self.terms], dtype=np.float64)
+        cim = np.array([t[0].imag for t in self.terms], dtype=np.float64)
+        nfac = np.array([len(t[1]) for t in self.terms], dtype=np.int32)
+        ops = [f[0] for t in self.terms for f in t[1]]
+        sites = np.array([f[1] for t in self.terms for f in t[1]], dtype=np.int32)
+        return cre, cim, nfac, ops, sites
+
+
+# --------------------------------------------------------------------------- Hamiltonian
+class MatrixFreeHamiltonian:
+    """Matrix-free H on the GPU (reference hamiltonian.py / matrix_free_hamiltonian.hpp).
+
+    ``apply(x)`` computes y[a] = sum_terms conj(amp(a->b)) x[b] in FP64.
+    """
+
+    def __init__(self, basis: Basis, site: Optional[Site] = None, ops: Optional[OpSum] = None):
+        if ops is None and isinstance(site, OpSum):
+            site, ops = None, site
+        if site is None:
+            site = site_for(basis)
+        self.basis, self.site, self.ops = basis, site, ops
+        lib = load()
+        oh = ops._handle()
+        try:
+            self._h = lib.qkrylov_hamiltonian_create(basis._h, site._h, oh)
+        finally:
+            lib.qkrylov_opsum_destroy(oh)
+        if not self._h:
+            raise RuntimeError(f"Hamiltonian create failed: {last_error()}")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().qkrylov_hamiltonian_destroy(self._h)
+            self._h = None
+
+    @property
+    def dimension(self) -> int:
+        return int(load().qkrylov_hamiltonian_dimension(self._h))
+
+    def info(self) -> dict:
+        import json
+        buf = ct.create_string_buffer(4096)
+        check(load().qkrylov_hamiltonian_info(self._h, buf, 4096), "info")
+        return json.loads(buf.value.decode())
+
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        d = self.dimension
+        if x.shape[-1] != d or x.ndim > 2:
+            raise ValueError("Input array size does not match Hamiltonian dimension")
+        nvec = 1 if x.ndim == 1 else x.shape[0]
+        y = np.empty_like(x)
+        if d == 0 or nvec == 0:
+            return y
+        check(load().qkrylov_hamiltonian_apply_f64(self._h, _ptr(x), _ptr(y), nvec), "apply")
+        return y
+
+    __matmul__ = apply
+
+    def apply_device(self, x_ptr: int, y_ptr: int, nvec: int = 1, ld: int = 0, stream: int = 0):
+        check(load().qkrylov_hamiltonian_apply_device_f64(self._h, x_ptr, y_ptr, nvec, ld,
+                                                          stream or None), "apply_device")
+
+    def sz_q_device(self, psi_ptr: int, phi_ptr: int, nq: int, ld: int, stream: int = 0):
+        check(load().qkrylov_sz_q_device_f64(self._h, psi_ptr, phi_ptr, nq, ld, stream or None),
+              "sz_q")
+
+    def sync(self):
+        check(load().qkrylov_hamiltonian_sync(self._h), "sync")
+
+    def diagonal(self) -> np.ndarray:
+        out = np.empty(self.dimension, dtype=np.complex128)
+        check(load().qkrylov_hamiltonian_diagonal_f64(self._h, _ptr(out)), "diagonal")
+        return out
+
+    def __repr__(self):
+        return f"MatrixFreeHamiltonian(dim={self.dimension})"
+
+
+# --------------------------------------------------------------------------- solvers
+class LanczosResult:
+    def __init__(self, energy, eigenvector, iterations, converged):
+        self.energy, self.eigenvector = energy, eigenvector
+        self.iterations, self.converged = iterations, converged
+
+    def __iter__(self):
+        yield self.energy
+        yield self.eigenvector
+
+    def __repr__(self):
+        return (f"LanczosResult(energy={self.energy}, iterations={self.iterations}, "
+                f"converged={self.converged})")
+
+
+def lanczos_ground_state(H: MatrixFreeHamiltonian, maxiter: int = 200, tol: float = 1e-12,
+                         start: Optional[np.ndarray] = None, return_state: bool = True):
+    d = H.dimension
+    res = _lib.LanczosResultF64()
+    vec = np.empty(d, dtype=np.complex128) if return_state else None
+    st = None if start is None else np.ascontiguousarray(start, dtype=np.complex128)
+    check(load().qkrylov_lanczos_ground_state_f64(
+        H._h, int(maxiter), float(tol), None if st is None else _ptr(st), ct.byref(res),
+        None if vec is None else _ptr(vec)), "lanczos")
+    return LanczosResult(res.energy, vec, res.iterations, bool(res.converged))
+
+
+class DavidsonResult:
+    def __init__(self, eigenvalues, eigenvectors, iterations, converged):
+        self.eigenvalues, self.eigenvectors = eigenvalues, eigenvectors
+        self.iterations, self.converged = iterations, converged
+
+
+def davidson_lowest(H: MatrixFreeHamiltonian, n_eig: int = 1, max_subspace: int = 20,
+                    tol: float = 1e-8, return_vectors: bool = True):
+    d = H.dimension
+    k = min(n_eig, d)
+    ev = np.zeros(max(n_eig, 1), dtype=np.float64)
+    vecs = np.zeros((max(n_eig, 1), d), dtype=np.complex128) if return_vectors else None
+    info = _lib.DavidsonResultC()
+    check(load().qkrylov_davidson_lowest_f64(H._h, int(n_eig), int(max_subspace), float(tol),
+                                             _ptr(ev), None if vecs is None else _ptr(vecs),
+                                             ct.byref(info)), "davidson")
+    return DavidsonResult(ev[:k], None if vecs is None else list(vecs[:k]), info.iterations,
+                          bool(info.converged))
+
+
+class DynamicsResult:
+    def __init__(self, alphas, betas, norm_phi0):
+        self.alphas, self.betas, self.norm_phi0 = alphas, betas, norm_phi0
+
+
+def continued_fraction_block(H: MatrixFreeHamiltonian, phi0: np.ndarray, n_iter: int = 100):
+    p = np.ascontiguousarray(np.atleast_2d(phi0), dtype=np.complex128)
+    nb = p.shape[0]
+    a = np.zeros((nb, n_iter))
+    b = np.zeros((nb, n_iter))
+    nrm = np.zeros(nb)
+    cnt = np.zeros(nb, dtype=np.int32)
+    check(load().qkrylov_continued_fraction_block_f64(H._h, _ptr(p), nb, int(n_iter), _ptr(a),
+                                                      _ptr(b), _ptr(nrm), _ptr(cnt)), "cf")
+    out = []
+    for j in range(nb):
+        n = int(cnt[j])
+        nbeta = n if n == n_iter and b[j, n - 1] >= 1e-15 else max(n - 1, 0)
+        out.append(DynamicsResult(a[j, :n].copy(), b[j, :nbeta].copy(), float(nrm[j])))
+    return out
+
+
+def continued_fraction_coeffs(H: MatrixFreeHamiltonian, phi0: np.ndarray, n_iter: int = 100):
+    return continued_fraction_block(H, phi0, n_iter)[0]
+
+
+def evaluate_spectral_function(alphas, betas, norm_phi0: float, omega: float, E0: float,
+                               eta: float = 0.1) -> float:
+    if isinstance(alphas, DynamicsResult):
+        # evaluate_spectral_function(cfr, omega, E0, eta) as in the Julia wrapper
+        cfr = alphas
+        return evaluate_spectral_function(cfr.alphas, cfr.betas, cfr.norm_phi0, betas, norm_phi0,
+                                          omega)
+    a = np.ascontiguousarray(alphas, dtype=np.float64)
+    b = np.ascontiguousarray(betas, dtype=np.float64)
+    if a.size == 0:
+        return 0.0
+    if b.size < a.size - 1:
+        raise ValueError("alphas array length must match betas array length or betas array length + 1")
+    return float(load().qkrylov_evaluate_spectral_function_f64(_ptr(a), _ptr(b), a.size,
+                                                               float(norm_phi0), float(omega),
+                                                               float(E0), float(eta)))
+
+
+class FTLMResult:
+    def __init__(self, beta, Z, E, Cv):
+        self.beta, self.partition_function = beta, Z
+        self.internal_energy, self.specific_heat = E, Cv
+
+
+def ftlm(H: MatrixFreeHamiltonian, beta: float, n_random: int = 50, n_steps: int = 100):
+    r = _lib.FTLMResultF64()
+    check(load().qkrylov_ftlm_f64(H._h, float(beta), int(n_random), int(n_steps), ct.byref(r)),
+          "ftlm")
+    return FTLMResult(r.beta, r.partition_function, r.internal_energy, r.specific_heat)
+
+
+def ftlm_sweep(Hs: Sequence[MatrixFreeHamiltonian], betas, n_random: int = 50,
+               n_steps: int = 100, batched: bool = True):
+    """Z, E, Cv arrays of shape [len(Hs), len(betas)]; row s equals ftlm(Hs[s], beta)."""
+    b = np.ascontiguousarray(betas, dtype=np.float64)
+    ns, nb = len(Hs), b.size
+    hs = (ct.c_void_p * ns)(*[h._h for h in Hs])
+    Z = np.zeros((ns, nb))
+    E = np.zeros((ns, nb))
+    Cv = np.zeros((ns, nb))
+    check(load().qkrylov_ftlm_sweep_f64(hs, ns, _ptr(b), nb, int(n_random), int(n_steps),
+                                        0 if batched else 1, _ptr(Z), _ptr(E), _ptr(Cv)), "ftlm_sweep")
+    return Z, E, Cv

@@ -1,0 +1,170 @@
+// Host small eigensolvers (see qk_hostla.hpp).  Compiled with
+// -ffp-contract=off: every product and sum is rounded separately, as on the
+// reference's baseline x86-64 build, so the tridiagonal eigenvalues match the
+// reference's QL sweep operation for operation.
+#include "qk_hostla.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace qk {
+
+void tridiag_ql(const double* a, const double* b, int n, double* eig, double* z0, double* Z) {
+    if (n <= 0) return;
+    if (n == 1) {
+        eig[0] = a[0];
+        if (z0) z0[0] = 1.0;
+        if (Z) Z[0] = 1.0;
+        return;
+    }
+    std::vector<double> d(a, a + n), e(n, 0.0);
+    for (int i = 0; i + 1 < n; ++i) e[i] = b[i];
+    std::vector<double> row0;
+    if (z0) {
+        row0.assign(n, 0.0);
+        row0[0] = 1.0;
+    }
+    if (Z) {
+        std::fill(Z, Z + size_t(n) * n, 0.0);
+        for (int i = 0; i < n; ++i) Z[size_t(i) * n + i] = 1.0;
+    }
+
+    for (int sweep = 0; sweep < 1000; ++sweep) {
+        // deflate negligible couplings
+        for (int i = 0; i + 1 < n; ++i)
+            if (std::abs(e[i]) < 1e-14 * (std::abs(d[i]) + std::abs(d[i + 1]))) e[i] = 0.0;
+        // trailing unreduced block [lo, hi]
+        int hi = n - 1;
+        while (hi > 0 && e[hi - 1] == 0.0) --hi;
+        if (hi == 0) break;
+        int lo = hi - 1;
+        while (lo > 0 && e[lo - 1] != 0.0) --lo;
+
+        // shift from the trailing 2x2 (eigenvalue nearer d[hi])
+        const double half = (d[hi - 1] - d[hi]) / 2.0;
+        const double cc = e[hi - 1] * e[hi - 1];
+        const double rt = std::sqrt(half * half + cc);
+        const double mu = (half > 0) ? d[hi] - cc / (half + rt) : d[hi] - cc / (half - rt);
+
+        double x = d[lo] - mu;
+        double y = e[lo];
+        for (int i = lo; i < hi; ++i) {
+            const double r = std::hypot(x, y);
+            const double cs = x / r;
+            const double sn = y / r;
+            if (i > lo) e[i - 1] = r;
+            const double f = cs * d[i] + sn * e[i];
+            const double g = cs * e[i] + sn * d[i + 1];
+            const double h = sn * d[i] - cs * e[i];
+            const double k = sn * e[i] - cs * d[i + 1];
+            d[i] = cs * f + sn * g;
+            e[i] = cs * h + sn * k;
+            d[i + 1] = sn * h - cs * k;
+            if (z0) {
+                const double z1 = row0[i], z2 = row0[i + 1];
+                row0[i] = cs * z1 + sn * z2;
+                row0[i + 1] = sn * z1 - cs * z2;
+            }
+            if (Z) {
+                for (int j = 0; j < n; ++j) {
+                    double* zr = Z + size_t(j) * n;
+                    const double z1 = zr[i], z2 = zr[i + 1];
+                    zr[i] = cs * z1 + sn * z2;
+                    zr[i + 1] = sn * z1 - cs * z2;
+                }
+            }
+            if (i < hi - 1) {
+                x = e[i];
+                y = sn * e[i + 1];
+                e[i + 1] = -cs * e[i + 1];
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) eig[i] = d[i];
+    if (z0)
+        for (int i = 0; i < n; ++i) z0[i] = row0[i];
+}
+
+double tridiag_ground(const std::vector<double>& a, const std::vector<double>& b, int n,
+                      std::vector<double>* vec) {
+    if (n == 0) {
+        if (vec) vec->clear();
+        return 0.0;
+    }
+    std::vector<double> eig(n);
+    std::vector<double> Z;
+    if (vec) Z.assign(size_t(n) * n, 0.0);
+    tridiag_ql(a.data(), b.data(), n, eig.data(), nullptr, vec ? Z.data() : nullptr);
+    int imin = 0;
+    for (int i = 1; i < n; ++i)
+        if (eig[i] < eig[imin]) imin = i;
+    if (vec) {
+        vec->resize(n);
+        for (int j = 0; j < n; ++j) (*vec)[j] = Z[size_t(j) * n + imin];
+    }
+    return eig[imin];
+}
+
+void hermitian_jacobi(const std::vector<cplx>& Ain, int n, std::vector<double>& evals,
+                      std::vector<cplx>& vecs) {
+    std::vector<cplx> A = Ain, V(size_t(n) * n, cplx{0.0, 0.0});
+    for (int i = 0; i < n; ++i) V[size_t(i) * n + i] = cplx{1.0, 0.0};
+    auto at = [&](std::vector<cplx>& M, int i, int j) -> cplx& { return M[size_t(i) * n + j]; };
+    auto cabs_ = [](cplx z) { return std::hypot(z.re, z.im); };
+
+    for (int it = 0; it < 1000; ++it) {
+        double off = 0.0;
+        int p = 0, q = 0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                const double v = cabs_(at(A, i, j));
+                if (v > off) {
+                    off = v;
+                    p = i;
+                    q = j;
+                }
+            }
+        if (off < 1e-15) break;
+        const cplx app = at(A, p, p), aqq = at(A, q, q), apq = at(A, p, q);
+        const double r = cabs_(apq);
+        const double phi = 0.5 * std::atan2(2.0 * r, app.re - aqq.re);
+        const double c = std::cos(phi), sphi = std::sin(phi);
+        // J = [[c, -conj(s)], [s, c]] on (p, q); s = sin(phi) * conj(apq)/|apq|
+        // makes (J^H A J)_{pq} vanish for any complex apq (for real apq this
+        // is the reference's rotation).
+        const cplx s{sphi * apq.re / r, -sphi * apq.im / r};
+        const cplx sc{s.re, -s.im};
+        for (int i = 0; i < n; ++i) {
+            const cplx vip = at(V, i, p), viq = at(V, i, q);
+            at(V, i, p) = cadd(cmul({c, 0.0}, vip), cmul(s, viq));
+            at(V, i, q) = cadd(cmul({-sc.re, -sc.im}, vip), cmul({c, 0.0}, viq));
+            if (i != p && i != q) {
+                const cplx aip = at(A, i, p), aiq = at(A, i, q);
+                const cplx nip = cadd(cmul({c, 0.0}, aip), cmul(s, aiq));
+                const cplx niq = cadd(cmul({-sc.re, -sc.im}, aip), cmul({c, 0.0}, aiq));
+                at(A, i, p) = nip;
+                at(A, p, i) = cconj(nip);
+                at(A, i, q) = niq;
+                at(A, q, i) = cconj(niq);
+            }
+        }
+        const double ss = s.re * s.re + s.im * s.im;
+        const double cross = 2.0 * c * cmul(s, apq).re;
+        at(A, p, p) = cplx{c * c * app.re + ss * aqq.re + cross, 0.0};
+        at(A, q, q) = cplx{ss * app.re + c * c * aqq.re - cross, 0.0};
+        at(A, p, q) = cplx{0.0, 0.0};
+        at(A, q, p) = cplx{0.0, 0.0};
+    }
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int i, int j) { return at(A, i, i).re < at(A, j, j).re; });
+    evals.resize(n);
+    vecs.assign(size_t(n) * n, cplx{0.0, 0.0});
+    for (int k = 0; k < n; ++k) {
+        evals[k] = at(A, idx[k], idx[k]).re;
+        for (int j = 0; j < n; ++j) vecs[size_t(k) * n + j] = at(V, j, idx[k]);
+    }
+}
+
+}  // namespace qk

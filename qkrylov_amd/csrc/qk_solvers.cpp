@@ -1,0 +1,624 @@
+// Krylov solver drivers.  Vectors live in HBM; the host only sees the
+// Krylov scalars (alpha, beta, projected matrices) and runs the small
+// eigensolves.  Reference semantics are cited per routine.
+#include "qk_solvers.hpp"
+
+#include "qk_hostla.hpp"
+#include "qk_rng.hpp"
+#include "qk_vec.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+namespace qk {
+
+namespace {
+
+template <typename T>
+T fetch(const void* dev, hipStream_t st) {
+    T v;
+    QK_HIP(hipMemcpyAsync(&v, dev, sizeof(T), hipMemcpyDeviceToHost, st));
+    QK_HIP(hipStreamSynchronize(st));
+    return v;
+}
+
+void put_ptr(DevBuf& arr, int slot, const void* p, hipStream_t st) {
+    // pointer arrays are small; a synchronous copy keeps the host value alive
+    QK_HIP(hipMemcpyAsync(arr.get<char>() + sizeof(void*) * slot, &p, sizeof(void*),
+                          hipMemcpyHostToDevice, st));
+    QK_HIP(hipStreamSynchronize(st));
+}
+
+double dev_norm(const cplx* x, u64 n, DevBuf& red, DevBuf& sc, hipStream_t st) {
+    vec::norm2(x, n, sc.get<double>(), red, st);
+    return std::sqrt(fetch<double>(sc.get<double>(), st));
+}
+
+template <typename F>
+void parallel_for(int n, F&& f) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int nt = int(std::min<unsigned>(hw ? hw : 1, 16));
+    nt = std::max(1, std::min(nt, n));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            for (int i = t; i < n; i += nt) f(i);
+        });
+    for (auto& x : th) x.join();
+}
+
+// Two classical Gram-Schmidt passes of w against k vectors (pointer array
+// on the device).  Replaces the reference's single modified Gram-Schmidt
+// sweep (lanczos.cpp:149-151); both leave w orthogonal to the same span.
+void cgs2(const DevBuf& ptrs, int k, cplx* w, u64 n, DevBuf& h, DevBuf& red, hipStream_t st) {
+    if (k <= 0) return;
+    h.resize(sizeof(cplx) * k);
+    for (int pass = 0; pass < 2; ++pass) {
+        vec::mdot(ptrs.get<const cplx*>(), k, w, n, h.get<cplx>(), red, st);
+        vec::msub(ptrs.get<const cplx*>(), k, h.get<cplx>(), w, n, st);
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Lanczos ground state (reference src/solvers/lanczos.cpp:101-202)
+// ---------------------------------------------------------------------------
+LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol,
+                                   const cplx* start, bool want_vector) {
+    LanczosResult res;
+    const u64 d = H.dim();
+    if (d == 0) return res;
+    hipStream_t st = H.stream;
+    const int nmax = int(std::min<i64>(i64(maxiter), i64(d)));
+
+    std::vector<DevBuf> V;
+    V.reserve(size_t(std::max(nmax, 1)) + 1);
+    DevBuf w(sizeof(cplx) * d), red, sc(64 * sizeof(double)), h;
+    DevBuf ptrs(sizeof(void*) * (size_t(std::max(nmax, 1)) + 1));
+
+    V.emplace_back(sizeof(cplx) * d);
+    if (start) {
+        QK_HIP(hipMemcpyAsync(V[0].get(), start, sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+    } else {
+        const std::vector<cplx> s0 = uniform_start(d, 1234);
+        QK_HIP(hipMemcpyAsync(V[0].get(), s0.data(), sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+        QK_HIP(hipStreamSynchronize(st));
+    }
+    {   // normalize (vector_ops.hpp:91-97: unchanged when the norm is 0)
+        const double nrm = dev_norm(V[0].get<cplx>(), d, red, sc, st);
+        if (nrm != 0.0) vec::scale(V[0].get<cplx>(), d, cplx{1.0 / nrm, 0.0}, st);
+    }
+    put_ptr(ptrs, 0, V[0].get(), st);
+
+    std::vector<double> alphas, betas;
+    double e_old = 1e100;
+    for (int iter = 0; iter < nmax; ++iter) {
+        res.iterations = iter + 1;
+        cplx* v = V[iter].get<cplx>();
+        H.apply(v, w.get<cplx>(), 1, d, st);
+        vec::dot(v, w.get<cplx>(), d, sc.get<cplx>(), red, st);
+        const double alpha = fetch<cplx>(sc.get<cplx>(), st).re;
+        alphas.push_back(alpha);
+        const double bprev = iter > 0 ? betas.back() : 0.0;
+        sc.resize(64 * sizeof(double));
+        double ab[2] = {alpha, bprev};
+        QK_HIP(hipMemcpyAsync(sc.get<double>() + 8, ab, sizeof(ab), hipMemcpyHostToDevice, st));
+        vec::lanczos_sub(w.get<cplx>(), v, iter > 0 ? V[iter - 1].get<cplx>() : nullptr, d,
+                         sc.get<double>() + 8, sc.get<double>() + 9, st);
+        cgs2(ptrs, iter + 1, w.get<cplx>(), d, h, red, st);
+        const double beta = dev_norm(w.get<cplx>(), d, red, sc, st);
+        if (beta < 1e-15) {
+            res.converged = true;
+            break;
+        }
+        if (iter + 1 == nmax) break;
+        betas.push_back(beta);
+        V.emplace_back(sizeof(cplx) * d);
+        QK_HIP(hipMemcpyAsync(V.back().get(), w.get(), sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
+        vec::scale(V.back().get<cplx>(), d, cplx{1.0 / beta, 0.0}, st);
+        put_ptr(ptrs, iter + 1, V.back().get(), st);
+        if (iter > 0) {
+            const double e = tridiag_ground(alphas, betas, int(alphas.size()), nullptr);
+            if (std::abs(e - e_old) < tol) {
+                res.converged = true;
+                break;
+            }
+            e_old = e;
+        }
+    }
+
+    std::vector<double> z;
+    const int n = int(alphas.size());
+    res.energy = tridiag_ground(alphas, betas, n, want_vector ? &z : nullptr);
+    if (want_vector) {
+        DevBuf ritz(sizeof(cplx) * d), coef(sizeof(cplx) * std::max(n, 1));
+        QK_HIP(hipMemsetAsync(ritz.get(), 0, sizeof(cplx) * d, st));
+        if (n > 0) {
+            std::vector<cplx> cz(n);
+            for (int i = 0; i < n; ++i) cz[i] = cplx{z[i], 0.0};
+            QK_HIP(hipMemcpyAsync(coef.get(), cz.data(), sizeof(cplx) * n, hipMemcpyHostToDevice, st));
+            vec::lincomb(ptrs.get<const cplx*>(), n, coef.get<cplx>(), ritz.get<cplx>(), d, st);
+            QK_HIP(hipStreamSynchronize(st));
+        }
+        const double nrm = dev_norm(ritz.get<cplx>(), d, red, sc, st);
+        if (nrm != 0.0) vec::scale(ritz.get<cplx>(), d, cplx{1.0 / nrm, 0.0}, st);
+        res.eigenvector.resize(d);
+        QK_HIP(hipMemcpyAsync(res.eigenvector.data(), ritz.get(), sizeof(cplx) * d,
+                              hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// Davidson (reference src/solvers/davidson.cpp:85-227)
+// ---------------------------------------------------------------------------
+DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace, double tol,
+                               bool want_vectors, Preconditioner* pre) {
+    DavidsonResult res;
+    const u64 d = H.dim();
+    if (d == 0) return res;
+    hipStream_t st = H.stream;
+    n_eig = int(std::min<i64>(n_eig, i64(d)));
+    max_subspace = std::max(max_subspace, 2 * n_eig);
+    const int cap = max_subspace + std::max(n_eig, 1) + 2;
+
+    DevBuf diag(sizeof(cplx) * d), red, sc(64 * sizeof(double)), h;
+    H.diagonal(diag.get<cplx>(), st);
+    std::vector<cplx> dh(d);
+    QK_HIP(hipMemcpyAsync(dh.data(), diag.get(), sizeof(cplx) * d, hipMemcpyDeviceToHost, st));
+    QK_HIP(hipStreamSynchronize(st));
+    std::vector<u64> idx(d);
+    std::iota(idx.begin(), idx.end(), u64(0));
+    std::sort(idx.begin(), idx.end(), [&](u64 i, u64 j) { return dh[i].re < dh[j].re; });
+
+    std::vector<DevBuf> V, HV;
+    DevBuf pV(sizeof(void*) * cap * 2), pHV(sizeof(void*) * cap), pVHV(sizeof(void*) * cap * 2);
+    auto refresh_ptrs = [&]() {
+        std::vector<const void*> a, b, ab;
+        for (auto& x : V) a.push_back(x.get());
+        for (auto& x : HV) b.push_back(x.get());
+        ab = b;
+        ab.insert(ab.end(), a.begin(), a.end());
+        if (!a.empty())
+            QK_HIP(hipMemcpyAsync(pV.get(), a.data(), sizeof(void*) * a.size(), hipMemcpyHostToDevice, st));
+        if (!b.empty())
+            QK_HIP(hipMemcpyAsync(pHV.get(), b.data(), sizeof(void*) * b.size(), hipMemcpyHostToDevice, st));
+        if (!ab.empty())
+            QK_HIP(hipMemcpyAsync(pVHV.get(), ab.data(), sizeof(void*) * ab.size(), hipMemcpyHostToDevice, st));
+        QK_HIP(hipStreamSynchronize(st));
+    };
+
+    // Start vectors: unit vectors on the n_eig smallest Re(diag) (:111-139).
+    // Distinct unit vectors are exactly orthonormal, so the reference's
+    // orthogonalisation leaves them unchanged and its random fallback never
+    // triggers.
+    for (int i = 0; i < n_eig; ++i) {
+        V.emplace_back(sizeof(cplx) * d);
+        QK_HIP(hipMemsetAsync(V.back().get(), 0, sizeof(cplx) * d, st));
+        const cplx one{1.0, 0.0};
+        QK_HIP(hipMemcpyAsync(V.back().get<cplx>() + idx[i], &one, sizeof(cplx),
+                              hipMemcpyHostToDevice, st));
+        HV.emplace_back(sizeof(cplx) * d);
+        H.apply(V.back().get<cplx>(), HV.back().get<cplx>(), 1, d, st);
+        QK_HIP(hipStreamSynchronize(st));
+    }
+    refresh_ptrs();
+
+    std::vector<cplx> Hs;   // projected matrix, row-major m*m
+    int valid_rows = 0;
+    std::vector<DevBuf> R;
+    DevBuf coef;
+    for (int iter = 0; iter < 100; ++iter) {
+        const int m = int(V.size());
+        // H_sub[i][j] = <V_i, HV_j>, j <= i (:143-150); new rows only.
+        std::vector<cplx> Hn(size_t(m) * m, cplx{0.0, 0.0});
+        for (int i = 0; i < std::min(valid_rows, m); ++i)
+            for (int j = 0; j < std::min(valid_rows, m); ++j) Hn[size_t(i) * m + j] = Hs[size_t(i) * valid_rows + j];
+        h.resize(sizeof(cplx) * m);
+        for (int i = valid_rows; i < m; ++i) {
+            vec::mdot(pHV.get<const cplx*>(), i + 1, V[i].get<cplx>(), d, h.get<cplx>(), red, st);
+            std::vector<cplx> hv(i + 1);
+            QK_HIP(hipMemcpyAsync(hv.data(), h.get(), sizeof(cplx) * (i + 1), hipMemcpyDeviceToHost, st));
+            QK_HIP(hipStreamSynchronize(st));
+            for (int j = 0; j <= i; ++j) {
+                const cplx vij = cconj(hv[j]);   // <V_i, HV_j>
+                Hn[size_t(i) * m + j] = vij;
+                if (i != j) Hn[size_t(j) * m + i] = cconj(vij);
+            }
+        }
+        Hs = Hn;
+        valid_rows = m;
+
+        std::vector<double> ev;
+        std::vector<cplx> S;
+        hermitian_jacobi(Hs, m, ev, S);
+
+        // residuals r_k = sum_i s_ki (HV_i - lambda_k V_i) (:161-165)
+        const int ne = std::min(n_eig, m);
+        std::vector<cplx> C(size_t(ne) * 2 * m);
+        for (int k = 0; k < ne; ++k)
+            for (int i = 0; i < m; ++i) {
+                const cplx s = S[size_t(k) * m + i];
+                C[size_t(k) * 2 * m + i] = s;
+                C[size_t(k) * 2 * m + m + i] = cplx{-ev[k] * s.re, -ev[k] * s.im};
+            }
+        coef.resize(sizeof(cplx) * C.size());
+        QK_HIP(hipMemcpyAsync(coef.get(), C.data(), sizeof(cplx) * C.size(), hipMemcpyHostToDevice, st));
+        DevBuf Rb(sizeof(cplx) * d * std::max(ne, 1));
+        vec::lincomb_many(pVHV.get<const cplx*>(), 2 * m, coef.get<cplx>(), ne, Rb.get<cplx>(), d, d, st);
+        res.residuals.assign(ne, 0.0);
+        bool all_conv = true;
+        std::vector<int> need;
+        for (int k = 0; k < ne; ++k) {
+            res.residuals[k] = dev_norm(Rb.get<cplx>() + u64(k) * d, d, red, sc, st);
+            if (res.residuals[k] > tol) {
+                all_conv = false;
+                need.push_back(k);
+            }
+        }
+        if (all_conv || iter == 99) {
+            res.iterations = iter + 1;
+            res.converged = all_conv;
+            for (int k = 0; k < ne; ++k) res.eigenvalues.push_back(ev[k]);
+            if (want_vectors && ne > 0) {
+                std::vector<cplx> Sv(size_t(ne) * m);
+                for (int k = 0; k < ne; ++k)
+                    for (int i = 0; i < m; ++i) Sv[size_t(k) * m + i] = S[size_t(k) * m + i];
+                coef.resize(sizeof(cplx) * Sv.size());
+                QK_HIP(hipMemcpyAsync(coef.get(), Sv.data(), sizeof(cplx) * Sv.size(), hipMemcpyHostToDevice, st));
+                DevBuf E(sizeof(cplx) * d * ne);
+                vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, E.get<cplx>(), d, d, st);
+                res.eigenvectors.assign(ne, std::vector<cplx>(d));
+                for (int k = 0; k < ne; ++k)
+                    QK_HIP(hipMemcpyAsync(res.eigenvectors[k].data(), E.get<cplx>() + u64(k) * d,
+                                          sizeof(cplx) * d, hipMemcpyDeviceToHost, st));
+                QK_HIP(hipStreamSynchronize(st));
+            }
+            return res;
+        }
+
+        // corrections t = M(r) (:166-177)
+        std::vector<DevBuf> T;
+        for (int k : need) {
+            T.emplace_back(sizeof(cplx) * d);
+            const cplx* rk = Rb.get<cplx>() + u64(k) * d;
+            if (pre) pre->apply(H, rk, diag.get<cplx>(), ev[k], T.back().get<cplx>(), st);
+            else vec::jacobi_precond(rk, diag.get<cplx>(), ev[k], T.back().get<cplx>(), d, st);
+        }
+
+        // restart to the n_eig Ritz pairs (:196-211)
+        if (V.size() + T.size() > size_t(max_subspace)) {
+            std::vector<cplx> Sv(size_t(ne) * m);
+            for (int k = 0; k < ne; ++k)
+                for (int i = 0; i < m; ++i) Sv[size_t(k) * m + i] = S[size_t(k) * m + i];
+            coef.resize(sizeof(cplx) * Sv.size());
+            QK_HIP(hipMemcpyAsync(coef.get(), Sv.data(), sizeof(cplx) * Sv.size(), hipMemcpyHostToDevice, st));
+            DevBuf nv(sizeof(cplx) * d * ne), nhv(sizeof(cplx) * d * ne);
+            vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, nv.get<cplx>(), d, d, st);
+            vec::lincomb_many(pHV.get<const cplx*>(), m, coef.get<cplx>(), ne, nhv.get<cplx>(), d, d, st);
+            QK_HIP(hipStreamSynchronize(st));
+            std::vector<DevBuf> V2, HV2;
+            for (int k = 0; k < ne; ++k) {
+                V2.emplace_back(sizeof(cplx) * d);
+                HV2.emplace_back(sizeof(cplx) * d);
+                QK_HIP(hipMemcpyAsync(V2.back().get(), nv.get<cplx>() + u64(k) * d, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
+                QK_HIP(hipMemcpyAsync(HV2.back().get(), nhv.get<cplx>() + u64(k) * d, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
+            }
+            QK_HIP(hipStreamSynchronize(st));
+            V = std::move(V2);
+            HV = std::move(HV2);
+            valid_rows = 0;
+            refresh_ptrs();
+        }
+
+        // orthogonalise and expand (:213-223)
+        for (auto& t : T) {
+            cgs2(pV, int(V.size()), t.get<cplx>(), d, h, red, st);
+            const double nrm = dev_norm(t.get<cplx>(), d, red, sc, st);
+            if (nrm > 1e-10) {
+                vec::scale(t.get<cplx>(), d, cplx{1.0 / nrm, 0.0}, st);
+                HV.emplace_back(sizeof(cplx) * d);
+                H.apply(t.get<cplx>(), HV.back().get<cplx>(), 1, d, st);
+                V.push_back(std::move(t));
+                refresh_ptrs();
+            }
+        }
+    }
+    return DavidsonResult{};   // not reached (iter == 99 returns), as :226
+}
+
+// ---------------------------------------------------------------------------
+// Block three-term recurrence without reorthogonalisation, shared by FTLM
+// (reference ftlm.cpp:24-53) and continued fractions (dynamics.cpp:12-59).
+// Vectors are kept unnormalised: v_k = c_k u_k, so one step is one apply
+// plus one fused update pass.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BlockRec {
+    struct Grp {
+        const Hamiltonian* h;
+        int nvec;
+        u64 off;
+    };
+    std::vector<Grp> grps;
+    Segments seg;
+    DevBuf bufU, bufY, bufP, c, cp, bp, alive, alpha, beta, nrm;
+    int nsamp = 0;
+    hipStream_t st = nullptr;
+
+    void setup(const std::vector<std::pair<const Hamiltonian*, int>>& g, hipStream_t stream) {
+        st = stream;
+        std::vector<u64> lens;
+        u64 off = 0;
+        grps.clear();
+        for (auto& x : g) {
+            grps.push_back({x.first, x.second, off});
+            for (int r = 0; r < x.second; ++r) lens.push_back(x.first->dim());
+            off += x.first->dim() * u64(x.second);
+        }
+        seg.build(lens, st);
+        nsamp = seg.nsamp;
+        const size_t bytes = sizeof(cplx) * std::max<u64>(seg.total, 1);
+        bufU.resize(bytes);
+        bufY.resize(bytes);
+        bufP.resize(bytes);
+        const size_t ns = std::max(nsamp, 1);
+        c.resize(sizeof(double) * ns);
+        cp.resize(sizeof(double) * ns);
+        bp.resize(sizeof(double) * ns);
+        alive.resize(sizeof(int) * ns);
+        nrm.resize(sizeof(double) * ns);
+    }
+
+    // U holds the start vectors; returns their norms (host).
+    std::vector<double> start(double thresh) {
+        std::vector<double> n(nsamp);
+        vec::seg_norm(seg, bufU.get<cplx>(), nrm.get<double>(), st);
+        QK_HIP(hipMemcpyAsync(n.data(), nrm.get(), sizeof(double) * nsamp, hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+        std::vector<double> cv(nsamp), zero(nsamp, 0.0);
+        std::vector<int> al(nsamp);
+        for (int j = 0; j < nsamp; ++j) {
+            al[j] = n[j] >= thresh ? 1 : 0;
+            // normalize() leaves a zero vector unchanged (vector_ops.hpp:94-96)
+            cv[j] = n[j] > 0.0 ? 1.0 / n[j] : 1.0;
+        }
+        QK_HIP(hipMemcpyAsync(c.get(), cv.data(), sizeof(double) * nsamp, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(cp.get(), zero.data(), sizeof(double) * nsamp, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(bp.get(), zero.data(), sizeof(double) * nsamp, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(alive.get(), al.data(), sizeof(int) * nsamp, hipMemcpyHostToDevice, st));
+        QK_HIP(hipStreamSynchronize(st));
+        return n;
+    }
+
+    // M steps; alpha/beta on the host as [step][sample].
+    void run(int M, std::vector<double>& ah, std::vector<double>& bh) {
+        const size_t tot = size_t(std::max(M, 1)) * std::max(nsamp, 1);
+        alpha.resize(sizeof(double) * tot);
+        beta.resize(sizeof(double) * tot);
+        QK_HIP(hipMemsetAsync(alpha.get(), 0, sizeof(double) * tot, st));
+        QK_HIP(hipMemsetAsync(beta.get(), 0, sizeof(double) * tot, st));
+        cplx* U = bufU.get<cplx>();
+        cplx* Y = bufY.get<cplx>();
+        cplx* P = bufP.get<cplx>();
+        for (int step = 0; step < M; ++step) {
+            for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
+            double* a = alpha.get<double>() + size_t(step) * nsamp;
+            double* b = beta.get<double>() + size_t(step) * nsamp;
+            vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);
+            vec::seg_update(seg, Y, U, P, c.get<double>(), cp.get<double>(), bp.get<double>(), a, b,
+                            alive.get<int>(), step == 0 ? 1 : 0, st);
+            cplx* oldP = P;
+            P = U;
+            U = Y;
+            Y = oldP;
+        }
+        ah.resize(size_t(M) * nsamp);
+        bh.resize(size_t(M) * nsamp);
+        if (M > 0 && nsamp > 0) {
+            QK_HIP(hipMemcpyAsync(ah.data(), alpha.get(), sizeof(double) * M * nsamp, hipMemcpyDeviceToHost, st));
+            QK_HIP(hipMemcpyAsync(bh.data(), beta.get(), sizeof(double) * M * nsamp, hipMemcpyDeviceToHost, st));
+        }
+        QK_HIP(hipStreamSynchronize(st));
+    }
+};
+
+// Sequence of sample j: alphas until (and including) the breakdown step,
+// betas strictly before it (ftlm.cpp:44-46, dynamics.cpp:274-277).
+void extract(const std::vector<double>& ah, const std::vector<double>& bh, int M, int ns, int j,
+             std::vector<double>& a, std::vector<double>& b) {
+    a.clear();
+    b.clear();
+    for (int k = 0; k < M; ++k) {
+        a.push_back(ah[size_t(k) * ns + j]);
+        const double bt = bh[size_t(k) * ns + j];
+        if (bt < 1e-15) break;
+        b.push_back(bt);
+    }
+}
+
+}  // namespace
+
+std::vector<CFResult> continued_fraction_block_dev(const Hamiltonian& H, const cplx* phi0_dev,
+                                                   int nblock, int n_iter) {
+    std::vector<CFResult> out(std::max(nblock, 0));
+    const u64 d = H.dim();
+    if (d == 0 || nblock <= 0) return out;
+    hipStream_t st = H.stream;
+    BlockRec rec;
+    rec.setup({{&H, nblock}}, st);
+    QK_HIP(hipMemcpyAsync(rec.bufU.get(), phi0_dev, sizeof(cplx) * d * nblock,
+                          hipMemcpyDeviceToDevice, st));
+    const std::vector<double> nrm = rec.start(1e-15);
+    std::vector<double> ah, bh;
+    rec.run(n_iter, ah, bh);
+    for (int j = 0; j < nblock; ++j) {
+        if (nrm[j] < 1e-15) {   // dynamics.cpp:26
+            out[j].norm_phi0 = 0.0;
+            continue;
+        }
+        out[j].norm_phi0 = nrm[j];
+        extract(ah, bh, n_iter, nblock, j, out[j].alphas, out[j].betas);
+    }
+    return out;
+}
+
+std::vector<CFResult> continued_fraction_block(const Hamiltonian& H, const cplx* phi0, int nblock,
+                                               int n_iter) {
+    const u64 d = H.dim();
+    if (d == 0 || nblock <= 0) return std::vector<CFResult>(std::max(nblock, 0));
+    DevBuf buf(sizeof(cplx) * d * nblock);
+    QK_HIP(hipMemcpyAsync(buf.get(), phi0, sizeof(cplx) * d * nblock, hipMemcpyHostToDevice, H.stream));
+    QK_HIP(hipStreamSynchronize(H.stream));
+    return continued_fraction_block_dev(H, buf.get<cplx>(), nblock, n_iter);
+}
+
+double spectral_function(const double* alphas, const double* betas, size_t n, double norm_phi0,
+                         double omega, double E0, double eta) {
+    // dynamics.cpp:61-89: z = omega + E0 + i eta, backward recursion
+    if (n == 0) return 0.0;
+    const double zr = omega + E0, zi = eta;
+    double fr = 0.0, fi = 0.0;
+    for (i64 i = i64(n) - 1; i >= 0; --i) {
+        double dr = zr - alphas[i], di = zi;
+        if (i != i64(n) - 1) {
+            const double b2 = betas[i] * betas[i];
+            dr -= b2 * fr;
+            di -= b2 * fi;
+        }
+        // f = 1 / (dr + i di)
+        const double den = dr * dr + di * di;
+        fr = dr / den;
+        fi = -di / den;
+    }
+    const double n2 = norm_phi0 * norm_phi0;
+    return -1.0 / M_PI * (n2 * fi);
+}
+
+// ---------------------------------------------------------------------------
+// FTLM (reference ftlm.cpp:122-170) over sectors and a beta grid.
+// ---------------------------------------------------------------------------
+std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
+                                  const std::vector<double>& betas, int n_random, int n_steps,
+                                  int mode) {
+    const int nsec = int(hs.size()), nb = int(betas.size());
+    std::vector<FTLMPoint> out(size_t(nsec) * nb);
+    for (int s = 0; s < nsec; ++s)
+        for (int b = 0; b < nb; ++b) out[size_t(s) * nb + b] = FTLMPoint{betas[b], 0.0, 0.0, 0.0};
+    if (nsec == 0 || nb == 0) return out;
+    u64 dmax = 0;
+    for (auto* h : hs) dmax = std::max(dmax, h->dim());
+    if (dmax == 0) return out;
+    const int R = std::max(n_random, 0);
+
+    // The reference re-seeds std::mt19937(42) on every call, so sector s uses
+    // the first R*d_s complex draws of one common stream.
+    std::vector<cplx> stream(size_t(R) * dmax);
+    if (R > 0) {
+        NormalStream ns(42);
+        ns.fill(stream.data(), stream.size());
+    }
+
+    // per (sector, vector) tridiagonal results
+    struct Sample {
+        int sec;
+        double nrm;
+        std::vector<double> a, b;
+    };
+    std::vector<Sample> samples;
+    hipStream_t st = hs[0]->stream;
+    for (auto* h : hs) QK_HIP(hipStreamSynchronize(h->stream));
+
+    if (mode == 0) {
+        std::vector<std::pair<const Hamiltonian*, int>> g;
+        std::vector<int> secs;
+        for (int s = 0; s < nsec; ++s)
+            if (hs[s]->dim() > 0 && R > 0) {
+                g.push_back({hs[s], R});
+                secs.push_back(s);
+            }
+        if (!g.empty()) {
+            BlockRec rec;
+            rec.setup(g, st);
+            for (size_t gi = 0; gi < g.size(); ++gi)
+                QK_HIP(hipMemcpyAsync(rec.bufU.get<cplx>() + rec.grps[gi].off, stream.data(),
+                                      sizeof(cplx) * g[gi].first->dim() * R,
+                                      hipMemcpyHostToDevice, st));
+            const std::vector<double> nrm = rec.start(0.0);
+            std::vector<double> ah, bh;
+            rec.run(n_steps, ah, bh);
+            int j = 0;
+            for (size_t gi = 0; gi < g.size(); ++gi)
+                for (int r = 0; r < R; ++r, ++j) {
+                    Sample smp{secs[gi], nrm[j], {}, {}};
+                    extract(ah, bh, n_steps, rec.nsamp, j, smp.a, smp.b);
+                    samples.push_back(std::move(smp));
+                }
+        }
+    } else {
+        for (int s = 0; s < nsec; ++s) {
+            const u64 d = hs[s]->dim();
+            if (d == 0) continue;
+            BlockRec rec;
+            rec.setup({{hs[s], 1}}, st);
+            for (int r = 0; r < R; ++r) {
+                QK_HIP(hipMemcpyAsync(rec.bufU.get(), stream.data() + size_t(r) * d,
+                                      sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+                const std::vector<double> nrm = rec.start(0.0);
+                std::vector<double> ah, bh;
+                rec.run(n_steps, ah, bh);
+                Sample smp{s, nrm[0], {}, {}};
+                extract(ah, bh, n_steps, 1, 0, smp.a, smp.b);
+                samples.push_back(std::move(smp));
+            }
+        }
+    }
+
+    // small eigensolves, then Boltzmann weights for every beta (:151-167)
+    std::vector<std::vector<double>> eig(samples.size()), z0(samples.size());
+    parallel_for(int(samples.size()), [&](int i) {
+        const int n = int(samples[i].a.size());
+        eig[i].resize(n);
+        z0[i].resize(n);
+        tridiag_ql(samples[i].a.data(), samples[i].b.data(), n, eig[i].data(), z0[i].data(), nullptr);
+    });
+    std::vector<double> Zs(size_t(nsec) * nb, 0.0), Es(Zs.size(), 0.0), E2s(Zs.size(), 0.0);
+    parallel_for(nb, [&](int b) {
+        const double beta = betas[b];
+        for (size_t i = 0; i < samples.size(); ++i) {
+            const int s = samples[i].sec;
+            const double n2 = samples[i].nrm * samples[i].nrm;
+            double& Z = Zs[size_t(s) * nb + b];
+            double& E = Es[size_t(s) * nb + b];
+            double& E2 = E2s[size_t(s) * nb + b];
+            for (size_t k = 0; k < eig[i].size(); ++k) {
+                const double w = n2 * z0[i][k] * z0[i][k] * std::exp(-beta * eig[i][k]);
+                Z += w;
+                E += eig[i][k] * w;
+                E2 += eig[i][k] * eig[i][k] * w;
+            }
+        }
+    });
+    for (int s = 0; s < nsec; ++s) {
+        if (hs[s]->dim() == 0) continue;
+        for (int b = 0; b < nb; ++b) {
+            const size_t o = size_t(s) * nb + b;
+            const double beta = betas[b];
+            const double Z = Zs[o] / R, E = Es[o] / R, E2 = E2s[o] / R;
+            out[o] = FTLMPoint{beta, Z, E / Z, (beta * beta) * (E2 / Z - (E / Z) * (E / Z))};
+        }
+    }
+    return out;
+}
+
+}  // namespace qk

@@ -1,0 +1,68 @@
+// Krylov solvers on one GPU (FP64 vectors in HBM, device-resident scalars,
+// host small eigensolves).  Semantics follow the reference solvers
+// (src/solvers/lanczos.cpp, davidson.cpp, ftlm.cpp, dynamics.cpp).
+#pragma once
+
+#include "qk_hamiltonian.hpp"
+
+#include <vector>
+
+namespace qk {
+
+struct LanczosResult {
+    double energy = 0.0;
+    int iterations = 0;
+    bool converged = false;
+    std::vector<cplx> eigenvector;   // host, empty unless requested
+};
+
+// start: host vector of dim() entries, or null for the reference start
+// (std::mt19937(1234), uniform(-1,1)).
+LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol,
+                                   const cplx* start, bool want_vector);
+
+struct DavidsonResult {
+    std::vector<double> eigenvalues;
+    std::vector<std::vector<cplx>> eigenvectors;   // host, empty unless requested
+    int iterations = 0;
+    bool converged = false;
+    std::vector<double> residuals;                 // final residual norms
+};
+
+// Preconditioner hook: t = M(r; lambda).  When null, Jacobi (reference).
+struct Preconditioner {
+    virtual ~Preconditioner() = default;
+    // r, t: device vectors of dim entries; diag: device complex diagonal.
+    virtual void apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda,
+                       cplx* t, hipStream_t st) = 0;
+};
+
+DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace, double tol,
+                               bool want_vectors, Preconditioner* pre = nullptr);
+
+struct CFResult {
+    std::vector<double> alphas, betas;
+    double norm_phi0 = 0.0;
+};
+
+// nblock start vectors (host, vector b at offset b*dim) as one block.
+std::vector<CFResult> continued_fraction_block(const Hamiltonian& H, const cplx* phi0, int nblock,
+                                               int n_iter);
+// Same, with the start vectors already on the device (ld = dim).
+std::vector<CFResult> continued_fraction_block_dev(const Hamiltonian& H, const cplx* phi0_dev,
+                                                   int nblock, int n_iter);
+
+double spectral_function(const double* alphas, const double* betas, size_t n, double norm_phi0,
+                         double omega, double E0, double eta);
+
+struct FTLMPoint {
+    double beta, Z, E, Cv;
+};
+
+// ftlm over sectors and a beta grid; out[s*nbeta + b].  mode 0: all
+// (sector, vector) pairs advanced together; mode 1: one vector at a time.
+std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
+                                  const std::vector<double>& betas, int n_random, int n_steps,
+                                  int mode);
+
+}  // namespace qk

@@ -1,0 +1,512 @@
+// FP64 Krylov vector kernels (see qk_vec.hpp).  256-thread workgroups,
+// wave64 butterfly reductions (__shfl_xor), then a fixed-order LDS reduction
+// over the 4 waves.  Reference operations replaced: vector_ops.hpp:14-137
+// (dot, norm, scal, axpy), the MGS loops (lanczos.cpp:149-151,
+// davidson.cpp:213-215), Ritz/residual axpy chains (lanczos.cpp:194-196,
+// davidson.cpp:161-165, 187, 202-205) and davidson_precond (:171-175).
+#include "qk_vec.hpp"
+
+#include <algorithm>
+
+namespace qk {
+
+namespace {
+
+constexpr int kT = 256;                 // threads per workgroup
+constexpr u32 kCH = Segments::kChunk;   // elements per reduction chunk
+constexpr int kPer = kCH / kT;          // elements per thread per chunk
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the workgroup; result valid in thread 0.  Fixed order.
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+    return r;
+}
+
+__device__ __forceinline__ cplx cmuld(cplx a, cplx b) {
+    return cplx{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// ---------------- single-vector reductions ----------------
+__global__ void __launch_bounds__(kT) k_dot_part(const cplx* __restrict__ a,
+                                                 const cplx* __restrict__ b, u64 n,
+                                                 cplx* __restrict__ part) {
+    __shared__ double sh[4];
+    const u64 start = u64(blockIdx.x) * kCH;
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < kPer; ++k) {
+        const u64 i = start + u64(k) * kT + threadIdx.x;
+        if (i < n) {
+            const cplx x = a[i], y = b[i];
+            re += x.re * y.re + x.im * y.im;
+            im += x.re * y.im - x.im * y.re;
+        }
+    }
+    re = block_sum(re, sh);
+    im = block_sum(im, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = cplx{re, im};
+}
+
+__global__ void __launch_bounds__(kT) k_norm_part(const cplx* __restrict__ a, u64 n,
+                                                  double* __restrict__ part) {
+    __shared__ double sh[4];
+    const u64 start = u64(blockIdx.x) * kCH;
+    double s = 0.0;
+    for (int k = 0; k < kPer; ++k) {
+        const u64 i = start + u64(k) * kT + threadIdx.x;
+        if (i < n) {
+            const cplx x = a[i];
+            s += x.re * x.re + x.im * x.im;
+        }
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[j] = sum_{b < nb} part[j*nb + b]   (one workgroup per j)
+__global__ void __launch_bounds__(kT) k_finish_c(const cplx* __restrict__ part, u32 nb,
+                                                 cplx* __restrict__ out) {
+    __shared__ double sh[4];
+    const cplx* p = part + u64(blockIdx.x) * nb;
+    double re = 0.0, im = 0.0;
+    for (u32 b = threadIdx.x; b < nb; b += kT) {
+        re += p[b].re;
+        im += p[b].im;
+    }
+    re = block_sum(re, sh);
+    im = block_sum(im, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = cplx{re, im};
+}
+
+__global__ void __launch_bounds__(kT) k_finish_r(const double* __restrict__ part, u32 nb,
+                                                 double* __restrict__ out) {
+    __shared__ double sh[4];
+    const double* p = part + u64(blockIdx.x) * nb;
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < nb; b += kT) s += p[b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// ---------------- elementwise ----------------
+__global__ void k_scale(cplx* __restrict__ x, u64 n, cplx s) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i < n) x[i] = cmuld(x[i], s);
+}
+
+__global__ void k_scale_to(const cplx* __restrict__ x, cplx* __restrict__ y, u64 n,
+                           const double* __restrict__ s) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    const double c = s[0];
+    if (i < n) y[i] = cplx{x[i].re * c, x[i].im * c};
+}
+
+__global__ void k_axpy(cplx a, const cplx* __restrict__ x, cplx* __restrict__ y, u64 n) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i < n) {
+        const cplx p = cmuld(a, x[i]);
+        y[i] = cplx{y[i].re + p.re, y[i].im + p.im};
+    }
+}
+
+__global__ void k_lanczos_sub(cplx* __restrict__ w, const cplx* __restrict__ v,
+                              const cplx* __restrict__ u, u64 n, const double* __restrict__ alpha,
+                              const double* __restrict__ beta) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i >= n) return;
+    const double a = alpha[0];
+    cplx r = w[i];
+    const cplx vv = v[i];
+    r.re -= a * vv.re;
+    r.im -= a * vv.im;
+    if (u) {
+        const double b = beta[0];
+        const cplx uu = u[i];
+        r.re -= b * uu.re;
+        r.im -= b * uu.im;
+    }
+    w[i] = r;
+}
+
+// h partials: part[j*nb + blk] = sum_{rows of blk} conj(V_j) w
+__global__ void __launch_bounds__(kT) k_mdot_part(const cplx* const* __restrict__ V, int k,
+                                                  const cplx* __restrict__ w, u64 n,
+                                                  cplx* __restrict__ part, u32 nb) {
+    __shared__ double sh[4];
+    const u64 start = u64(blockIdx.x) * kCH;
+    cplx wl[kPer];
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const u64 i = start + u64(e) * kT + threadIdx.x;
+        wl[e] = (i < n) ? w[i] : cplx{0.0, 0.0};
+    }
+    for (int j = 0; j < k; ++j) {
+        const cplx* vj = V[j];
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) {
+            const u64 i = start + u64(e) * kT + threadIdx.x;
+            if (i < n) {
+                const cplx x = vj[i];
+                re += x.re * wl[e].re + x.im * wl[e].im;
+                im += x.re * wl[e].im - x.im * wl[e].re;
+            }
+        }
+        re = block_sum(re, sh);
+        im = block_sum(im, sh);
+        if (threadIdx.x == 0) part[u64(j) * nb + blockIdx.x] = cplx{re, im};
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_msub(const cplx* const* __restrict__ V, int k,
+                                             const cplx* __restrict__ h, cplx* __restrict__ w,
+                                             u64 n) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i >= n) return;
+    cplx r = w[i];
+    for (int j = 0; j < k; ++j) {
+        const cplx x = V[j][i];
+        const cplx c = h[j];
+        r.re -= c.re * x.re - c.im * x.im;
+        r.im -= c.re * x.im + c.im * x.re;
+    }
+    w[i] = r;
+}
+
+template <int NO>
+__global__ void __launch_bounds__(kT) k_lincomb(const cplx* const* __restrict__ V, int m,
+                                                const cplx* __restrict__ c, int nout,
+                                                cplx* __restrict__ out, u64 ldo, u64 n) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i >= n) return;
+    cplx acc[NO];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) acc[o] = cplx{0.0, 0.0};
+    for (int j = 0; j < m; ++j) {
+        const cplx x = V[j][i];
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            if (o < nout) {
+                const cplx cc = c[o * m + j];
+                acc[o].re += cc.re * x.re - cc.im * x.im;
+                acc[o].im += cc.re * x.im + cc.im * x.re;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o)
+        if (o < nout) out[u64(o) * ldo + i] = acc[o];
+}
+
+__global__ void k_jacobi(const cplx* __restrict__ r, const cplx* __restrict__ d, double lambda,
+                         cplx* __restrict__ t, u64 n) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i >= n) return;
+    double dre = d[i].re - lambda, dim = d[i].im;
+    if (sqrt(dre * dre + dim * dim) < 1e-10) {
+        dre = (dre >= 0.0) ? 1e-10 : -1e-10;
+        dim = 0.0;
+    }
+    const cplx x = r[i];
+    if (dim == 0.0) {
+        t[i] = cplx{x.re / dre, x.im / dre};
+    } else {
+        const double den = dre * dre + dim * dim;
+        t[i] = cplx{(x.re * dre + x.im * dim) / den, (x.im * dre - x.re * dim) / den};
+    }
+}
+
+// ---------------- segmented (batched) recurrences ----------------
+__global__ void __launch_bounds__(kT) k_seg_alpha_part(
+    const u64* __restrict__ coff, const u32* __restrict__ clen, const u32* __restrict__ csamp,
+    const cplx* __restrict__ U, const cplx* __restrict__ Y, const int* __restrict__ alive,
+    double* __restrict__ part) {
+    __shared__ double sh[4];
+    const u32 c = blockIdx.x;
+    const u32 j = csamp[c];
+    if (!alive[j]) {
+        if (threadIdx.x == 0) part[c] = 0.0;
+        return;
+    }
+    const u64 off = coff[c];
+    const u32 len = clen[c];
+    double s = 0.0;
+    for (u32 e = threadIdx.x; e < len; e += kT) {
+        const cplx u = U[off + e], y = Y[off + e];
+        s += u.re * y.re + u.im * y.im;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[c] = s;
+}
+
+__global__ void __launch_bounds__(kT) k_seg_alpha_fin(const u32* __restrict__ first,
+                                                      const u32* __restrict__ count,
+                                                      const double* __restrict__ part,
+                                                      const double* __restrict__ cs,
+                                                      double* __restrict__ alpha,
+                                                      const int* __restrict__ alive) {
+    __shared__ double sh[4];
+    const u32 j = blockIdx.x;
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < count[j]; b += kT) s += part[first[j] + b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0 && alive[j]) alpha[j] = cs[j] * cs[j] * s;
+}
+
+__global__ void __launch_bounds__(kT) k_seg_update_part(
+    const u64* __restrict__ coff, const u32* __restrict__ clen, const u32* __restrict__ csamp,
+    cplx* __restrict__ Y, const cplx* __restrict__ U, const cplx* __restrict__ P,
+    const double* __restrict__ cs, const double* __restrict__ cps,
+    const double* __restrict__ bprev, const double* __restrict__ alpha,
+    const int* __restrict__ alive, int first_step, double* __restrict__ part) {
+    __shared__ double sh[4];
+    const u32 c = blockIdx.x;
+    const u32 j = csamp[c];
+    if (!alive[j]) {
+        if (threadIdx.x == 0) part[c] = 0.0;
+        return;
+    }
+    const u64 off = coff[c];
+    const u32 len = clen[c];
+    const double cc = cs[j];
+    const double ac = alpha[j] * cc;
+    const double bp = first_step ? 0.0 : bprev[j] * cps[j];
+    double s = 0.0;
+    for (u32 e = threadIdx.x; e < len; e += kT) {
+        const cplx y = Y[off + e], u = U[off + e];
+        cplx w{cc * y.re - ac * u.re, cc * y.im - ac * u.im};
+        if (!first_step) {
+            const cplx p = P[off + e];
+            w.re -= bp * p.re;
+            w.im -= bp * p.im;
+        }
+        Y[off + e] = w;
+        s += w.re * w.re + w.im * w.im;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[c] = s;
+}
+
+__global__ void __launch_bounds__(kT) k_seg_update_fin(
+    const u32* __restrict__ first, const u32* __restrict__ count, const double* __restrict__ part,
+    double* __restrict__ cs, double* __restrict__ cps, double* __restrict__ bprev,
+    double* __restrict__ beta, int* __restrict__ alive) {
+    __shared__ double sh[4];
+    const u32 j = blockIdx.x;
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < count[j]; b += kT) s += part[first[j] + b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0 && alive[j]) {
+        const double bt = sqrt(s);
+        beta[j] = bt;
+        cps[j] = cs[j];
+        bprev[j] = bt;
+        if (bt < 1e-15) {
+            alive[j] = 0;
+            cs[j] = 0.0;
+        } else {
+            cs[j] = 1.0 / bt;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kT) k_seg_norm_part(const u64* __restrict__ coff,
+                                                      const u32* __restrict__ clen,
+                                                      const cplx* __restrict__ X,
+                                                      double* __restrict__ part) {
+    __shared__ double sh[4];
+    const u32 c = blockIdx.x;
+    const u64 off = coff[c];
+    const u32 len = clen[c];
+    double s = 0.0;
+    for (u32 e = threadIdx.x; e < len; e += kT) {
+        const cplx x = X[off + e];
+        s += x.re * x.re + x.im * x.im;
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[c] = s;
+}
+
+__global__ void __launch_bounds__(kT) k_seg_norm_fin(const u32* __restrict__ first,
+                                                     const u32* __restrict__ count,
+                                                     const double* __restrict__ part,
+                                                     double* __restrict__ nrm) {
+    __shared__ double sh[4];
+    const u32 j = blockIdx.x;
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < count[j]; b += kT) s += part[first[j] + b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) nrm[j] = sqrt(s);
+}
+
+inline u32 nblk(u64 n) { return static_cast<u32>((n + kCH - 1) / kCH); }
+inline u32 ngrid(u64 n) { return static_cast<u32>((n + kT - 1) / kT); }
+
+}  // namespace
+
+void Segments::build(const std::vector<u64>& lens, hipStream_t st) {
+    nsamp = static_cast<int>(lens.size());
+    len = lens;
+    base.assign(lens.size(), 0);
+    total = 0;
+    std::vector<u64> coff;
+    std::vector<u32> clen, csamp, first(nsamp), count(nsamp);
+    for (int j = 0; j < nsamp; ++j) {
+        base[j] = total;
+        first[j] = static_cast<u32>(coff.size());
+        for (u64 o = 0; o < lens[j]; o += kChunk) {
+            coff.push_back(total + o);
+            clen.push_back(static_cast<u32>(std::min<u64>(kChunk, lens[j] - o)));
+            csamp.push_back(static_cast<u32>(j));
+        }
+        count[j] = static_cast<u32>(coff.size()) - first[j];
+        total += lens[j];
+    }
+    nchunks = static_cast<u32>(coff.size());
+    upload(d_chunk_off, coff.data(), coff.size(), st);
+    upload(d_chunk_len, clen.data(), clen.size(), st);
+    upload(d_chunk_sample, csamp.data(), csamp.size(), st);
+    upload(d_first, first.data(), first.size(), st);
+    upload(d_count, count.data(), count.size(), st);
+    partials.resize(sizeof(double) * (nchunks ? nchunks : 1));
+    QK_HIP(hipStreamSynchronize(st));
+}
+
+namespace vec {
+
+void dot(const cplx* a, const cplx* b, u64 n, cplx* out, DevBuf& scratch, hipStream_t st) {
+    const u32 nb = std::max<u32>(1, nblk(n));
+    scratch.resize(sizeof(cplx) * nb);
+    hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(kT), 0, st, a, b, n, scratch.get<cplx>());
+    hipLaunchKernelGGL(k_finish_c, dim3(1), dim3(kT), 0, st, scratch.get<cplx>(), nb, out);
+    QK_HIP(hipGetLastError());
+}
+
+void norm2(const cplx* a, u64 n, double* out, DevBuf& scratch, hipStream_t st) {
+    const u32 nb = std::max<u32>(1, nblk(n));
+    scratch.resize(sizeof(double) * nb);
+    hipLaunchKernelGGL(k_norm_part, dim3(nb), dim3(kT), 0, st, a, n, scratch.get<double>());
+    hipLaunchKernelGGL(k_finish_r, dim3(1), dim3(kT), 0, st, scratch.get<double>(), nb, out);
+    QK_HIP(hipGetLastError());
+}
+
+void scale(cplx* x, u64 n, cplx s, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_scale, dim3(ngrid(n)), dim3(kT), 0, st, x, n, s);
+    QK_HIP(hipGetLastError());
+}
+
+void scale_to(const cplx* x, cplx* y, u64 n, const double* s_dev, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_scale_to, dim3(ngrid(n)), dim3(kT), 0, st, x, y, n, s_dev);
+    QK_HIP(hipGetLastError());
+}
+
+void axpy(cplx a, const cplx* x, cplx* y, u64 n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_axpy, dim3(ngrid(n)), dim3(kT), 0, st, a, x, y, n);
+    QK_HIP(hipGetLastError());
+}
+
+void lanczos_sub(cplx* w, const cplx* v, const cplx* u, u64 n, const double* alpha,
+                 const double* beta, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_lanczos_sub, dim3(ngrid(n)), dim3(kT), 0, st, w, v, u, n, alpha, beta);
+    QK_HIP(hipGetLastError());
+}
+
+void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& scratch,
+          hipStream_t st) {
+    if (k <= 0) return;
+    const u32 nb = std::max<u32>(1, nblk(n));
+    scratch.resize(sizeof(cplx) * nb * size_t(k));
+    hipLaunchKernelGGL(k_mdot_part, dim3(nb), dim3(kT), 0, st, V, k, w, n, scratch.get<cplx>(),
+                       nb);
+    hipLaunchKernelGGL(k_finish_c, dim3(k), dim3(kT), 0, st, scratch.get<cplx>(), nb, h);
+    QK_HIP(hipGetLastError());
+}
+
+void msub(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, hipStream_t st) {
+    if (k <= 0 || !n) return;
+    hipLaunchKernelGGL(k_msub, dim3(ngrid(n)), dim3(kT), 0, st, V, k, h, w, n);
+    QK_HIP(hipGetLastError());
+}
+
+void lincomb(const cplx* const* V, int k, const cplx* c, cplx* out, u64 n, hipStream_t st) {
+    lincomb_many(V, k, c, 1, out, n, n, st);
+}
+
+void lincomb_many(const cplx* const* V, int m, const cplx* c, int nout, cplx* out, u64 ldo,
+                  u64 n, hipStream_t st) {
+    if (!n || nout <= 0) return;
+    for (int o0 = 0; o0 < nout; o0 += 8) {
+        const int no = std::min(8, nout - o0);
+        const cplx* cc = c + size_t(o0) * m;
+        cplx* oo = out + u64(o0) * ldo;
+        if (no == 1)
+            hipLaunchKernelGGL(k_lincomb<1>, dim3(ngrid(n)), dim3(kT), 0, st, V, m, cc, no, oo,
+                               ldo, n);
+        else if (no <= 4)
+            hipLaunchKernelGGL(k_lincomb<4>, dim3(ngrid(n)), dim3(kT), 0, st, V, m, cc, no, oo,
+                               ldo, n);
+        else
+            hipLaunchKernelGGL(k_lincomb<8>, dim3(ngrid(n)), dim3(kT), 0, st, V, m, cc, no, oo,
+                               ldo, n);
+    }
+    QK_HIP(hipGetLastError());
+}
+
+void jacobi_precond(const cplx* r, const cplx* d, double lambda, cplx* t, u64 n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_jacobi, dim3(ngrid(n)), dim3(kT), 0, st, r, d, lambda, t, n);
+    QK_HIP(hipGetLastError());
+}
+
+void seg_alpha(Segments& seg, const cplx* U, const cplx* Y, const double* c, double* alpha_out,
+               const int* alive, hipStream_t st) {
+    if (!seg.nchunks) return;
+    hipLaunchKernelGGL(k_seg_alpha_part, dim3(seg.nchunks), dim3(kT), 0, st,
+                       seg.d_chunk_off.get<u64>(), seg.d_chunk_len.get<u32>(),
+                       seg.d_chunk_sample.get<u32>(), U, Y, alive, seg.partials.get<double>());
+    hipLaunchKernelGGL(k_seg_alpha_fin, dim3(seg.nsamp), dim3(kT), 0, st, seg.d_first.get<u32>(),
+                       seg.d_count.get<u32>(), seg.partials.get<double>(), c, alpha_out, alive);
+    QK_HIP(hipGetLastError());
+}
+
+void seg_update(Segments& seg, cplx* Y, const cplx* U, const cplx* P, double* c, double* cp,
+                double* beta_prev, const double* alpha, double* beta_out, int* alive,
+                int first_step, hipStream_t st) {
+    if (!seg.nchunks) return;
+    hipLaunchKernelGGL(k_seg_update_part, dim3(seg.nchunks), dim3(kT), 0, st,
+                       seg.d_chunk_off.get<u64>(), seg.d_chunk_len.get<u32>(),
+                       seg.d_chunk_sample.get<u32>(), Y, U, P, c, cp, beta_prev, alpha, alive,
+                       first_step, seg.partials.get<double>());
+    hipLaunchKernelGGL(k_seg_update_fin, dim3(seg.nsamp), dim3(kT), 0, st,
+                       seg.d_first.get<u32>(), seg.d_count.get<u32>(),
+                       seg.partials.get<double>(), c, cp, beta_prev, beta_out, alive);
+    QK_HIP(hipGetLastError());
+}
+
+void seg_norm(Segments& seg, const cplx* X, double* nrm_out, hipStream_t st) {
+    if (!seg.nchunks) return;
+    hipLaunchKernelGGL(k_seg_norm_part, dim3(seg.nchunks), dim3(kT), 0, st,
+                       seg.d_chunk_off.get<u64>(), seg.d_chunk_len.get<u32>(), X,
+                       seg.partials.get<double>());
+    hipLaunchKernelGGL(k_seg_norm_fin, dim3(seg.nsamp), dim3(kT), 0, st, seg.d_first.get<u32>(),
+                       seg.d_count.get<u32>(), seg.partials.get<double>(), nrm_out);
+    QK_HIP(hipGetLastError());
+}
+
+}  // namespace vec
+}  // namespace qk

@@ -1,0 +1,77 @@
+// FP64 Krylov vector kernels for gfx950.  Every reduction is two-stage with
+// a fixed decomposition (per-block partials, then one block per output in a
+// fixed tree), so results are deterministic run to run.  Scalars that feed
+// later kernels stay on the device (pointer arguments), so a sequence of
+// steps needs no host synchronisation.
+#pragma once
+
+#include "qk_device.hpp"
+
+#include <vector>
+
+namespace qk {
+
+// A batch of independent vectors laid out back to back in one flat buffer:
+// sample j occupies [base[j], base[j] + len[j]).  Reductions are split into
+// chunks of at most kChunk elements; chunk c belongs to sample chunk_sample[c]
+// and the chunks of one sample are contiguous.
+struct Segments {
+    static constexpr u32 kChunk = 4096;
+    int nsamp = 0;
+    u64 total = 0;
+    std::vector<u64> base, len;
+    // device
+    DevBuf d_chunk_off, d_chunk_len, d_chunk_sample, d_first, d_count;
+    u32 nchunks = 0;
+    DevBuf partials;   // nchunks cplx
+
+    void build(const std::vector<u64>& lens, hipStream_t st);
+};
+
+namespace vec {
+
+// out[0] = sum conj(a) b over n elements (device scalar).
+void dot(const cplx* a, const cplx* b, u64 n, cplx* out, DevBuf& scratch, hipStream_t st);
+// out[0] = sum |a|^2.
+void norm2(const cplx* a, u64 n, double* out, DevBuf& scratch, hipStream_t st);
+// x *= s (host scalar)
+void scale(cplx* x, u64 n, cplx s, hipStream_t st);
+// y = s * x (s on device, real)
+void scale_to(const cplx* x, cplx* y, u64 n, const double* s_dev, hipStream_t st);
+// y += a x  (host scalar)
+void axpy(cplx a, const cplx* x, cplx* y, u64 n, hipStream_t st);
+// w -= alpha v + beta u   (alpha, beta real device scalars; u may be null)
+void lanczos_sub(cplx* w, const cplx* v, const cplx* u, u64 n, const double* alpha,
+                 const double* beta, hipStream_t st);
+// h[j] = sum conj(V_j) w, j < k; V: device array of k vector pointers.
+void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& scratch,
+          hipStream_t st);
+// w -= sum_j V_j h[j]
+void msub(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, hipStream_t st);
+// out = sum_j c[j] V_j  (c on device)
+void lincomb(const cplx* const* V, int k, const cplx* c, cplx* out, u64 n, hipStream_t st);
+// out_k = sum_j c[k*m + j] V_j for k < nout (c on device), out_k at out + k*ldo
+void lincomb_many(const cplx* const* V, int m, const cplx* c, int nout, cplx* out, u64 ldo,
+                  u64 n, hipStream_t st);
+// t = r / (d - lambda) with |d - lambda| < 1e-10 clamped to +-1e-10
+// (reference src/solvers/davidson.cpp:171-175); d: complex diagonal.
+void jacobi_precond(const cplx* r, const cplx* d, double lambda, cplx* t, u64 n, hipStream_t st);
+
+// ---- batched recurrences over Segments (FTLM / continued fractions) ----
+// Buffers U, Y, P are flat (seg.total elements).  Per-sample device arrays:
+// c (scale of the current vector), cp (scale of the previous vector),
+// beta_prev, alpha_out, beta_out, alive (1 while the recurrence runs).
+//
+// alpha_j = c_j^2 Re<U_j, Y_j>
+void seg_alpha(Segments& seg, const cplx* U, const cplx* Y, const double* c, double* alpha_out,
+               const int* alive, hipStream_t st);
+// W = c Y - alpha c U - beta_prev cp P (written into Y), beta_j = ||W_j||;
+// then cp <- c, c <- 1/beta (0 and alive = 0 when beta < 1e-15), beta_prev <- beta.
+void seg_update(Segments& seg, cplx* Y, const cplx* U, const cplx* P, double* c, double* cp,
+                double* beta_prev, const double* alpha, double* beta_out, int* alive,
+                int first_step, hipStream_t st);
+// nrm_j = ||X_j||  (X flat)
+void seg_norm(Segments& seg, const cplx* X, double* nrm_out, hipStream_t st);
+
+}  // namespace vec
+}  // namespace qk

@@ -1,0 +1,149 @@
+"""Krylov solvers on the GPU against the oracle (reference src/solvers/*)
+and closed forms.  Bars: ground-state energies and Ritz values within 1e-10;
+FTLM Z, E, Cv and S(q, omega) within 1e-8 (relative)."""
+import numpy as np
+import pytest
+
+import qkrylov_amd as qkm
+from models import build_pair, make_sector
+
+pytestmark = pytest.mark.gpu
+
+
+def test_lanczos_known_energies(qk, lat):
+    # 2-site full basis: singlet -0.75 (reference tests/test_lanczos_ritz.cpp:46)
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(2), qk.SpinHalfSite(),
+                                 lat.heisenberg_chain(2, periodic=False))
+    r = qk.lanczos_ground_state(H, 200, 1e-12)
+    assert abs(r.energy + 0.75) < 1e-12
+    assert np.linalg.norm(H.apply(r.eigenvector) - r.energy * r.eigenvector) < 1e-10
+    # 4-site open chain, Sz=0 (tests/test_workflow.cpp:46, test_c_api.cpp:96)
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(4, sz=0), qk.SpinHalfSite(),
+                                 lat.heisenberg_chain(4, periodic=False))
+    r = qk.lanczos_ground_state(H, 200, 1e-12)
+    assert abs(r.energy - (-0.75 - np.sqrt(3) / 2 * 1.0)) < 1e-10   # -1.6160254038
+    assert abs(r.energy + 1.6160254038) < 1e-9
+    # 4-site periodic chain, full basis (bindings/julia/test/runtests.jl:122,165)
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(4), qk.SpinHalfSite(), lat.heisenberg_chain(4))
+    r = qk.lanczos_ground_state(H, 50, 1e-12)
+    assert abs(r.energy + 2.0) < 1e-10 and r.converged and r.iterations > 0
+    # maxiter semantics (runtests.jl:172-174)
+    r2 = qk.lanczos_ground_state(H, 2, 1e-15)
+    assert r2.iterations == 2 and not r2.converged
+
+
+def test_lanczos_config1_matches_oracle(qk, orc, lat):
+    """N=16 periodic Heisenberg chain, Sz=0 (d = 12,870), reference start."""
+    print("[lanczos] N=16", flush=True)
+    os_ = lat.heisenberg_chain(16)
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(16, sz=0), qk.SpinHalfSite(), os_)
+    ob = orc.Basis("spin", 16, {"use_sz": True, "sz2": 0})
+    oh = orc.Ham(ob, "spin", os_)
+    r = qk.lanczos_ground_state(H, 200, 1e-12)
+    e_ref, it_ref, conv_ref, _ = oh.lanczos(200, 1e-12)
+    import scipy.sparse.linalg as sla
+    e_exact = sla.eigsh(oh.to_scipy(), k=1, which="SA", tol=1e-14)[0][0]
+    assert abs(r.energy - e_ref) < 1e-10
+    assert abs(r.energy - e_exact) < 1e-10
+    assert r.converged and conv_ref
+    res = np.linalg.norm(H.apply(r.eigenvector) - r.energy * r.eigenvector)
+    assert res < 1e-5
+    assert abs(np.linalg.norm(r.eigenvector) - 1) < 1e-12
+
+
+def test_lanczos_given_start_and_models(qk, orc):
+    for name in ("tj_4x3", "hubbard_chain8_half", "fermion_complex_11_n4"):
+        print(f"[lanczos] {name}", flush=True)
+        H, oh, _ = build_pair(orc, name)
+        d = H.dimension
+        start = orc.uniform_start(d, 99)
+        r = qk.lanczos_ground_state(H, 300, 1e-13, start=start)
+        e_ref = oh.lanczos(300, 1e-13, start=start)[0]
+        assert abs(r.energy - e_ref) < 1e-10, (name, r.energy, e_ref)
+
+
+def test_davidson_matches_oracle(qk, orc):
+    # reference tests/test_c_api.cpp:105-127 and runtests.jl:207-209 shapes.
+    # Residual tolerance 1e-8 (the reference default): with its 100-iteration
+    # cap the reference algorithm does not reach 1e-10 residuals on these
+    # degenerate spectra, while the eigenvalues are exact to ~1e-14 already.
+    for name, k in (("heis_chain10_sz0", 4), ("tj_3x2", 3), ("j1j2_4x3_sz0", 2)):
+        print(f"[davidson] {name}", flush=True)
+        H, oh, _ = build_pair(orc, name)
+        r = qk.davidson_lowest(H, k, 20, 1e-8)
+        ev_ref, _, it_ref, conv_ref = oh.davidson(k, 20, 1e-8)
+        dense = np.linalg.eigvalsh(oh.to_scipy().toarray())[:k]
+        assert r.converged and conv_ref
+        assert np.abs(r.eigenvalues - dense).max() < 1e-10
+        assert np.abs(r.eigenvalues - np.asarray(ev_ref)).max() < 1e-10
+        for lam, v in zip(r.eigenvalues, r.eigenvectors):
+            assert np.linalg.norm(H.apply(v) - lam * v) < 1e-7
+
+
+def test_davidson_periodic4(qk, lat):
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(4), qk.SpinHalfSite(), lat.heisenberg_chain(4))
+    r = qk.davidson_lowest(H, 2, 10, 1e-6)
+    assert len(r.eigenvalues) == 2 and abs(r.eigenvalues[0] + 2.0) < 1e-5
+    assert len(r.eigenvectors) == 2 and len(r.eigenvectors[0]) == 16
+
+
+def test_continued_fraction_matches_oracle(qk, orc):
+    H, oh, _ = build_pair(orc, "heis_chain10_sz0")
+    d = H.dimension
+    rng = np.random.default_rng(4)
+    phis = rng.standard_normal((3, d)) + 1j * rng.standard_normal((3, d))
+    block = qk.continued_fraction_block(H, phis, 30)
+    e0 = qk.lanczos_ground_state(H, 200, 1e-12).energy
+    omegas = np.linspace(0.0, 5.0, 41)
+    for j in range(3):
+        a_ref, b_ref, n_ref = oh.cf(phis[j], 30)
+        single = qk.continued_fraction_coeffs(H, phis[j], 30)
+        got = block[j]
+        assert len(got.alphas) == len(a_ref) and len(got.betas) == len(b_ref)
+        assert np.array_equal(got.alphas, single.alphas)
+        assert abs(got.norm_phi0 - n_ref) <= 1e-13 * n_ref
+        # the first steps agree tightly; orthogonality loss later amplifies
+        # rounding, so the derived spectrum is compared instead
+        assert np.abs(got.alphas[:10] - a_ref[:10]).max() < 1e-10
+        for w in omegas:
+            s = qk.evaluate_spectral_function(got.alphas, got.betas, got.norm_phi0, w, e0, 0.1)
+            s_ref = orc.spectral(a_ref, b_ref, n_ref, w, e0, 0.1)
+            assert abs(s - s_ref) <= 1e-8 * max(abs(s_ref), 1e-3)
+
+
+def test_continued_fraction_zero_start(qk, orc):
+    H, oh, _ = build_pair(orc, "heis_chain10_sz0")
+    r = qk.continued_fraction_coeffs(H, np.zeros(H.dimension, np.complex128), 10)
+    assert len(r.alphas) == 0 and r.norm_phi0 == 0.0
+
+
+def test_ftlm_matches_oracle(qk, orc, lat):
+    betas = np.array([0.1, 1.0, 4.0])
+    H, oh, _ = build_pair(orc, "heis_chain10_sz0")
+    print("[ftlm] chain10", flush=True)
+    Z, E, Cv = qk.ftlm_sweep([H], betas, 8, 20, batched=True)
+    Zp, Ep, Cvp = qk.ftlm_sweep([H], betas, 8, 20, batched=False)
+    Zr, Er, Cvr = oh.ftlm(betas, 8, 20)
+    for a, b in ((Z[0], Zr), (E[0], Er), (Cv[0], Cvr), (Zp[0], Zr), (Ep[0], Er), (Cvp[0], Cvr)):
+        assert np.all(np.abs(a - b) <= 1e-8 * np.abs(b)), (a, b)
+    one = qk.ftlm(H, 1.0, 8, 20)
+    assert abs(one.partition_function - Zr[1]) <= 1e-8 * Zr[1]
+
+
+def test_ftlm_sweep_all_sectors(qk, orc, lat):
+    """All Sz sectors of an N=10 XXZ chain in one batched run."""
+    n = 10
+    os_ = lat.xxz_chain(n, 1.0, 1.5)
+    hs, ohs = [], []
+    for k in range(n + 1):
+        sec = {"use_sz": True, "sz2": 2 * k - n}
+        hs.append(qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(n, sector=make_sector(sec)),
+                                           qk.SpinHalfSite(), os_))
+        ohs.append(orc.Ham(orc.Basis("spin", n, sec), "spin", os_))
+    betas = np.array([0.2, 2.0])
+    Z, E, Cv = qk.ftlm_sweep(hs, betas, 6, 15, batched=True)
+    for s in range(n + 1):
+        Zr, Er, Cvr = ohs[s].ftlm(betas, 6, 15)
+        assert np.all(np.abs(Z[s] - Zr) <= 1e-8 * np.abs(Zr))
+        assert np.all(np.abs(E[s] - Er) <= 1e-8 * np.maximum(np.abs(Er), 1e-12))
+        assert np.all(np.abs(Cv[s] - Cvr) <= 1e-8 * np.maximum(np.abs(Cvr), 1e-6))
