@@ -1,0 +1,226 @@
+"""Benchmark: y = H·x for the periodic N=28 spin-1/2 Heisenberg chain in the
+Sz=0 sector (d = 40,116,600, complex FP64), one GPU.
+
+A step is one application of H to one vector already resident in HBM.  The
+line also carries the block rate (28 vectors per launch), the roofline of
+the apply kernel, the batched-vs-plain FTLM comparison on the N=20 XXZ
+chain (all 21 Sz sectors, R=64, M=100, 200 temperatures) and the CPU
+baseline (the oracle's CSR gather SpMV on a bounded row sample, timed on
+the host cores).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-ftlm] [--skip-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+BYTES_PER_STATE = 32           # read x (16 B) + write y (16 B), complex FP64
+N_SITES = 28
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str, bsize: int):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a copy kernel), if a summary is committed."""
+    p = ROOT / "profiles" / "hx_heisenberg_n28_pmc.md"
+    if not p.exists():
+        return None
+    m = re.search(rf"^traffic_bytes_per_launch\[{kernel},b={bsize}\]:\s*([0-9.eE+]+)", p.read_text(),
+                  re.M)
+    return float(m.group(1)) if m else None
+
+
+def bench_apply(qk, torch, H, d, nvec, steps, warmup):
+    g = torch.Generator(device="cuda")
+    g.manual_seed(7)
+    x = torch.randn((nvec, d), dtype=torch.complex128, device="cuda", generator=g)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        H.apply_device(x.data_ptr(), y.data_ptr(), nvec, d, sp)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        H.apply_device(x.data_ptr(), y.data_ptr(), nvec, d, sp)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    # one launch per 8 vectors (block kernel); average launch duration
+    launches = (nvec + 7) // 8
+    del x, y
+    torch.cuda.empty_cache()
+    return ms, launches
+
+
+def cpu_baseline(lat, n_rows=4_000_000, reps=15):
+    """Oracle CSR gather SpMV (the reference algorithm, matrix_free_hamiltonian.cpp
+    :45-91 and :156-167) on rows [0, n_rows) of the same H, OpenMP on the host."""
+    from oracle import oracle as orc
+    threads = orc.lib().orc_threads()
+    os_ = lat.heisenberg_chain(N_SITES)
+    t0 = time.time()
+    ob = orc.Basis("spin", N_SITES, {"use_sz": True, "sz2": 0})
+    d = ob.size
+    part = orc.Ham(ob, "spin", os_, 0, n_rows)
+    setup = time.time() - t0
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    for _ in range(2):
+        part.apply(x)
+    ts = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        part.apply(x)
+        ts.append(time.perf_counter() - t)
+    med = float(np.median(ts))
+    rows_per_s = n_rows / med
+    return {
+        "value": rows_per_s / d,
+        "unit": "applies/s",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": (f"rows [0, {n_rows}) of the N=28 chain (d={d}); CSR built by the oracle in "
+                   f"{setup:.1f} s; median of {reps} gather SpMVs after 2 warm-ups = "
+                   f"{med * 1e3:.1f} ms, scaled to whole applies"),
+    }
+
+
+def bench_ftlm(qk, lat):
+    n = 20
+    os_ = lat.xxz_chain(n, 1.0, 1.5)
+    hs = []
+    for k in range(n + 1):
+        sec = qk.Sector().set_sz(2 * k - n)
+        hs.append(qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(n, sector=sec), qk.SpinHalfSite(), os_))
+    temps = 0.05 * 200.0 ** (np.arange(200) / 199.0)
+    betas = 1.0 / temps
+    R, M = 64, 100
+    nsamp = R * len(hs)
+    out = {}
+    for name, batched in (("batched", True), ("plain", False)):
+        t = time.perf_counter()
+        Z, E, Cv = qk.ftlm_sweep(hs, betas, R, M, batched=batched)
+        dt = time.perf_counter() - t
+        out[name] = (dt, Z, E, Cv)
+        log(f"ftlm {name}: {dt:.3f} s")
+    (tb, Zb, Eb, Cb), (tp, Zp, Ep, Cp) = out["batched"], out["plain"]
+
+    def reldiff(a, b):
+        return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+    # combine sectors (thermodynamics of the whole chain)
+    Zt = Zb.sum(0)
+    Et = (Zb * Eb).sum(0) / Zt
+    return {
+        "workload": "XXZ chain N=20 (Jxy=1, Jz=1.5), all 21 Sz sectors, R=64, M=100, 200 temperatures",
+        "samples": nsamp,
+        "samples_per_s_batched": nsamp / tb,
+        "samples_per_s_plain": nsamp / tp,
+        "speedup": tp / tb,
+        "max_rel_diff_Z": reldiff(Zb, Zp),
+        "max_rel_diff_E": reldiff(Eb, Ep),
+        "max_rel_diff_Cv": reldiff(Cb, Cp),
+        "E_per_site_T0.05": float(Et[0] / n),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--block", type=int, default=28)
+    ap.add_argument("--skip-ftlm", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--apply-only", action="store_true", help="time the b=1 apply only")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device 5 times (calibrates PMC byte counters)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30}))
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box of this build has one GPU and the H·x path "
+              "runs on one device; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import qkrylov_amd as qk
+    from qkrylov_amd import lattices as lat
+
+    log(f"device: {qk.device_info()}")
+    t = time.time()
+    basis = qk.SpinHalfBasis(N_SITES, sz=0)
+    H = qk.MatrixFreeHamiltonian(basis, qk.SpinHalfSite(), lat.heisenberg_chain(N_SITES))
+    d = H.dimension
+    log(f"setup N={N_SITES} d={d} in {time.time() - t:.2f} s  info={H.info()}")
+
+    ms, launches = bench_apply(qk, torch, H, d, 1, args.steps, args.warmup)
+    achieved = BYTES_PER_STATE * d / (ms * 1e-3) / 1e9
+    log(f"b=1: {ms:.4f} ms/apply  {1e3 / ms:.1f} applies/s  {achieved:.1f} GB/s")
+    extra = {}
+    if not args.apply_only and args.block > 1:
+        nb = args.block
+        msb, lb = bench_apply(qk, torch, H, d, nb, max(3, args.steps // 10), 1)
+        ach_b = BYTES_PER_STATE * d * nb / (msb * 1e-3) / 1e9
+        extra["block"] = {"vectors": nb, "ms_per_block": msb, "vector_applies_per_s": nb * 1e3 / msb,
+                          "achieved_GBs": ach_b, "frac": ach_b / HBM_PEAK_GBS}
+        log(f"b={nb}: {msb:.3f} ms/block  {nb * 1e3 / msb:.1f} vector-applies/s  {ach_b:.1f} GB/s")
+    if not args.apply_only and not args.skip_ftlm:
+        extra["ftlm"] = bench_ftlm(qk, lat)
+    cpu = None
+    if not args.apply_only and not args.skip_cpu:
+        cpu = cpu_baseline(lat)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "H·x applies/s (% HBM roofline); FTLM samples/s batched vs plain at 1e-10",
+        "value": 1e3 / ms,
+        "unit": "applies/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "periodic Heisenberg chain N=28, J=1, Sz=0 sector, d=40,116,600; "
+                               "y = H x, one complex FP64 vector (seeded normal) resident in HBM",
+                   "nsites": N_SITES, "dim": d, "vectors_per_step": 1, "parallelism": "1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles("k_apply", 1)},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

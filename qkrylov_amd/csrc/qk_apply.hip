@@ -28,14 +28,17 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <sstream>
+#include <string>
 
 namespace qk {
 
 namespace {
 
-constexpr int kWaves = 4;            // waves per workgroup
+constexpr int kWaves = 4;            // waves per workgroup (generic kernels)
 constexpr int kBlock = 64 * kWaves;  // threads per workgroup
+constexpr int kXWaves = 16;          // waves per workgroup (exchange kernel)
 constexpr int kXcd = 8;              // XCDs on MI355X
 
 __device__ __forceinline__ void cfma(cplx& acc, const cplx a, const cplx x) {
@@ -52,23 +55,55 @@ __device__ __forceinline__ int popc_s(S v) {
 template <typename S>
 __device__ __forceinline__ u32 config_of(S s, const DevGroup& g) {
     u32 c = 0;
-    for (u32 k = 0; k < g.nsup; ++k) c |= u32((s >> g.pos[k]) & 1) << k;
+    u64 pw = g.posw;
+    for (u32 k = 0; k < g.nsup; ++k, pw >>= 6) c |= u32((s >> u32(pw & 63)) & 1) << k;
     return c;
 }
 
-// Tile index range of this wave.  Workgroup b lives on XCD b % 8; XCD k owns
-// tiles [k*chunk, (k+1)*chunk) and its workgroups sweep that chunk together.
-__device__ __forceinline__ void tile_range(const ApplyParams& p, u32 wave, u32& t0, u32& tend,
-                                           u32& stride) {
-    const u32 nb = gridDim.x;
-    const u32 xcd = blockIdx.x % kXcd;
-    const u32 kx = blockIdx.x / kXcd;
-    const u32 per_xcd = nb / kXcd;
-    const u32 chunk = p.tiles_per_xcd;
-    const u32 start = xcd * chunk;
-    t0 = start + kx * kWaves + wave;
-    tend = min(start + chunk, p.ntiles);
-    stride = per_xcd * kWaves;
+// Tile schedule.  The tile list is cut into chunks of `chunk` tiles; chunk c
+// belongs to XCD c % 8 (workgroup b runs on XCD b % 8).  The waves of one
+// XCD sweep its chunks in list order, so at any moment the 8 XCDs work on 8
+// neighbouring chunks of the (class-major) list: each XCD's L2 holds the
+// rows of its chunk and the MALL holds the current class.
+struct TileIter {
+    u32 next_id, nchunks, w, W, chunk, ntiles, t, t_end;
+    __device__ __forceinline__ bool next_chunk() {
+        while (next_id < nchunks) {
+            const u32 c = next_id;
+            next_id += kXcd;
+            t = c * chunk + w;
+            t_end = min((c + 1) * chunk, ntiles);
+            if (t < t_end) return true;
+        }
+        return false;
+    }
+};
+
+template <int WV = kWaves, typename PP>
+__device__ __forceinline__ TileIter tile_iter(const PP& p, u32 wave) {
+    TileIter it;
+    it.w = (blockIdx.x / kXcd) * WV + wave;
+    it.W = (gridDim.x / kXcd) * WV;
+    it.chunk = p.tiles_per_xcd;   // tiles per chunk
+    it.ntiles = p.ntiles;
+    it.next_id = blockIdx.x % kXcd;
+    it.nchunks = (p.ntiles + it.chunk - 1) / it.chunk;
+    it.t = it.t_end = 0;
+    return it;
+}
+
+// Iterate the tiles of this wave: QK_FOR_TILES(p, wave, t) { ... }
+#define QK_FOR_TILES(p, wave, t)                                             \
+    for (TileIter it_ = tile_iter(p, wave); it_.next_chunk();)              \
+        for (u32 t = it_.t; t < it_.t_end; t += it_.W)
+#define QK_FOR_TILES_W(p, wave, t, WV)                                       \
+    for (TileIter it_ = tile_iter<WV>(p, wave); it_.next_chunk();)          \
+        for (u32 t = it_.t; t < it_.t_end; t += it_.W)
+
+template <typename T>
+__device__ __forceinline__ void store_nt(T* p, const T& v) {
+    __builtin_nontemporal_store(v.re, &p->re);
+    __builtin_nontemporal_store(v.im, &p->im);
 }
 
 template <typename S, int B>
@@ -91,10 +126,8 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 lane = threadIdx.x & 63;
     const S lomask = (S(1) << p.L) - 1;
-    u32 t, tend, stride;
-    tile_range(p, wave, t, tend, stride);
 
-    for (; t < tend; t += stride) {
+    QK_FOR_TILES(p, wave, t) {
         const uint2 td = p.tiles[t];
         const u32 hi = __builtin_amdgcn_readfirstlane(td.x);
         const u32 r0 = __builtin_amdgcn_readfirstlane(td.y);
@@ -179,6 +212,107 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
     }
 }
 
+// Exchange fast path (see XBond).  Per lane: the diagonal from two table
+// loads (low-half terms, aligned with lo_list; high-half terms per row,
+// uniform) plus the few terms that straddle the split; then each bond with
+// differing bits contributes a*x[target].  Low bonds rank the target through
+// the LDS table, high bonds are wave-uniform (one scalar test and one scalar
+// row lookup, then a coalesced 64-lane read of the other row), mixed bonds do
+// both.  Exchange conserves the counted popcount, so no class test is needed.
+// Workgroups of kXWaves waves share one LDS copy of the rank table, so the
+// 32 KB table costs LDS once per 16 waves and the CU holds 8 waves per SIMD.
+template <int B>
+__global__ void __launch_bounds__(64 * kXWaves)
+k_apply_xchg(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y, const u64 ld) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr u32 NT = 64 * kXWaves;
+    cplx* s_tab = reinterpret_cast<cplx*>(smem);
+    u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ndmix_tab) * sizeof(cplx));
+    for (u32 i = threadIdx.x; i < p.ndmix_tab; i += NT) s_tab[i] = p.dmix_tab[i];
+    if ((p.nlo & 7) == 0) {
+        const uint4* src = reinterpret_cast<const uint4*>(p.jlo);
+        uint4* dst = reinterpret_cast<uint4*>(s_jlo);
+        for (u32 i = threadIdx.x; i < p.nlo / 8; i += NT) dst[i] = src[i];
+    } else {
+        for (u32 i = threadIdx.x; i < p.nlo; i += NT) s_jlo[i] = p.jlo[i];
+    }
+    __syncthreads();
+
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const u32 lane = threadIdx.x & 63;
+    const u32 hi_end = p.n_lo + p.n_hi;
+    const u32 mix_end = hi_end + p.n_mix;
+
+    QK_FOR_TILES_W(p, wave, t, kXWaves) {
+        const XTile T = p.xtiles[t];                  // one scalar load
+        const u32 hi = __builtin_amdgcn_readfirstlane(T.hi);
+        const u64 base = T.base;
+        if (lane >= T.nvalid) continue;
+        const u32 r = T.r0 + lane;
+        const u32 lo = p.lo_list[T.pos0 + lane];
+        const u64 a = base + r;
+
+        double dg = p.dlo[T.pos0 + lane] + T.dh;
+        if (p.n_dmix) {
+            const u32 s = (hi << p.L) | lo;
+            for (u32 g = 0; g < p.n_dmix; ++g) {
+                const DevGroup G = p.dmix[g];
+                dg += s_tab[G.tab + config_of(s, G)].re;
+            }
+        }
+        cplx acc[B];
+#pragma unroll
+        for (int v = 0; v < B; ++v) {
+            const cplx xa = x[v * ld + a];
+            acc[v] = cplx{dg * xa.re, dg * xa.im};
+        }
+        u32 b = 0;
+        for (; b < p.n_lo; ++b) {
+            const XBond X = p.bonds[b];
+            const u32 vi = (lo >> X.bi) & 1, vj = (lo >> X.bj) & 1;
+            if (vi == vj) continue;
+            const double amp = vi ? X.a10 : X.a01;
+            const u64 idx = base + s_jlo[lo ^ X.mlo];
+#pragma unroll
+            for (int v = 0; v < B; ++v) {
+                const cplx xv = x[v * ld + idx];
+                acc[v].re += amp * xv.re;
+                acc[v].im += amp * xv.im;
+            }
+        }
+        for (; b < hi_end; ++b) {
+            const XBond X = p.bonds[b];
+            const u32 vi = (hi >> (X.bi - p.L)) & 1, vj = (hi >> (X.bj - p.L)) & 1;
+            if (vi == vj) continue;                          // uniform
+            const double amp = vi ? X.a10 : X.a01;
+            const u64 idx = p.jhi[hi ^ X.mhi] + r;
+#pragma unroll
+            for (int v = 0; v < B; ++v) {
+                const cplx xv = x[v * ld + idx];
+                acc[v].re += amp * xv.re;
+                acc[v].im += amp * xv.im;
+            }
+        }
+        for (; b < mix_end; ++b) {
+            const XBond X = p.bonds[b];
+            const u32 vj = (hi >> (X.bj - p.L)) & 1;       // uniform
+            const u64 base2 = p.jhi[hi ^ X.mhi];            // uniform
+            const u32 vi = (lo >> X.bi) & 1;
+            if (vi == vj || base2 == ~u64(0)) continue;
+            const double amp = vi ? X.a10 : X.a01;
+            const u64 idx = base2 + s_jlo[lo ^ X.mlo];
+#pragma unroll
+            for (int v = 0; v < B; ++v) {
+                const cplx xv = x[v * ld + idx];
+                acc[v].re += amp * xv.re;
+                acc[v].im += amp * xv.im;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < B; ++v) store_nt(&y[v * ld + a], acc[v]);
+    }
+}
+
 // Exact diagonal: one group per diagonal term, summed in OpSum order with
 // plain additions, as the reference accumulates h_diagonal (cpp:87-89).
 template <typename S>
@@ -187,9 +321,7 @@ k_diagonal(const ApplyParams p, cplx* __restrict__ d) {
 #pragma clang fp contract(off)
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 lane = threadIdx.x & 63;
-    u32 t, tend, stride;
-    tile_range(p, wave, t, tend, stride);
-    for (; t < tend; t += stride) {
+    QK_FOR_TILES(p, wave, t) {
         const uint2 td = p.tiles[t];
         const u32 hi = __builtin_amdgcn_readfirstlane(td.x);
         const u32 r0 = __builtin_amdgcn_readfirstlane(td.y);
@@ -217,9 +349,7 @@ k_sz_q(const ApplyParams p, const cplx* __restrict__ phase, int nsites, int nq,
        const cplx* __restrict__ psi, cplx* __restrict__ phi, u64 ld, double scale) {
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 lane = threadIdx.x & 63;
-    u32 t, tend, stride;
-    tile_range(p, wave, t, tend, stride);
-    for (; t < tend; t += stride) {
+    QK_FOR_TILES(p, wave, t) {
         const uint2 td = p.tiles[t];
         const u32 hi = __builtin_amdgcn_readfirstlane(td.x);
         const u32 r0 = __builtin_amdgcn_readfirstlane(td.y);
@@ -249,6 +379,13 @@ k_sz_q(const ApplyParams p, const cplx* __restrict__ phase, int nsites, int nq,
 
 template <typename S, int B>
 void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStream_t st) {
+    if constexpr (sizeof(S) == 4) {
+        if (h.xchg) {
+            hipLaunchKernelGGL((k_apply_xchg<B>), dim3(h.xgrid), dim3(64 * kXWaves), h.xlds_bytes,
+                               st, h.xparams, x, y, ld);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_apply<S, B>), dim3(h.grid), dim3(kBlock), h.lds_bytes, st, h.params, x,
                        y, ld);
 }
@@ -279,7 +416,7 @@ std::vector<DevGroup> pack_groups(const std::vector<Group>& gs, std::vector<cplx
         d.nsup = static_cast<u32>(g.pos.size());
         d.kind = static_cast<u32>(g.kind);
         d.hi_only = g.hi_only ? 1u : 0u;
-        for (size_t k = 0; k < g.pos.size(); ++k) d.pos[k] = static_cast<unsigned char>(g.pos[k]);
+        for (size_t k = 0; k < g.pos.size(); ++k) d.posw |= u64(g.pos[k] & 63) << (6 * k);
         tables.insert(tables.end(), g.table.begin(), g.table.end());
         out.push_back(d);
     }
@@ -287,6 +424,24 @@ std::vector<DevGroup> pack_groups(const std::vector<Group>& gs, std::vector<cplx
 }
 
 }  // namespace
+
+// Resident persistent grid for a kernel using `lds` bytes of LDS per
+// workgroup: as many workgroups as fit on the chip at once (LDS and the
+// 8-waves-per-SIMD register budget), a multiple of the XCD count.  The tile
+// chunk is a few tiles per resident wave of one XCD.
+static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
+    const int cus = device_cu_count();
+    const int by_lds = (160 * 1024) / std::max(lds, 1);
+    const int by_waves = 32 / waves;   // 8 waves per SIMD, 4 SIMDs per CU
+    const int per_cu = std::max(1, std::min({8, by_lds, by_waves}));
+    const int want = int((ntiles + waves - 1) / waves);
+    int blocks = std::min(want, cus * per_cu);
+    blocks = std::max(kXcd, (blocks + kXcd - 1) / kXcd * kXcd);
+    int mult = 2;
+    if (const char* m = std::getenv("QKRYLOV_CHUNK_MULT")) mult = std::max(1, std::atoi(m));
+    *chunk = u32(std::max(64, mult * (blocks / kXcd) * waves));
+    return blocks;
+}
 
 int device_cu_count() {
     static int cus = [] {
@@ -307,9 +462,17 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
     const Basis& B = *basis;
     state32 = B.nbits() <= 32;
 
-    // Tiles: 64 consecutive ranks of one row.
+    // Tiles: 64 consecutive ranks of one row, rows ordered class-major (all
+    // rows whose low halves share one class, i.e. whose high halves share
+    // one popcount, are adjacent: high-half flips stay inside that set).
+    std::vector<u32> rows(B.valid_hi().begin(), B.valid_hi().end());
+    const char* order = std::getenv("QKRYLOV_TILE_ORDER");
+    if (!(order && std::string(order) == "hi"))
+        std::stable_sort(rows.begin(), rows.end(), [&](u32 a, u32 b) {
+            return B.hi_cls()[a] < B.hi_cls()[b];
+        });
     std::vector<uint2> tiles;
-    for (u32 hi : B.valid_hi()) {
+    for (u32 hi : rows) {
         const u32 cnt = B.cls_cnt()[B.hi_cls()[hi]];
         for (u32 r0 = 0; r0 < cnt; r0 += 64) tiles.push_back(make_uint2(hi, r0));
     }
@@ -324,6 +487,7 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
     upload(d_clscnt, B.cls_cnt().data(), B.cls_cnt().size(), stream);
     upload(d_lolist, B.lo_list().data(), B.lo_list().size(), stream);
     upload(d_tiles, tiles.data(), tiles.size(), stream);
+    tiles_host = tiles;
     upload(d_groups, groups.data(), groups.size(), stream);
     upload(d_tables, tables.data(), tables.size(), stream);
     upload(d_dgroups, dgroups.data(), dgroups.size(), stream);
@@ -353,16 +517,8 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
     p.L = B.L();
     p.ncls2 = B.ncls2();
 
-    // Persistent grid: enough workgroups to fill every CU a few times over,
-    // a multiple of the XCD count, never more than there are tiles to run.
-    const int cus = device_cu_count();
-    const int want_blocks = int((tiles.size() + kWaves - 1) / kWaves);
-    int blocks = std::min(want_blocks, cus * 8);
-    blocks = std::max(kXcd, (blocks + kXcd - 1) / kXcd * kXcd);
-    grid = blocks;
-    p.tiles_per_xcd = static_cast<u32>((tiles.size() + kXcd - 1) / kXcd);
-
     lds_bytes = int(p.ntab * sizeof(cplx) + ((p.nlo * sizeof(u16) + 15) / 16) * 16);
+    grid = grid_for(lds_bytes, tiles.size(), &p.tiles_per_xcd);
     if (lds_bytes > 160 * 1024) fail("group tables exceed LDS");
     if (lds_bytes > 64 * 1024) {
         auto set = [&](const void* f) {
@@ -382,6 +538,134 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
     dparams.tables = d_dtables.get<cplx>();
     dparams.ngroups = static_cast<u32>(dgroups.size());
     dparams.ntab = static_cast<u32>(dtables.size());
+
+    const char* mode = std::getenv("QKRYLOV_APPLY");
+    force_generic = mode && std::string(mode) == "generic";
+    if (state32 && !force_generic) setup_exchange();
+}
+
+// Select the exchange fast path when every off-diagonal group is a real
+// two-bit exchange and the sector counts all bits or none.
+void Hamiltonian::setup_exchange() {
+    const Basis& B = *basis;
+    const u64 all = B.nbits() >= 64 ? ~u64(0) : ((u64(1) << B.nbits()) - 1);
+    if (B.kind() != SiteKind::SpinHalf && B.kind() != SiteKind::Fermion) return;
+    if (B.m2() != 0 || (B.m1() != 0 && B.m1() != all)) return;
+    const int L = B.L();
+    std::vector<XBond> lo_b, hi_b, mix_b;
+    std::vector<const Group*> dlo_g, dhi_g, dmix_g;
+    for (const Group& g : prog.groups) {
+        if (g.jw != 0) return;
+        for (const cplx& v : g.table)
+            if (v.im != 0.0) return;
+        if (g.kind == kDiag) continue;   // rebuilt per term below
+        if (popc(g.flip) != 2 || g.pos.size() != 2) return;
+        const int bi = g.pos[0], bj = g.pos[1];
+        if (g.flip != ((u64(1) << bi) | (u64(1) << bj))) return;
+        if (g.table[0].re != 0.0 || g.table[3].re != 0.0) return;
+        XBond x{};
+        x.bi = u32(bi);
+        x.bj = u32(bj);
+        x.a10 = g.table[1].re;   // config bit0 = bit(bi) = 1, bit1 = bit(bj) = 0
+        x.a01 = g.table[2].re;
+        x.mlo = u32(g.flip & B.lomask());
+        x.mhi = u32(g.flip >> L);
+        if (g.kind == kLo) lo_b.push_back(x);
+        else if (g.kind == kHi) hi_b.push_back(x);
+        else mix_b.push_back(x);
+    }
+    // Diagonal, term by term: terms on the low half go into a table aligned
+    // with lo_list, terms on the high half into a per-row value; only terms
+    // that straddle the split are evaluated per lane (merged by support).
+    std::vector<Group> mixg;
+    for (const Group& g : prog.diag_terms) {
+        if (g.jw != 0) return;
+        for (const cplx& v : g.table)
+            if (v.im != 0.0) return;
+        u64 sup = 0;
+        for (int p : g.pos) sup |= u64(1) << p;
+        if ((sup >> L) == 0) dlo_g.push_back(&g);
+        else if ((sup & B.lomask()) == 0) dhi_g.push_back(&g);
+        else {
+            bool merged = false;
+            for (Group& m : mixg)
+                if (m.pos == g.pos) {
+                    for (size_t c = 0; c < m.table.size(); ++c) m.table[c] = cadd(m.table[c], g.table[c]);
+                    merged = true;
+                    break;
+                }
+            if (!merged) mixg.push_back(g);
+        }
+    }
+    auto eval = [](const Group& g, u64 s) {
+        u32 c = 0;
+        for (size_t k = 0; k < g.pos.size(); ++k) c |= u32((s >> g.pos[k]) & 1) << k;
+        return g.table[c].re;
+    };
+    std::vector<double> dlo(B.lo_list().size(), 0.0), dhi(B.jhi().size(), 0.0);
+    for (size_t i = 0; i < dlo.size(); ++i)
+        for (const Group* g : dlo_g) dlo[i] += eval(*g, B.lo_list()[i]);
+    for (size_t h = 0; h < dhi.size(); ++h)
+        for (const Group* g : dhi_g) dhi[h] += eval(*g, u64(h) << L);
+    std::vector<cplx> mixtab;
+    std::vector<DevGroup> mixdev = pack_groups(mixg, mixtab);
+
+    std::vector<XTile> xt;
+    xt.reserve(tiles_host.size());
+    for (const uint2& tl : tiles_host) {
+        const u32 hi = tl.x, r0 = tl.y;
+        const int c = B.hi_cls()[hi];
+        XTile T{};
+        T.base = B.jhi()[hi];
+        T.pos0 = B.cls_off()[c] + r0;
+        T.hi = hi;
+        T.r0 = r0;
+        T.nvalid = std::min<u32>(64, B.cls_cnt()[c] - r0);
+        T.dh = dhi[hi];
+        xt.push_back(T);
+    }
+    upload(d_xtiles, xt.data(), xt.size(), stream);
+    std::vector<XBond> bonds = lo_b;
+    bonds.insert(bonds.end(), hi_b.begin(), hi_b.end());
+    bonds.insert(bonds.end(), mix_b.begin(), mix_b.end());
+
+    upload(d_xbonds, bonds.data(), bonds.size(), stream);
+    upload(d_dlo, dlo.data(), dlo.size(), stream);
+    upload(d_dhi, dhi.data(), dhi.size(), stream);
+    upload(d_dmix, mixdev.data(), mixdev.size(), stream);
+    upload(d_dmixtab, mixtab.data(), mixtab.size(), stream);
+    QK_HIP(hipStreamSynchronize(stream));
+
+    XParams& x = xparams;
+    x.xtiles = d_xtiles.get<XTile>();
+    x.jlo = params.jlo;
+    x.jhi = params.jhi;
+    x.hi_cls = params.hi_cls;
+    x.cls_off = params.cls_off;
+    x.cls_cnt = params.cls_cnt;
+    x.lo_list = params.lo_list;
+    x.dlo = d_dlo.get<double>();
+    x.dhi = d_dhi.get<double>();
+    x.tiles = params.tiles;
+    x.bonds = d_xbonds.get<XBond>();
+    x.dmix = d_dmix.get<DevGroup>();
+    x.dmix_tab = d_dmixtab.get<cplx>();
+    x.ntiles = params.ntiles;
+    x.n_lo = u32(lo_b.size());
+    x.n_hi = u32(hi_b.size());
+    x.n_mix = u32(mix_b.size());
+    x.n_dmix = u32(mixdev.size());
+    x.ndmix_tab = u32(mixtab.size());
+    x.nlo = params.nlo;
+    x.L = L;
+    xlds_bytes = int(x.ndmix_tab * sizeof(cplx) + ((x.nlo * sizeof(u16) + 15) / 16) * 16);
+    xgrid = grid_for(xlds_bytes, x.ntiles, &x.tiles_per_xcd, kXWaves);
+    if (xlds_bytes > 64 * 1024) {
+        for (const void* f : {(const void*)k_apply_xchg<1>, (const void*)k_apply_xchg<2>,
+                              (const void*)k_apply_xchg<4>, (const void*)k_apply_xchg<8>})
+            QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
+    }
+    xchg = true;
 }
 
 Hamiltonian::~Hamiltonian() {
@@ -440,7 +724,9 @@ std::string Hamiltonian::info() const {
        << ", \"lo\": " << prog.n_lo << ", \"hi\": " << prog.n_hi << ", \"mix\": " << prog.n_mix
        << ", \"tiles\": " << params.ntiles << ", \"grid\": " << grid
        << ", \"lds\": " << lds_bytes << ", \"state32\": " << (state32 ? 1 : 0)
-       << ", \"max_support\": " << prog.max_support << "}";
+       << ", \"max_support\": " << prog.max_support << ", \"exchange_path\": " << (xchg ? 1 : 0)
+       << ", \"xbonds\": [" << xparams.n_lo << ", " << xparams.n_hi << ", " << xparams.n_mix
+       << "], \"diag_mixed\": " << xparams.n_dmix << "}";
     return os.str();
 }
 

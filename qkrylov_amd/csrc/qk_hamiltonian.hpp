@@ -13,18 +13,63 @@
 
 namespace qk {
 
-// Group descriptor as the kernels read it (scalar loads, 48 B).
+// Group descriptor as the kernels read it (scalar loads, 48 B).  Support
+// positions are packed 6 bits each into `posw` so that no array is indexed
+// at run time (a dynamically indexed member would live in scratch memory).
 struct DevGroup {
     u64 flip;
     u64 jw;
+    u64 posw;      // support bit k at bits [6k, 6k+6)
     u32 tab;       // offset of the first table entry (cplx units)
     u32 nsup;      // number of support bits
     u32 kind;      // GroupKind
     u32 hi_only;   // support and jw inside the high half
-    unsigned char pos[8];
     u64 pad;
 };
 static_assert(sizeof(DevGroup) == 48, "DevGroup layout");
+
+// Exchange fast path: every off-diagonal group flips two bits i, j with real
+// amplitude a01 (bit i = 0, bit j = 1 before the flip) or a10, on a basis
+// whose sectors count all bits (spin-1/2 Sz, spinless fermion N) or none.
+struct XBond {
+    u32 mlo;       // flip mask inside the low half
+    u32 mhi;       // flip mask inside the high half (shifted down by L)
+    u32 bi, bj;    // bit positions of the two flipped bits (bi < bj)
+    double a01, a10;
+};
+static_assert(sizeof(XBond) == 32, "XBond layout");
+
+// Self-contained tile for the exchange kernel: 64 consecutive ranks of one
+// row, everything the wave needs in one 32-byte scalar load.
+struct XTile {
+    u64 base;      // jhi[hi]: global index of rank 0 of the row
+    u32 pos0;      // lo_list position of lane 0 (class offset + r0)
+    u32 hi;
+    u32 r0;        // rank of lane 0 inside the row
+    u32 nvalid;    // lanes holding a state (<= 64)
+    double dh;     // diagonal part from high-half-only terms
+};
+static_assert(sizeof(XTile) == 32, "XTile layout");
+
+struct XParams {
+    const XTile* xtiles;
+    const u16* jlo;
+    const u64* jhi;
+    const int* hi_cls;
+    const u32* cls_off;
+    const u32* cls_cnt;
+    const u32* lo_list;
+    const double* dlo;      // diagonal part from low-half-only terms, lo_list order
+    const double* dhi;      // diagonal part from high-half-only terms, per hi
+    const uint2* tiles;
+    const XBond* bonds;     // lo bonds, then hi bonds, then mixed bonds
+    const DevGroup* dmix;   // diagonal groups spanning both halves
+    const cplx* dmix_tab;
+    u32 ntiles, n_lo, n_hi, n_mix, n_dmix, ndmix_tab;
+    u32 nlo;
+    int L;
+    u32 tiles_per_xcd;
+};
 
 // Everything one apply launch needs, passed by value as the kernel argument.
 struct ApplyParams {
@@ -64,6 +109,7 @@ struct Hamiltonian {
     void sz_q(const cplx* psi, cplx* phi, int nq, u64 ld, hipStream_t st) const;
 
     std::string info() const;
+    void setup_exchange();
 
     std::shared_ptr<const Basis> basis;
     SiteKind site;
@@ -75,8 +121,15 @@ struct Hamiltonian {
     // device-resident tables
     DevBuf d_jlo, d_jhi, d_hicls, d_clsoff, d_clscnt, d_lolist, d_tiles, d_groups, d_tables;
     DevBuf d_dgroups, d_dtables;   // per-term diagonal program
+    DevBuf d_xbonds, d_dlo, d_dhi, d_dmix, d_dmixtab, d_xtiles;   // exchange fast path
+    std::vector<uint2> tiles_host;   // (hi, r0) per tile, launch order
     ApplyParams params{};
     ApplyParams dparams{};
+    XParams xparams{};
+    bool xchg = false;          // exchange fast path selected
+    int xlds_bytes = 0;
+    int xgrid = 0;
+    bool force_generic = false; // QKRYLOV_APPLY=generic
     bool state32 = true;
     int lds_bytes = 0;
     int dlds_bytes = 0;

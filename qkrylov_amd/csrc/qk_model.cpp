@@ -4,6 +4,7 @@
 #include "qk_model.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <tuple>
 
@@ -90,6 +91,12 @@ Basis::Basis(SiteKind kind, int nsites, const Sector& sector)
     } else {
         L_ = std::min(14, std::max(nbits_ - 14, (nbits_ + 1) / 2));
         if (two_species && (L_ & 1)) L_ -= 1;
+    }
+    // Split override for tuning; the split never changes ranks or results.
+    if (const char* e = std::getenv("QKRYLOV_SPLIT_L")) {
+        const int l = std::atoi(e);
+        if (l >= 1 && l <= 15 && l <= nbits_ && nbits_ - l <= 26 && !(two_species && (l & 1)))
+            L_ = l;
     }
     H_ = nbits_ - L_;
     if (H_ > 26) fail("basis too large: more than 2^26 distinct high halves");

@@ -19,10 +19,20 @@ def _rand(d, seed, nvec=None):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
+EXCHANGE_MODELS = {"heis_chain10_sz0", "heis_obc4_sz0", "j1j2_4x3_sz0", "heis_chain20_sz0"}
+
+
+@pytest.mark.parametrize("path", ["auto", "generic"])
 @pytest.mark.parametrize("name", sorted(MODELS))
-def test_apply_matches_oracle(qk, orc, name):
-    print(f"[apply] {name}", flush=True)
+def test_apply_matches_oracle(qk, orc, name, path, monkeypatch):
+    print(f"[apply] {name} path={path}", flush=True)
+    if path == "generic":
+        monkeypatch.setenv("QKRYLOV_APPLY", "generic")
     H, oh, _ = build_pair(orc, name)
+    if path == "auto" and name in EXCHANGE_MODELS:
+        assert H.info()["exchange_path"] == 1
+    if path == "generic":
+        assert H.info()["exchange_path"] == 0
     assert H.dimension == oh.dim
     x = _rand(H.dimension, 11)
     y = H.apply(x)
