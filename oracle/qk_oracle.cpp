@@ -138,9 +138,13 @@ static double jw(StateID s, int bitpos) {
 }
 
 // Mutation hook for tests that show the pins catch a planted mistake:
-// 1 flips the Jordan-Wigner sign convention of C/Cdag, 2 drops the conj of
-// stored amplitudes.
+// 1 counts the site's own bit in the Jordan-Wigner string (off by one),
+// 2 drops the conj of stored amplitudes.
 static int g_mutation = 0;
+
+static double jw_m(StateID s, int bitpos) {
+    return g_mutation == 1 ? jw(s, bitpos + 1) : jw(s, bitpos);
+}
 
 static LocalAction site_apply(int kind, const std::string& op, int site, StateID s) {
     LocalAction a;
@@ -155,7 +159,7 @@ static LocalAction site_apply(int kind, const std::string& op, int site, StateID
     }
     if (kind == FERMION) {  // src/sites/fermion_site.cpp:19-84
         const bool occ = (s >> site) & 1ULL;
-        const double ph = g_mutation == 1 ? -jw(s, site) : jw(s, site);
+        const double ph = jw_m(s, site);
         if (op == "N") { a.valid = true; a.new_state = s; a.me = occ ? 1.0 : 0.0; return a; }
         if (op == "Id") { a.valid = true; a.new_state = s; a.me = 1.0; return a; }
         if (op == "C") { if (!occ) return a; a.valid = true; a.new_state = s & ~(1ULL << site); a.me = ph; return a; }
@@ -166,8 +170,8 @@ static LocalAction site_apply(int kind, const std::string& op, int site, StateID
     const bool tj = (kind == TJ);
     const int bu = 2 * site, bd = 2 * site + 1;
     const bool up = (s >> bu) & 1ULL, dn = (s >> bd) & 1ULL;
-    const double pu = g_mutation == 1 ? -jw(s, bu) : jw(s, bu);
-    const double pd = g_mutation == 1 ? -jw(s, bd) : jw(s, bd);
+    const double pu = jw_m(s, bu);
+    const double pd = jw_m(s, bd);
     if (op == "Nup") { a.valid = true; a.new_state = s; a.me = up ? 1.0 : 0.0; return a; }
     if (op == "Ndn") { a.valid = true; a.new_state = s; a.me = dn ? 1.0 : 0.0; return a; }
     if (!tj && op == "Nupdn") { a.valid = true; a.new_state = s; a.me = (up && dn) ? 1.0 : 0.0; return a; }

@@ -313,6 +313,122 @@ k_apply_xchg(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y, 
     }
 }
 
+// Single-vector exchange kernel with SPL states per lane: a tile spans
+// 64*SPL consecutive ranks of one row and lane l holds ranks r0+l+64k.  The
+// SPL gathers of one bond are independent, so each wave keeps SPL loads in
+// flight where the one-state kernel has one.
+template <int SPL>
+__global__ void __launch_bounds__(64 * kXWaves)
+k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr u32 NT = 64 * kXWaves;
+    cplx* s_tab = reinterpret_cast<cplx*>(smem);
+    u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ndmix_tab) * sizeof(cplx));
+    for (u32 i = threadIdx.x; i < p.ndmix_tab; i += NT) s_tab[i] = p.dmix_tab[i];
+    if ((p.nlo & 7) == 0) {
+        const uint4* src = reinterpret_cast<const uint4*>(p.jlo);
+        uint4* dst = reinterpret_cast<uint4*>(s_jlo);
+        for (u32 i = threadIdx.x; i < p.nlo / 8; i += NT) dst[i] = src[i];
+    } else {
+        for (u32 i = threadIdx.x; i < p.nlo; i += NT) s_jlo[i] = p.jlo[i];
+    }
+    __syncthreads();
+
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const u32 lane = threadIdx.x & 63;
+    const u32 hi_end = p.n_lo + p.n_hi;
+    const u32 mix_end = hi_end + p.n_mix;
+
+    QK_FOR_TILES_W(p, wave, t, kXWaves) {
+        const XTile T = p.xtiles[t];
+        const u32 hi = __builtin_amdgcn_readfirstlane(T.hi);
+        const u64 base = T.base;
+        if (lane >= T.nvalid) continue;
+        u32 lo[SPL], r[SPL];
+        bool v[SPL];
+        cplx acc[SPL];
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            const u32 j = lane + 64u * k;
+            v[k] = j < T.nvalid;
+            r[k] = T.r0 + j;
+            lo[k] = v[k] ? p.lo_list[T.pos0 + j] : 0u;
+            double dg = v[k] ? p.dlo[T.pos0 + j] + T.dh : 0.0;
+            if (p.n_dmix) {
+                const u32 s = (hi << p.L) | lo[k];
+                for (u32 g = 0; g < p.n_dmix; ++g) {
+                    const DevGroup G = p.dmix[g];
+                    dg += s_tab[G.tab + config_of(s, G)].re;
+                }
+            }
+            cplx xa{0.0, 0.0};
+            if (v[k]) xa = x[base + r[k]];
+            acc[k] = cplx{dg * xa.re, dg * xa.im};
+        }
+        u32 b = 0;
+        for (; b < p.n_lo; ++b) {
+            const XBond X = p.bonds[b];
+            cplx xv[SPL];
+            double amp[SPL];
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                const u32 vi = (lo[k] >> X.bi) & 1, vj = (lo[k] >> X.bj) & 1;
+                const bool ok = v[k] && vi != vj;
+                amp[k] = ok ? (vi ? X.a10 : X.a01) : 0.0;
+                xv[k] = cplx{0.0, 0.0};
+                if (ok) xv[k] = x[base + s_jlo[lo[k] ^ X.mlo]];
+            }
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                acc[k].re += amp[k] * xv[k].re;
+                acc[k].im += amp[k] * xv[k].im;
+            }
+        }
+        for (; b < hi_end; ++b) {
+            const XBond X = p.bonds[b];
+            const u32 vi = (hi >> (X.bi - p.L)) & 1, vj = (hi >> (X.bj - p.L)) & 1;
+            if (vi == vj) continue;                          // uniform
+            const double amp = vi ? X.a10 : X.a01;
+            const u64 base2 = p.jhi[hi ^ X.mhi];
+            cplx xv[SPL];
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                xv[k] = cplx{0.0, 0.0};
+                if (v[k]) xv[k] = x[base2 + r[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                acc[k].re += amp * xv[k].re;
+                acc[k].im += amp * xv[k].im;
+            }
+        }
+        for (; b < mix_end; ++b) {
+            const XBond X = p.bonds[b];
+            const u32 vj = (hi >> (X.bj - p.L)) & 1;       // uniform
+            const u64 base2 = p.jhi[hi ^ X.mhi];            // uniform
+            if (base2 == ~u64(0)) continue;
+            cplx xv[SPL];
+            double amp[SPL];
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                const u32 vi = (lo[k] >> X.bi) & 1;
+                const bool ok = v[k] && vi != vj;
+                amp[k] = ok ? (vi ? X.a10 : X.a01) : 0.0;
+                xv[k] = cplx{0.0, 0.0};
+                if (ok) xv[k] = x[base2 + s_jlo[lo[k] ^ X.mlo]];
+            }
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                acc[k].re += amp[k] * xv[k].re;
+                acc[k].im += amp[k] * xv[k].im;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < SPL; ++k)
+            if (v[k]) store_nt(&y[base + r[k]], acc[k]);
+    }
+}
+
 // Exact diagonal: one group per diagonal term, summed in OpSum order with
 // plain additions, as the reference accumulates h_diagonal (cpp:87-89).
 template <typename S>
@@ -381,6 +497,18 @@ template <typename S, int B>
 void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStream_t st) {
     if constexpr (sizeof(S) == 4) {
         if (h.xchg) {
+            if constexpr (B == 1) {
+                if (h.spl == 4)
+                    hipLaunchKernelGGL((k_apply_xchg_s<4>), dim3(h.xgrid_s), dim3(64 * kXWaves),
+                                       h.xlds_bytes, st, h.xparams_s, x, y);
+                else if (h.spl == 2)
+                    hipLaunchKernelGGL((k_apply_xchg_s<2>), dim3(h.xgrid_s), dim3(64 * kXWaves),
+                                       h.xlds_bytes, st, h.xparams_s, x, y);
+                else
+                    hipLaunchKernelGGL((k_apply_xchg_s<1>), dim3(h.xgrid_s), dim3(64 * kXWaves),
+                                       h.xlds_bytes, st, h.xparams_s, x, y);
+                return;
+            }
             hipLaunchKernelGGL((k_apply_xchg<B>), dim3(h.xgrid), dim3(64 * kXWaves), h.xlds_bytes,
                                st, h.xparams, x, y, ld);
             return;
@@ -610,21 +738,49 @@ void Hamiltonian::setup_exchange() {
     std::vector<cplx> mixtab;
     std::vector<DevGroup> mixdev = pack_groups(mixg, mixtab);
 
-    std::vector<XTile> xt;
-    xt.reserve(tiles_host.size());
-    for (const uint2& tl : tiles_host) {
-        const u32 hi = tl.x, r0 = tl.y;
-        const int c = B.hi_cls()[hi];
-        XTile T{};
-        T.base = B.jhi()[hi];
-        T.pos0 = B.cls_off()[c] + r0;
-        T.hi = hi;
-        T.r0 = r0;
-        T.nvalid = std::min<u32>(64, B.cls_cnt()[c] - r0);
-        T.dh = dhi[hi];
-        xt.push_back(T);
-    }
+    // Tiles of `span` consecutive ranks, rows in the launch order of tiles_host.
+    auto make_xtiles = [&](u32 span) {
+        std::vector<XTile> xt;
+        u32 last = ~0u;
+        for (const uint2& tl : tiles_host) {
+            const u32 hi = tl.x;
+            if (hi == last) continue;
+            last = hi;
+            const int c = B.hi_cls()[hi];
+            const u32 cnt = B.cls_cnt()[c];
+            for (u32 r0 = 0; r0 < cnt; r0 += span) {
+                XTile T{};
+                T.base = B.jhi()[hi];
+                T.pos0 = B.cls_off()[c] + r0;
+                T.hi = hi;
+                T.r0 = r0;
+                T.nvalid = std::min<u32>(span, cnt - r0);
+                T.dh = dhi[hi];
+                xt.push_back(T);
+            }
+        }
+        return xt;
+    };
+    const std::vector<XTile> xt = make_xtiles(64);
     upload(d_xtiles, xt.data(), xt.size(), stream);
+    spl = 2;
+    if (const char* e = std::getenv("QKRYLOV_SPL")) spl = std::atoi(e);
+    if (spl != 1 && spl != 2 && spl != 4) spl = 2;
+    std::vector<XTile> xts = make_xtiles(64u * u32(spl));
+    // Column-block order: inside a class, all rows visit rank window
+    // [w*cb, (w+1)*cb) before any row moves to the next window, so the
+    // high-half gathers (same ranks, other rows) find their rows' windows
+    // recently touched.  cb = 0 keeps plain row order.
+    u32 cb = 0;
+    if (const char* e = std::getenv("QKRYLOV_COLBLOCK")) cb = u32(std::max(0, std::atoi(e)));
+    if (cb > 0) {
+        std::stable_sort(xts.begin(), xts.end(), [&](const XTile& a, const XTile& b) {
+            const int ca = B.hi_cls()[a.hi], cb2 = B.hi_cls()[b.hi];
+            if (ca != cb2) return ca < cb2;
+            return a.r0 / cb < b.r0 / cb;
+        });
+    }
+    upload(d_xtiles_s, xts.data(), xts.size(), stream);
     std::vector<XBond> bonds = lo_b;
     bonds.insert(bonds.end(), hi_b.begin(), hi_b.end());
     bonds.insert(bonds.end(), mix_b.begin(), mix_b.end());
@@ -659,10 +815,17 @@ void Hamiltonian::setup_exchange() {
     x.nlo = params.nlo;
     x.L = L;
     xlds_bytes = int(x.ndmix_tab * sizeof(cplx) + ((x.nlo * sizeof(u16) + 15) / 16) * 16);
+    x.ntiles = u32(xt.size());
     xgrid = grid_for(xlds_bytes, x.ntiles, &x.tiles_per_xcd, kXWaves);
+    xparams_s = x;
+    xparams_s.xtiles = d_xtiles_s.get<XTile>();
+    xparams_s.ntiles = u32(xts.size());
+    xgrid_s = grid_for(xlds_bytes, xparams_s.ntiles, &xparams_s.tiles_per_xcd, kXWaves);
     if (xlds_bytes > 64 * 1024) {
         for (const void* f : {(const void*)k_apply_xchg<1>, (const void*)k_apply_xchg<2>,
-                              (const void*)k_apply_xchg<4>, (const void*)k_apply_xchg<8>})
+                              (const void*)k_apply_xchg<4>, (const void*)k_apply_xchg<8>,
+                              (const void*)k_apply_xchg_s<1>, (const void*)k_apply_xchg_s<2>,
+                              (const void*)k_apply_xchg_s<4>})
             QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
     }
     xchg = true;

@@ -121,7 +121,10 @@ struct Hamiltonian {
     // device-resident tables
     DevBuf d_jlo, d_jhi, d_hicls, d_clsoff, d_clscnt, d_lolist, d_tiles, d_groups, d_tables;
     DevBuf d_dgroups, d_dtables;   // per-term diagonal program
-    DevBuf d_xbonds, d_dlo, d_dhi, d_dmix, d_dmixtab, d_xtiles;   // exchange fast path
+    DevBuf d_xbonds, d_dlo, d_dhi, d_dmix, d_dmixtab, d_xtiles, d_xtiles_s;   // exchange path
+    XParams xparams_s{};        // single-vector kernel, spl states per lane
+    int xgrid_s = 0;
+    int spl = 2;
     std::vector<uint2> tiles_host;   // (hi, r0) per tile, launch order
     ApplyParams params{};
     ApplyParams dparams{};

@@ -8,14 +8,58 @@
 #include "qk_vec.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
 namespace qk {
 
 namespace {
+
+// Phase timer, printed to stderr when QKRYLOV_TIMING is set.
+struct Timer {
+    const char* name;
+    bool on;
+    std::chrono::steady_clock::time_point t0;
+    explicit Timer(const char* n)
+        : name(n), on(std::getenv("QKRYLOV_TIMING") != nullptr), t0(std::chrono::steady_clock::now()) {}
+    void mark(const char* what) {
+        if (!on) return;
+        const auto t = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[qkrylov %s] %-24s %9.3f ms\n", name, what,
+                     std::chrono::duration<double, std::milli>(t - t0).count());
+        t0 = t;
+    }
+};
+
+// The reference normal stream for a seed (std::mt19937 + normal(0,1),
+// ftlm.cpp:133-143) in pinned host memory, kept for the process: a longer
+// request extends it (the stream for n draws is a prefix of the stream for
+// more), so repeated FTLM calls neither regenerate nor restage it.
+std::shared_ptr<const cplx> reference_normals(unsigned seed, size_t n) {
+    static std::mutex m;
+    // never destroyed: the last buffer is released by process exit, not by a
+    // static destructor that could run after the HIP runtime is torn down
+    static auto* cached = new std::shared_ptr<const cplx>();
+    static unsigned cseed = 0;
+    static size_t cn = 0;
+    std::lock_guard<std::mutex> lk(m);
+    if (*cached && cseed == seed && cn >= n) return *cached;
+    cplx* p = nullptr;
+    QK_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(cplx) * std::max<size_t>(n, 1)));
+    NormalStream ns(seed);
+    ns.fill(p, n);
+    *cached = std::shared_ptr<const cplx>(p, [](const cplx* q) { (void)hipHostFree(const_cast<cplx*>(q)); });
+    cseed = seed;
+    cn = n;
+    return *cached;
+}
 
 template <typename T>
 T fetch(const void* dev, hipStream_t st) {
@@ -401,27 +445,57 @@ struct BlockRec {
         return n;
     }
 
-    // M steps; alpha/beta on the host as [step][sample].
-    void run(int M, std::vector<double>& ah, std::vector<double>& bh) {
+    // M steps; alpha/beta on the host as [step][sample].  With `graph` the
+    // M steps (every apply and recurrence kernel, no host work in between:
+    // alpha, beta and the scales stay on the device) are captured once into
+    // a hipGraph and launched as one unit.
+    void run(int M, std::vector<double>& ah, std::vector<double>& bh, bool graph = false) {
         const size_t tot = size_t(std::max(M, 1)) * std::max(nsamp, 1);
         alpha.resize(sizeof(double) * tot);
         beta.resize(sizeof(double) * tot);
         QK_HIP(hipMemsetAsync(alpha.get(), 0, sizeof(double) * tot, st));
         QK_HIP(hipMemsetAsync(beta.get(), 0, sizeof(double) * tot, st));
-        cplx* U = bufU.get<cplx>();
-        cplx* Y = bufY.get<cplx>();
-        cplx* P = bufP.get<cplx>();
-        for (int step = 0; step < M; ++step) {
-            for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
-            double* a = alpha.get<double>() + size_t(step) * nsamp;
-            double* b = beta.get<double>() + size_t(step) * nsamp;
-            vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);
-            vec::seg_update(seg, Y, U, P, c.get<double>(), cp.get<double>(), bp.get<double>(), a, b,
-                            alive.get<int>(), step == 0 ? 1 : 0, st);
-            cplx* oldP = P;
-            P = U;
-            U = Y;
-            Y = oldP;
+        auto steps = [&]() {
+            cplx* U = bufU.get<cplx>();
+            cplx* Y = bufY.get<cplx>();
+            cplx* P = bufP.get<cplx>();
+            for (int step = 0; step < M; ++step) {
+                for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
+                double* a = alpha.get<double>() + size_t(step) * nsamp;
+                double* b = beta.get<double>() + size_t(step) * nsamp;
+                vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);
+                vec::seg_update(seg, Y, U, P, c.get<double>(), cp.get<double>(), bp.get<double>(), a,
+                                b, alive.get<int>(), step == 0 ? 1 : 0, st);
+                cplx* oldP = P;
+                P = U;
+                U = Y;
+                Y = oldP;
+            }
+        };
+        if (graph && M > 0) {
+            QK_HIP(hipStreamSynchronize(st));
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+            QK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            try {
+                steps();
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            QK_HIP(hipStreamEndCapture(st, &g));
+            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                QK_HIP(e);
+            }
+            QK_HIP(hipGraphLaunch(ge, st));
+            QK_HIP(hipStreamSynchronize(st));
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+        } else {
+            steps();
         }
         ah.resize(size_t(M) * nsamp);
         bh.resize(size_t(M) * nsamp);
@@ -523,11 +597,10 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
 
     // The reference re-seeds std::mt19937(42) on every call, so sector s uses
     // the first R*d_s complex draws of one common stream.
-    std::vector<cplx> stream(size_t(R) * dmax);
-    if (R > 0) {
-        NormalStream ns(42);
-        ns.fill(stream.data(), stream.size());
-    }
+    Timer tm("ftlm");
+    const std::shared_ptr<const cplx> stream_p = reference_normals(42, size_t(R) * dmax);
+    const cplx* stream = stream_p.get();
+    tm.mark("random stream");
 
     // per (sector, vector) tridiagonal results
     struct Sample {
@@ -551,12 +624,14 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
             BlockRec rec;
             rec.setup(g, st);
             for (size_t gi = 0; gi < g.size(); ++gi)
-                QK_HIP(hipMemcpyAsync(rec.bufU.get<cplx>() + rec.grps[gi].off, stream.data(),
+                QK_HIP(hipMemcpyAsync(rec.bufU.get<cplx>() + rec.grps[gi].off, stream,
                                       sizeof(cplx) * g[gi].first->dim() * R,
                                       hipMemcpyHostToDevice, st));
             const std::vector<double> nrm = rec.start(0.0);
+            tm.mark("stage vectors");
             std::vector<double> ah, bh;
-            rec.run(n_steps, ah, bh);
+            rec.run(n_steps, ah, bh, /*graph=*/true);
+            tm.mark("block lanczos");
             int j = 0;
             for (size_t gi = 0; gi < g.size(); ++gi)
                 for (int r = 0; r < R; ++r, ++j) {
@@ -572,7 +647,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
             BlockRec rec;
             rec.setup({{hs[s], 1}}, st);
             for (int r = 0; r < R; ++r) {
-                QK_HIP(hipMemcpyAsync(rec.bufU.get(), stream.data() + size_t(r) * d,
+                QK_HIP(hipMemcpyAsync(rec.bufU.get(), stream + size_t(r) * d,
                                       sizeof(cplx) * d, hipMemcpyHostToDevice, st));
                 const std::vector<double> nrm = rec.start(0.0);
                 std::vector<double> ah, bh;
@@ -584,6 +659,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
         }
     }
 
+    tm.mark("lanczos (all)");
     // small eigensolves, then Boltzmann weights for every beta (:151-167)
     std::vector<std::vector<double>> eig(samples.size()), z0(samples.size());
     parallel_for(int(samples.size()), [&](int i) {
@@ -592,6 +668,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
         z0[i].resize(n);
         tridiag_ql(samples[i].a.data(), samples[i].b.data(), n, eig[i].data(), z0[i].data(), nullptr);
     });
+    tm.mark("tridiagonal eigensolves");
     std::vector<double> Zs(size_t(nsec) * nb, 0.0), Es(Zs.size(), 0.0), E2s(Zs.size(), 0.0);
     parallel_for(nb, [&](int b) {
         const double beta = betas[b];
