@@ -226,8 +226,15 @@ int qkrylov_hamiltonian_apply_device_f64(qkrylov_hamiltonian_h h, const void* x_
  * cpp:81-89); diag_complex: 2*dim doubles. */
 int qkrylov_hamiltonian_diagonal_f64(qkrylov_hamiltonian_h h, double* diag_complex);
 
-/* Number of compiled off-diagonal groups / diagonal groups (introspection). */
+/* Compiled program and kernel choice as JSON (groups per kind, tiles, grid,
+ * exchange path, states per lane, bond batch, block grouping). */
 int qkrylov_hamiltonian_info(qkrylov_hamiltonian_h h, char* buf, size_t len);
+
+/* Autotuner: time the apply-kernel variants of h (states per lane, bond
+ * batch, and for nvec > 1 the vectors per block launch) on scratch vectors
+ * and keep the fastest for later applies; buf receives the timings (JSON,
+ * may be NULL).  Allocates 2*nvec*dim complex doubles while it runs. */
+int qkrylov_hamiltonian_autotune(qkrylov_hamiltonian_h h, int nvec, char* buf, size_t len);
 
 /* Lanczos ground state in FP64 (reference src/solvers/lanczos.cpp:101-202).
  * start_complex: NULL for the reference start vector (std::mt19937(1234),
@@ -239,13 +246,63 @@ int qkrylov_lanczos_ground_state_f64(qkrylov_hamiltonian_h h, int maxiter, doubl
                                      qkrylov_lanczos_result_f64_t* result,
                                      double* eigvec_complex);
 
-/* Davidson lowest eigenpairs in FP64 (reference src/solvers/davidson.cpp:85-227).
- * eigenvalues_out[n_eig]; eigenvectors_complex_out: n_eig*2*dim doubles or NULL.
- * precond: 0 = Jacobi diagonal (reference), 1 = learned MLP (falls back to
- * Jacobi when no model is loaded). */
+/* Davidson lowest eigenpairs in FP64 (reference src/solvers/davidson.cpp:85-227),
+ * Jacobi diagonal correction, at most 100 iterations as the reference.
+ * eigenvalues_out[n_eig]; eigenvectors_complex_out: n_eig*2*dim doubles or NULL. */
 int qkrylov_davidson_lowest_f64(qkrylov_hamiltonian_h h, int n_eig, int max_subspace, double tol,
                                 double* eigenvalues_out, double* eigenvectors_complex_out,
                                 qkrylov_davidson_result_c_t* result_info);
+
+/* ---- Learned Davidson preconditioner.  No counterpart in the reference,
+ * whose Davidson corrects with the Jacobi diagonal (davidson.cpp:166-177).
+ * A 3-layer MLP (32 -> 128 -> 128 -> 1, ReLU, BF16 weights, FP32
+ * accumulation on MFMA) maps per-state features (diagonal, theta,
+ * connectivity, local occupations) to o_i, and the correction becomes
+ * t_i = (1/(H_ii - theta) + o_i) r_i; an untrained network (last layer 0)
+ * is the Jacobi method. ---- */
+typedef struct qkrylov_mlp_t* qkrylov_mlp_h;
+
+typedef struct {
+    int pairs;             /* (residual, accurate correction) pairs generated */
+    int epochs;
+    double loss_first;     /* mean 1 - cos^2 before training */
+    double loss_last;      /* after training */
+    double solve_rel_res;  /* worst relative residual of the accurate corrections */
+} qkrylov_mlp_train_info_t;
+
+/* Random hidden layers from std::mt19937(seed), output layer zero. */
+qkrylov_mlp_h qkrylov_mlp_create(unsigned seed);
+void qkrylov_mlp_destroy(qkrylov_mlp_h m);
+/* Parameter count and FP32 layout: W1[32][128], b1[128], W2[128][128],
+ * b2[128], w3[128], b3 (row index = input). */
+int qkrylov_mlp_num_params(void);
+int qkrylov_mlp_get_params(qkrylov_mlp_h m, float* params_out);
+int qkrylov_mlp_set_params(qkrylov_mlp_h m, const float* params);
+/* o = net(features) for n rows of 32 features (rounded to BF16 on entry);
+ * with grad_out and grad_params non-NULL also grad_params = d(sum_i
+ * grad_out[i] o[i]) / d(params).  Host buffers. */
+int qkrylov_mlp_eval(qkrylov_mlp_h m, const float* features, uint64_t n, const float* grad_out,
+                     float* o_out, float* grad_params);
+/* Features of every basis state of h for theta: dim x 32 floats (the BF16
+ * values the network reads). */
+int qkrylov_mlp_features(qkrylov_hamiltonian_h h, double theta, float* features_out);
+/* Accurate Jacobi-Davidson correction: MINRES on (I-uu^H)(H-theta)(I-uu^H) t
+ * = -r, t orthogonal to u (host complex vectors of dim). */
+int qkrylov_jd_correction_f64(qkrylov_hamiltonian_h h, const double* r_complex,
+                              const double* u_complex, double theta, int max_iter, double rtol,
+                              double* t_complex_out, double* rel_res_out);
+/* Train m on pairs generated by a Jacobi Davidson run (n_eig, max_subspace,
+ * max_iter, tol 1e-8) on h_small.  loss_history: epochs+1 doubles or NULL. */
+int qkrylov_mlp_train_f64(qkrylov_mlp_h m, qkrylov_hamiltonian_h h_small, int n_eig,
+                          int max_subspace, int max_iter, int max_pairs, int epochs,
+                          int batch_pairs, double lr, double* loss_history,
+                          qkrylov_mlp_train_info_t* info);
+/* Davidson with an iteration limit and an optional learned preconditioner
+ * (mlp NULL: the Jacobi diagonal, as qkrylov_davidson_lowest_f64). */
+int qkrylov_davidson_lowest_ex_f64(qkrylov_hamiltonian_h h, int n_eig, int max_subspace,
+                                   double tol, int max_iter, qkrylov_mlp_h mlp,
+                                   double* eigenvalues_out, double* eigenvectors_complex_out,
+                                   qkrylov_davidson_result_c_t* result_info);
 
 /* Continued-fraction coefficients in FP64 (reference src/solvers/dynamics.cpp:12-59). */
 int qkrylov_continued_fraction_coeffs_f64(qkrylov_hamiltonian_h h, const double* phi0_complex,

@@ -410,6 +410,14 @@ class MatrixFreeHamiltonian:
         check(load().qkrylov_hamiltonian_info(self._h, buf, 4096), "info")
         return json.loads(buf.value.decode())
 
+    def autotune(self, nvec: int = 1) -> dict:
+        """Time the apply-kernel variants for blocks of nvec vectors, keep the
+        fastest, and return the timings (ms per apply)."""
+        import json
+        buf = ct.create_string_buffer(4096)
+        check(load().qkrylov_hamiltonian_autotune(self._h, int(nvec), buf, 4096), "autotune")
+        return json.loads(buf.value.decode())
+
     def apply(self, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.complex128)
         d = self.dimension
@@ -487,6 +495,110 @@ def davidson_lowest(H: MatrixFreeHamiltonian, n_eig: int = 1, max_subspace: int 
     check(load().qkrylov_davidson_lowest_f64(H._h, int(n_eig), int(max_subspace), float(tol),
                                              _ptr(ev), None if vecs is None else _ptr(vecs),
                                              ct.byref(info)), "davidson")
+    return DavidsonResult(ev[:k], None if vecs is None else list(vecs[:k]), info.iterations,
+                          bool(info.converged))
+
+
+class MLPPreconditioner:
+    """Learned correction for Davidson: t_i = (1/(H_ii - theta) + net(f_i)) r_i.
+
+    The network (32 -> 128 -> 128 -> 1, ReLU, BF16 weights, FP32 accumulation on
+    MFMA) starts with a zero output layer, i.e. as the Jacobi method.  There is no
+    counterpart in the reference, whose Davidson uses the Jacobi diagonal
+    (src/solvers/davidson.cpp:166-177)."""
+
+    def __init__(self, seed: int = 7):
+        self._h = load().qkrylov_mlp_create(int(seed))
+        if not self._h:
+            raise RuntimeError(f"MLP create failed: {last_error()}")
+        self.last_train = None
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().qkrylov_mlp_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def num_params() -> int:
+        return int(load().qkrylov_mlp_num_params())
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.num_params(), dtype=np.float32)
+        check(load().qkrylov_mlp_get_params(self._h, _ptr(out)), "mlp params")
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params():
+            raise ValueError(f"expected {self.num_params()} parameters")
+        check(load().qkrylov_mlp_set_params(self._h, _ptr(p)), "mlp params")
+
+    def eval(self, features: np.ndarray, grad_out: Optional[np.ndarray] = None):
+        """Network output for rows of 32 features; with grad_out also the
+        parameter gradient of sum(grad_out * o)."""
+        X = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 32)
+        n = X.shape[0]
+        o = np.empty(max(n, 1), dtype=np.float32)
+        g = None
+        go = None
+        if grad_out is not None:
+            go = np.ascontiguousarray(grad_out, dtype=np.float32).reshape(-1)
+            g = np.empty(self.num_params(), dtype=np.float32)
+        check(load().qkrylov_mlp_eval(self._h, _ptr(X), n, None if go is None else _ptr(go),
+                                      _ptr(o), None if g is None else _ptr(g)), "mlp eval")
+        return (o[:n], g) if g is not None else o[:n]
+
+    def train(self, H_small: "MatrixFreeHamiltonian", n_eig: int = 8, max_subspace: int = 48,
+              max_iter: int = 40, max_pairs: int = 256, epochs: int = 30, batch_pairs: int = 8,
+              lr: float = 1e-3):
+        """Generate (residual, accurate correction) pairs with a Jacobi Davidson
+        run on H_small and train; returns the loss history (before, per epoch)."""
+        hist = np.zeros(epochs + 1)
+        info = _lib.MlpTrainInfo()
+        check(load().qkrylov_mlp_train_f64(self._h, H_small._h, int(n_eig), int(max_subspace),
+                                           int(max_iter), int(max_pairs), int(epochs),
+                                           int(batch_pairs), float(lr), _ptr(hist),
+                                           ct.byref(info)), "mlp train")
+        self.last_train = {"pairs": info.pairs, "epochs": info.epochs,
+                           "loss_first": info.loss_first, "loss_last": info.loss_last,
+                           "solve_rel_res": info.solve_rel_res}
+        return hist
+
+
+def mlp_features(H: MatrixFreeHamiltonian, theta: float) -> np.ndarray:
+    """The 32 network inputs of every basis state of H for Ritz value theta."""
+    out = np.empty((max(H.dimension, 1), 32), dtype=np.float32)
+    check(load().qkrylov_mlp_features(H._h, float(theta), _ptr(out)), "mlp features")
+    return out[:H.dimension]
+
+
+def jd_correction(H: MatrixFreeHamiltonian, r: np.ndarray, u: np.ndarray, theta: float,
+                  max_iter: int = 300, rtol: float = 1e-10):
+    """MINRES solution of (I-uu^H)(H-theta)(I-uu^H) t = -r with t orthogonal to u."""
+    r = np.ascontiguousarray(r, dtype=np.complex128)
+    u = np.ascontiguousarray(u, dtype=np.complex128)
+    t = np.empty_like(r)
+    rr = ct.c_double()
+    check(load().qkrylov_jd_correction_f64(H._h, _ptr(r), _ptr(u), float(theta), int(max_iter),
+                                           float(rtol), _ptr(t), ct.byref(rr)), "jd correction")
+    return t, rr.value
+
+
+def davidson_lowest_ex(H: MatrixFreeHamiltonian, n_eig: int = 1, max_subspace: int = 20,
+                       tol: float = 1e-8, max_iter: int = 100,
+                       preconditioner: Optional[MLPPreconditioner] = None,
+                       return_vectors: bool = False):
+    """Davidson with an iteration limit and an optional learned preconditioner
+    (None: the Jacobi diagonal)."""
+    d = H.dimension
+    k = min(n_eig, d)
+    ev = np.zeros(max(n_eig, 1), dtype=np.float64)
+    vecs = np.zeros((max(n_eig, 1), d), dtype=np.complex128) if return_vectors else None
+    info = _lib.DavidsonResultC()
+    check(load().qkrylov_davidson_lowest_ex_f64(
+        H._h, int(n_eig), int(max_subspace), float(tol), int(max_iter),
+        None if preconditioner is None else preconditioner._h, _ptr(ev),
+        None if vecs is None else _ptr(vecs), ct.byref(info)), "davidson")
     return DavidsonResult(ev[:k], None if vecs is None else list(vecs[:k]), info.iterations,
                           bool(info.converged))
 

@@ -49,6 +49,11 @@ class FTLMResultF64(C.Structure):
                 ("internal_energy", c_double), ("specific_heat", c_double)]
 
 
+class MlpTrainInfo(C.Structure):
+    _fields_ = [("pairs", c_int), ("epochs", c_int), ("loss_first", c_double),
+                ("loss_last", c_double), ("solve_rel_res", c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     # reference ABI (include/qkrylov/c_api.h:30-168 of the reference)
@@ -115,6 +120,7 @@ SIGNATURES = {
                                                      c_void_p]),
     "qkrylov_hamiltonian_diagonal_f64": (c_int, [c_void_p, c_void_p]),
     "qkrylov_hamiltonian_info": (c_int, [c_void_p, C.c_char_p, c_size_t]),
+    "qkrylov_hamiltonian_autotune": (c_int, [c_void_p, c_int, C.c_char_p, c_size_t]),
     "qkrylov_lanczos_ground_state_f64": (c_int, [c_void_p, c_int, c_double, c_void_p,
                                                  P(LanczosResultF64), c_void_p]),
     "qkrylov_davidson_lowest_f64": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p,
@@ -132,6 +138,20 @@ SIGNATURES = {
     "qkrylov_hamiltonian_sync": (c_int, [c_void_p]),
     "qkrylov_reference_start_f64": (c_int, [c_u64, C.c_uint, c_void_p]),
     "qkrylov_reference_normals_f64": (c_int, [c_u64, C.c_uint, c_void_p]),
+    # learned Davidson preconditioner
+    "qkrylov_mlp_create": (c_void_p, [C.c_uint]),
+    "qkrylov_mlp_destroy": (None, [c_void_p]),
+    "qkrylov_mlp_num_params": (c_int, []),
+    "qkrylov_mlp_get_params": (c_int, [c_void_p, c_void_p]),
+    "qkrylov_mlp_set_params": (c_int, [c_void_p, c_void_p]),
+    "qkrylov_mlp_eval": (c_int, [c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_void_p]),
+    "qkrylov_mlp_features": (c_int, [c_void_p, c_double, c_void_p]),
+    "qkrylov_jd_correction_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_double,
+                                          c_void_p, P(c_double)]),
+    "qkrylov_mlp_train_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                                      c_double, c_void_p, P(MlpTrainInfo)]),
+    "qkrylov_davidson_lowest_ex_f64": (c_int, [c_void_p, c_int, c_int, c_double, c_int, c_void_p,
+                                               c_void_p, c_void_p, P(DavidsonResultC)]),
 }
 
 _lib = None

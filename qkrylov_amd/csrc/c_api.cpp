@@ -7,6 +7,7 @@
 #include "qkrylov/c_api.h"
 
 #include "qk_hamiltonian.hpp"
+#include "qk_learned.hpp"
 #include "qk_rng.hpp"
 #include "qk_solvers.hpp"
 
@@ -32,6 +33,9 @@ struct qkrylov_opsum_t {
 };
 struct qkrylov_hamiltonian_t {
     std::unique_ptr<Hamiltonian> ptr;
+};
+struct qkrylov_mlp_t {
+    std::unique_ptr<Mlp> ptr;
 };
 
 namespace {
@@ -327,6 +331,16 @@ int qkrylov_hamiltonian_info(qkrylov_hamiltonian_h h, char* buf, size_t len) {
     buf[len - 1] = 0;
     return QKRYLOV_SUCCESS;
 }
+int qkrylov_hamiltonian_autotune(qkrylov_hamiltonian_h h, int nvec, char* buf, size_t len) {
+    if (!hok(h) || nvec <= 0) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const std::string s = h->ptr->autotune(nvec);
+        if (buf && len) {
+            std::strncpy(buf, s.c_str(), len - 1);
+            buf[len - 1] = 0;
+        }
+    });
+}
 int qkrylov_hamiltonian_sync(qkrylov_hamiltonian_h h) {
     if (!hok(h)) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] { QK_HIP(hipStreamSynchronize(h->ptr->stream)); });
@@ -462,6 +476,113 @@ int qkrylov_davidson_lowest_f64(qkrylov_hamiltonian_h h, int n_eig, int max_subs
     if (!hok(h) || !evals || n_eig <= 0) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {
         const DavidsonResult r = davidson_lowest(*h->ptr, n_eig, max_subspace, tol, evecs != nullptr);
+        const size_t k = std::min(size_t(n_eig), r.eigenvalues.size());
+        for (size_t i = 0; i < k; ++i) evals[i] = r.eigenvalues[i];
+        if (info) {
+            info->iterations = r.iterations;
+            info->converged = r.converged ? 1 : 0;
+        }
+        if (evecs) {
+            const u64 d = h->ptr->dim();
+            for (size_t idx = 0; idx < k && idx < r.eigenvectors.size(); ++idx)
+                std::memcpy(evecs + idx * 2 * d, r.eigenvectors[idx].data(), sizeof(cplx) * d);
+        }
+    });
+}
+
+// ---- learned Davidson preconditioner (no reference counterpart) ----
+qkrylov_mlp_h qkrylov_mlp_create(unsigned seed) {
+    return guarded_create<qkrylov_mlp_t>([&] {
+        auto m = std::make_unique<qkrylov_mlp_t>();
+        m->ptr = std::make_unique<Mlp>(seed);
+        return m.release();
+    });
+}
+void qkrylov_mlp_destroy(qkrylov_mlp_h m) { delete m; }
+int qkrylov_mlp_num_params(void) { return kMlpParams; }
+int qkrylov_mlp_get_params(qkrylov_mlp_h m, float* out) {
+    if (!m || !m->ptr || !out) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] { m->ptr->get_params(out); });
+}
+int qkrylov_mlp_set_params(qkrylov_mlp_h m, const float* in) {
+    if (!m || !m->ptr || !in) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] { m->ptr->set_params(in); });
+}
+int qkrylov_mlp_eval(qkrylov_mlp_h m, const float* features, uint64_t n, const float* grad_out,
+                     float* o_out, float* grad_params) {
+    if (!m || !m->ptr || (n && (!features || !o_out))) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] { m->ptr->eval_host(features, n, grad_out, o_out, grad_params); });
+}
+int qkrylov_mlp_features(qkrylov_hamiltonian_h h, double theta, float* out) {
+    if (!hok(h) || !out) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const Hamiltonian& H = *h->ptr;
+        MlpStates s;
+        s.build(H, H.stream);
+        const u64 d = H.dim();
+        DevBuf X(sizeof(u16) * std::max<u64>(d, 1) * kMlpIn);
+        mlp_features(s, theta, X.get<u16>(), H.stream);
+        std::vector<u16> xb(d * kMlpIn);
+        if (d) QK_HIP(hipMemcpyAsync(xb.data(), X.get(), sizeof(u16) * xb.size(), hipMemcpyDeviceToHost, H.stream));
+        QK_HIP(hipStreamSynchronize(H.stream));
+        for (size_t i = 0; i < xb.size(); ++i) out[i] = bf16_to_f32(xb[i]);
+    });
+}
+int qkrylov_jd_correction_f64(qkrylov_hamiltonian_h h, const double* r, const double* u, double theta,
+                              int max_iter, double rtol, double* t_out, double* rel_res) {
+    if (!hok(h) || !r || !u || !t_out || max_iter <= 0) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const Hamiltonian& H = *h->ptr;
+        const u64 d = H.dim();
+        DevBuf b(sizeof(cplx) * std::max<u64>(d, 1) * 3);
+        cplx* rd = b.get<cplx>();
+        cplx* ud = rd + d;
+        cplx* td = ud + d;
+        QK_HIP(hipMemcpyAsync(rd, r, sizeof(cplx) * d, hipMemcpyHostToDevice, H.stream));
+        QK_HIP(hipMemcpyAsync(ud, u, sizeof(cplx) * d, hipMemcpyHostToDevice, H.stream));
+        const double rr = jd_correction(H, rd, ud, theta, td, max_iter, rtol, H.stream);
+        QK_HIP(hipMemcpyAsync(t_out, td, sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
+        QK_HIP(hipStreamSynchronize(H.stream));
+        if (rel_res) *rel_res = rr;
+    });
+}
+int qkrylov_mlp_train_f64(qkrylov_mlp_h m, qkrylov_hamiltonian_h hs, int n_eig, int max_subspace,
+                          int max_iter, int max_pairs, int epochs, int batch_pairs, double lr,
+                          double* loss_history, qkrylov_mlp_train_info_t* info) {
+    if (!m || !m->ptr || !hok(hs) || n_eig <= 0 || max_pairs <= 0 || epochs < 0 || batch_pairs <= 0)
+        return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        MlpTrainConfig cfg;
+        cfg.n_eig = n_eig;
+        cfg.max_subspace = max_subspace;
+        cfg.max_iter = std::max(1, max_iter);
+        cfg.max_pairs = max_pairs;
+        cfg.epochs = epochs;
+        cfg.batch_pairs = batch_pairs;
+        cfg.lr = lr;
+        std::vector<double> hist;
+        const MlpTrainStats st = train_mlp(*m->ptr, *hs->ptr, cfg, &hist);
+        if (loss_history)
+            for (int e = 0; e <= epochs; ++e) loss_history[e] = e < int(hist.size()) ? hist[e] : 0.0;
+        if (info) {
+            info->pairs = st.pairs;
+            info->epochs = st.epochs;
+            info->loss_first = st.loss_first;
+            info->loss_last = st.loss_last;
+            info->solve_rel_res = st.solve_rel_res;
+        }
+    });
+}
+int qkrylov_davidson_lowest_ex_f64(qkrylov_hamiltonian_h h, int n_eig, int max_subspace, double tol,
+                                   int max_iter, qkrylov_mlp_h mlp, double* evals, double* evecs,
+                                   qkrylov_davidson_result_c_t* info) {
+    if (!hok(h) || !evals || n_eig <= 0 || max_iter <= 0 || (mlp && !mlp->ptr))
+        return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        std::unique_ptr<MlpPreconditioner> pre;
+        if (mlp) pre = std::make_unique<MlpPreconditioner>(*mlp->ptr, *h->ptr);
+        const DavidsonResult r =
+            davidson_lowest(*h->ptr, n_eig, max_subspace, tol, evecs != nullptr, pre.get(), max_iter);
         const size_t k = std::min(size_t(n_eig), r.eigenvalues.size());
         for (size_t i = 0; i < k; ++i) evals[i] = r.eigenvalues[i];
         if (info) {

@@ -313,11 +313,139 @@ k_apply_xchg(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y, 
     }
 }
 
+// Bond batches of the single-vector exchange kernel.  Every load of a batch
+// (bond descriptors, row offsets, LDS ranks, the SPL gathers of all NB bonds)
+// is issued before the first one is used, so a wave keeps NB*SPL gathers in
+// flight instead of waiting out one bond at a time.  Contributions are added
+// in bond order, as in the one-bond loop.
+
+// low-half bonds: same row, rank from the LDS table
+template <int SPL, int NB>
+__device__ __forceinline__ void xbonds_lo(const XBond* __restrict__ bonds, const u16* s_jlo,
+                                          const cplx* __restrict__ x, const u64 base,
+                                          const u32 (&lo)[SPL], const bool (&v)[SPL],
+                                          cplx (&acc)[SPL]) {
+    XBond X[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) X[u] = bonds[u];
+    double amp[NB][SPL];
+    u32 idx[NB][SPL];
+    bool ok[NB][SPL];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            const u32 vi = (lo[k] >> X[u].bi) & 1, vj = (lo[k] >> X[u].bj) & 1;
+            ok[u][k] = v[k] && vi != vj;
+            amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
+            idx[u][k] = ok[u][k] ? u32(s_jlo[lo[k] ^ X[u].mlo]) : 0u;
+        }
+    cplx xv[NB][SPL];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            xv[u][k] = cplx{0.0, 0.0};
+            if (ok[u][k]) xv[u][k] = x[base + idx[u][k]];
+        }
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            acc[k].re += amp[u][k] * xv[u][k].re;
+            acc[k].im += amp[u][k] * xv[u][k].im;
+        }
+}
+
+// high-half bonds: another row (wave-uniform), same ranks.  An exchange keeps
+// the popcount of the high half, so the partner row of a valid row is valid.
+template <int SPL, int NB>
+__device__ __forceinline__ void xbonds_hi(const XBond* __restrict__ bonds, const u64* __restrict__ jhi,
+                                          const u32 hi, const u32 L, const cplx* __restrict__ x,
+                                          const u32 (&r)[SPL], const bool (&v)[SPL],
+                                          cplx (&acc)[SPL]) {
+    XBond X[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) X[u] = bonds[u];
+    u64 b2[NB];
+    bool on[NB];
+    double amp[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        const u32 vi = (hi >> (X[u].bi - L)) & 1, vj = (hi >> (X[u].bj - L)) & 1;
+        on[u] = vi != vj;
+        amp[u] = vi ? X[u].a10 : X[u].a01;
+        b2[u] = jhi[hi ^ X[u].mhi];
+    }
+    cplx xv[NB][SPL];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            xv[u][k] = cplx{0.0, 0.0};
+            if (on[u] && v[k]) xv[u][k] = x[b2[u] + r[k]];
+        }
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+        if (on[u]) {
+#pragma unroll
+            for (int k = 0; k < SPL; ++k) {
+                acc[k].re += amp[u] * xv[u][k].re;
+                acc[k].im += amp[u] * xv[u][k].im;
+            }
+        }
+}
+
+// bonds straddling the split: another row and another low-half class
+template <int SPL, int NB>
+__device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, const u64* __restrict__ jhi,
+                                           const u16* s_jlo, const u32 hi, const u32 L,
+                                           const cplx* __restrict__ x, const u32 (&lo)[SPL],
+                                           const bool (&v)[SPL], cplx (&acc)[SPL]) {
+    XBond X[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) X[u] = bonds[u];
+    u64 b2[NB];
+    u32 vj[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+        vj[u] = (hi >> (X[u].bj - L)) & 1;
+        b2[u] = jhi[hi ^ X[u].mhi];
+    }
+    double amp[NB][SPL];
+    u32 idx[NB][SPL];
+    bool ok[NB][SPL];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            const u32 vi = (lo[k] >> X[u].bi) & 1;
+            ok[u][k] = v[k] && b2[u] != ~u64(0) && vi != vj[u];
+            amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
+            idx[u][k] = ok[u][k] ? u32(s_jlo[lo[k] ^ X[u].mlo]) : 0u;
+        }
+    cplx xv[NB][SPL];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            xv[u][k] = cplx{0.0, 0.0};
+            if (ok[u][k]) xv[u][k] = x[b2[u] + idx[u][k]];
+        }
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+            acc[k].re += amp[u][k] * xv[u][k].re;
+            acc[k].im += amp[u][k] * xv[u][k].im;
+        }
+}
+
 // Single-vector exchange kernel with SPL states per lane: a tile spans
-// 64*SPL consecutive ranks of one row and lane l holds ranks r0+l+64k.  The
-// SPL gathers of one bond are independent, so each wave keeps SPL loads in
-// flight where the one-state kernel has one.
-template <int SPL>
+// 64*SPL consecutive ranks of one row and lane l holds ranks r0+l+64k.
+// Bonds are taken NB at a time (xbonds_*), so up to NB*SPL gathers per wave
+// are in flight.
+template <int SPL, int NB>
 __global__ void __launch_bounds__(64 * kXWaves)
 k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -366,63 +494,13 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
             acc[k] = cplx{dg * xa.re, dg * xa.im};
         }
         u32 b = 0;
-        for (; b < p.n_lo; ++b) {
-            const XBond X = p.bonds[b];
-            cplx xv[SPL];
-            double amp[SPL];
-#pragma unroll
-            for (int k = 0; k < SPL; ++k) {
-                const u32 vi = (lo[k] >> X.bi) & 1, vj = (lo[k] >> X.bj) & 1;
-                const bool ok = v[k] && vi != vj;
-                amp[k] = ok ? (vi ? X.a10 : X.a01) : 0.0;
-                xv[k] = cplx{0.0, 0.0};
-                if (ok) xv[k] = x[base + s_jlo[lo[k] ^ X.mlo]];
-            }
-#pragma unroll
-            for (int k = 0; k < SPL; ++k) {
-                acc[k].re += amp[k] * xv[k].re;
-                acc[k].im += amp[k] * xv[k].im;
-            }
-        }
-        for (; b < hi_end; ++b) {
-            const XBond X = p.bonds[b];
-            const u32 vi = (hi >> (X.bi - p.L)) & 1, vj = (hi >> (X.bj - p.L)) & 1;
-            if (vi == vj) continue;                          // uniform
-            const double amp = vi ? X.a10 : X.a01;
-            const u64 base2 = p.jhi[hi ^ X.mhi];
-            cplx xv[SPL];
-#pragma unroll
-            for (int k = 0; k < SPL; ++k) {
-                xv[k] = cplx{0.0, 0.0};
-                if (v[k]) xv[k] = x[base2 + r[k]];
-            }
-#pragma unroll
-            for (int k = 0; k < SPL; ++k) {
-                acc[k].re += amp * xv[k].re;
-                acc[k].im += amp * xv[k].im;
-            }
-        }
-        for (; b < mix_end; ++b) {
-            const XBond X = p.bonds[b];
-            const u32 vj = (hi >> (X.bj - p.L)) & 1;       // uniform
-            const u64 base2 = p.jhi[hi ^ X.mhi];            // uniform
-            if (base2 == ~u64(0)) continue;
-            cplx xv[SPL];
-            double amp[SPL];
-#pragma unroll
-            for (int k = 0; k < SPL; ++k) {
-                const u32 vi = (lo[k] >> X.bi) & 1;
-                const bool ok = v[k] && vi != vj;
-                amp[k] = ok ? (vi ? X.a10 : X.a01) : 0.0;
-                xv[k] = cplx{0.0, 0.0};
-                if (ok) xv[k] = x[base2 + s_jlo[lo[k] ^ X.mlo]];
-            }
-#pragma unroll
-            for (int k = 0; k < SPL; ++k) {
-                acc[k].re += amp[k] * xv[k].re;
-                acc[k].im += amp[k] * xv[k].im;
-            }
-        }
+        for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB>(p.bonds + b, s_jlo, x, base, lo, v, acc);
+        for (; b < p.n_lo; ++b) xbonds_lo<SPL, 1>(p.bonds + b, s_jlo, x, base, lo, v, acc);
+        for (; b + NB <= hi_end; b += NB) xbonds_hi<SPL, NB>(p.bonds + b, p.jhi, hi, p.L, x, r, v, acc);
+        for (; b < hi_end; ++b) xbonds_hi<SPL, 1>(p.bonds + b, p.jhi, hi, p.L, x, r, v, acc);
+        for (; b + NB <= mix_end; b += NB)
+            xbonds_mix<SPL, NB>(p.bonds + b, p.jhi, s_jlo, hi, p.L, x, lo, v, acc);
+        for (; b < mix_end; ++b) xbonds_mix<SPL, 1>(p.bonds + b, p.jhi, s_jlo, hi, p.L, x, lo, v, acc);
 #pragma unroll
         for (int k = 0; k < SPL; ++k)
             if (v[k]) store_nt(&y[base + r[k]], acc[k]);
@@ -498,15 +576,11 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
     if constexpr (sizeof(S) == 4) {
         if (h.xchg) {
             if constexpr (B == 1) {
-                if (h.spl == 4)
-                    hipLaunchKernelGGL((k_apply_xchg_s<4>), dim3(h.xgrid_s), dim3(64 * kXWaves),
-                                       h.xlds_bytes, st, h.xparams_s, x, y);
-                else if (h.spl == 2)
-                    hipLaunchKernelGGL((k_apply_xchg_s<2>), dim3(h.xgrid_s), dim3(64 * kXWaves),
-                                       h.xlds_bytes, st, h.xparams_s, x, y);
-                else
-                    hipLaunchKernelGGL((k_apply_xchg_s<1>), dim3(h.xgrid_s), dim3(64 * kXWaves),
-                                       h.xlds_bytes, st, h.xparams_s, x, y);
+                const cplx* xa = x;
+                cplx* ya = y;
+                void* args[] = {const_cast<XParams*>(&h.xparams_s), &xa, &ya};
+                QK_HIP(hipLaunchKernel(h.xs_fn, dim3(h.xgrid_s), dim3(64 * kXWaves), args,
+                                       size_t(h.xlds_bytes), st));
                 return;
             }
             hipLaunchKernelGGL((k_apply_xchg<B>), dim3(h.xgrid), dim3(64 * kXWaves), h.xlds_bytes,
@@ -521,9 +595,12 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
 template <typename S>
 void launch_apply_s(const Hamiltonian& h, const cplx* x, cplx* y, int nvec, u64 ld,
                     hipStream_t st) {
+    // the exchange path may run blocks as groups of fewer vectors (1: one
+    // single-vector launch per vector), as chosen at set-up
+    const int gmax = (sizeof(S) == 4 && h.xchg) ? h.vec_group : 8;
     int v = 0;
     while (v < nvec) {
-        const int left = nvec - v;
+        const int left = std::min(nvec - v, gmax);
         const cplx* xv = x + u64(v) * ld;
         cplx* yv = y + u64(v) * ld;
         if (left >= 8) { launch_apply_t<S, 8>(h, xv, yv, ld, st); v += 8; }
@@ -738,49 +815,7 @@ void Hamiltonian::setup_exchange() {
     std::vector<cplx> mixtab;
     std::vector<DevGroup> mixdev = pack_groups(mixg, mixtab);
 
-    // Tiles of `span` consecutive ranks, rows in the launch order of tiles_host.
-    auto make_xtiles = [&](u32 span) {
-        std::vector<XTile> xt;
-        u32 last = ~0u;
-        for (const uint2& tl : tiles_host) {
-            const u32 hi = tl.x;
-            if (hi == last) continue;
-            last = hi;
-            const int c = B.hi_cls()[hi];
-            const u32 cnt = B.cls_cnt()[c];
-            for (u32 r0 = 0; r0 < cnt; r0 += span) {
-                XTile T{};
-                T.base = B.jhi()[hi];
-                T.pos0 = B.cls_off()[c] + r0;
-                T.hi = hi;
-                T.r0 = r0;
-                T.nvalid = std::min<u32>(span, cnt - r0);
-                T.dh = dhi[hi];
-                xt.push_back(T);
-            }
-        }
-        return xt;
-    };
-    const std::vector<XTile> xt = make_xtiles(64);
-    upload(d_xtiles, xt.data(), xt.size(), stream);
-    spl = 2;
-    if (const char* e = std::getenv("QKRYLOV_SPL")) spl = std::atoi(e);
-    if (spl != 1 && spl != 2 && spl != 4) spl = 2;
-    std::vector<XTile> xts = make_xtiles(64u * u32(spl));
-    // Column-block order: inside a class, all rows visit rank window
-    // [w*cb, (w+1)*cb) before any row moves to the next window, so the
-    // high-half gathers (same ranks, other rows) find their rows' windows
-    // recently touched.  cb = 0 keeps plain row order.
-    u32 cb = 0;
-    if (const char* e = std::getenv("QKRYLOV_COLBLOCK")) cb = u32(std::max(0, std::atoi(e)));
-    if (cb > 0) {
-        std::stable_sort(xts.begin(), xts.end(), [&](const XTile& a, const XTile& b) {
-            const int ca = B.hi_cls()[a.hi], cb2 = B.hi_cls()[b.hi];
-            if (ca != cb2) return ca < cb2;
-            return a.r0 / cb < b.r0 / cb;
-        });
-    }
-    upload(d_xtiles_s, xts.data(), xts.size(), stream);
+    dhi_host = dhi;
     std::vector<XBond> bonds = lo_b;
     bonds.insert(bonds.end(), hi_b.begin(), hi_b.end());
     bonds.insert(bonds.end(), mix_b.begin(), mix_b.end());
@@ -793,7 +828,6 @@ void Hamiltonian::setup_exchange() {
     QK_HIP(hipStreamSynchronize(stream));
 
     XParams& x = xparams;
-    x.xtiles = d_xtiles.get<XTile>();
     x.jlo = params.jlo;
     x.jhi = params.jhi;
     x.hi_cls = params.hi_cls;
@@ -806,7 +840,6 @@ void Hamiltonian::setup_exchange() {
     x.bonds = d_xbonds.get<XBond>();
     x.dmix = d_dmix.get<DevGroup>();
     x.dmix_tab = d_dmixtab.get<cplx>();
-    x.ntiles = params.ntiles;
     x.n_lo = u32(lo_b.size());
     x.n_hi = u32(hi_b.size());
     x.n_mix = u32(mix_b.size());
@@ -815,20 +848,153 @@ void Hamiltonian::setup_exchange() {
     x.nlo = params.nlo;
     x.L = L;
     xlds_bytes = int(x.ndmix_tab * sizeof(cplx) + ((x.nlo * sizeof(u16) + 15) / 16) * 16);
+    const std::vector<XTile> xt = xtiles_for(64);
+    xgrid = grid_for(xlds_bytes, xt.size(), &x.tiles_per_xcd, kXWaves);
     x.ntiles = u32(xt.size());
-    xgrid = grid_for(xlds_bytes, x.ntiles, &x.tiles_per_xcd, kXWaves);
-    xparams_s = x;
-    xparams_s.xtiles = d_xtiles_s.get<XTile>();
-    xparams_s.ntiles = u32(xts.size());
-    xgrid_s = grid_for(xlds_bytes, xparams_s.ntiles, &xparams_s.tiles_per_xcd, kXWaves);
+    upload(d_xtiles, xt.data(), xt.size(), stream);
+    x.xtiles = d_xtiles.get<XTile>();
+    QK_HIP(hipStreamSynchronize(stream));
     if (xlds_bytes > 64 * 1024) {
         for (const void* f : {(const void*)k_apply_xchg<1>, (const void*)k_apply_xchg<2>,
-                              (const void*)k_apply_xchg<4>, (const void*)k_apply_xchg<8>,
-                              (const void*)k_apply_xchg_s<1>, (const void*)k_apply_xchg_s<2>,
-                              (const void*)k_apply_xchg_s<4>})
+                              (const void*)k_apply_xchg<4>, (const void*)k_apply_xchg<8>})
             QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
     }
+    // defaults (measured on the N=28 chain); autotune() may change them
+    int s0 = 2, nb0 = 1;
+    vec_group = 2;
+    if (const char* e = std::getenv("QKRYLOV_SPL")) s0 = std::atoi(e);
+    if (const char* e = std::getenv("QKRYLOV_BOND_BATCH")) nb0 = std::atoi(e);
+    if (const char* e = std::getenv("QKRYLOV_VEC_GROUP")) vec_group = std::atoi(e);
+    if (vec_group != 1 && vec_group != 2 && vec_group != 4 && vec_group != 8) vec_group = 2;
     xchg = true;
+    set_single_kernel(s0, nb0);
+    xchg = true;
+}
+
+// Tiles of `span` consecutive ranks of one row, rows in the order of
+// tiles_host (class-major).
+std::vector<XTile> Hamiltonian::xtiles_for(u32 span) const {
+    const Basis& B = *basis;
+    std::vector<XTile> xt;
+    u32 last = ~0u;
+    for (const uint2& tl : tiles_host) {
+        const u32 hi = tl.x;
+        if (hi == last) continue;
+        last = hi;
+        const int c = B.hi_cls()[hi];
+        const u32 cnt = B.cls_cnt()[c];
+        for (u32 r0 = 0; r0 < cnt; r0 += span) {
+            XTile T{};
+            T.base = B.jhi()[hi];
+            T.pos0 = B.cls_off()[c] + r0;
+            T.hi = hi;
+            T.r0 = r0;
+            T.nvalid = std::min<u32>(span, cnt - r0);
+            T.dh = dhi_host[hi];
+            xt.push_back(T);
+        }
+    }
+    return xt;
+}
+
+// Single-vector exchange kernel with s states per lane and bonds in batches
+// of nb (xbonds_*): tiles of 64*s ranks, grid, LDS limit.
+void Hamiltonian::set_single_kernel(int s, int nb) {
+    if (!xchg) return;
+    if (s != 1 && s != 2 && s != 4) s = 2;
+    if (nb != 1 && nb != 2 && nb != 4) nb = 1;
+    if (s == 4 && nb == 4) nb = 2;   // that pair spills registers
+    if (s != spl || !d_xtiles_s.get()) {
+        const std::vector<XTile> xts = xtiles_for(64u * u32(s));
+        xparams_s = xparams;
+        xgrid_s = grid_for(xlds_bytes, xts.size(), &xparams_s.tiles_per_xcd, kXWaves);
+        xparams_s.ntiles = u32(xts.size());
+        upload(d_xtiles_s, xts.data(), xts.size(), stream);
+        xparams_s.xtiles = d_xtiles_s.get<XTile>();
+        QK_HIP(hipStreamSynchronize(stream));
+    }
+    const void* table[3][3] = {
+        {(const void*)k_apply_xchg_s<1, 1>, (const void*)k_apply_xchg_s<1, 2>, (const void*)k_apply_xchg_s<1, 4>},
+        {(const void*)k_apply_xchg_s<2, 1>, (const void*)k_apply_xchg_s<2, 2>, (const void*)k_apply_xchg_s<2, 4>},
+        {(const void*)k_apply_xchg_s<4, 1>, (const void*)k_apply_xchg_s<4, 2>, nullptr}};
+    const auto slot = [](int v) { return v == 1 ? 0 : v == 2 ? 1 : 2; };
+    spl = s;
+    bond_batch = nb;
+    xs_fn = table[slot(s)][slot(nb)];
+    if (xlds_bytes > 64 * 1024)
+        QK_HIP(hipFuncSetAttribute(xs_fn, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
+}
+
+// Autotuner: times every single-vector variant (states per lane x bond
+// batch) and every block grouping on this Hamiltonian with the caller's
+// block width, keeps the fastest, and returns the timings as JSON.
+std::string Hamiltonian::autotune(int nvec) {
+    std::ostringstream os;
+    os << "{";
+    if (!xchg || dim() == 0) {
+        os << "\"tuned\": 0}";
+        return os.str();
+    }
+    nvec = std::max(1, std::min(nvec, 8));
+    const u64 d = dim();
+    DevBuf x(sizeof(cplx) * d * nvec), y(sizeof(cplx) * d * nvec);
+    QK_HIP(hipMemsetAsync(x.get(), 0, sizeof(cplx) * d * nvec, stream));
+    hipEvent_t e0, e1;
+    QK_HIP(hipEventCreate(&e0));
+    QK_HIP(hipEventCreate(&e1));
+    auto time_ms = [&](int nv, int reps) {
+        apply(x.get<cplx>(), y.get<cplx>(), nv, d, stream);
+        QK_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < reps; ++r) apply(x.get<cplx>(), y.get<cplx>(), nv, d, stream);
+        QK_HIP(hipEventRecord(e1, stream));
+        QK_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        QK_HIP(hipEventElapsedTime(&ms, e0, e1));
+        return double(ms) / reps;
+    };
+    const int reps = d > (u64(1) << 24) ? 3 : 10;
+    double best = 1e300;
+    int bs = spl, bn = bond_batch;
+    os << "\"single\": {";
+    bool first = true;
+    for (int s : {1, 2, 4})
+        for (int nb : {1, 2, 4}) {
+            if (s == 4 && nb == 4) continue;
+            set_single_kernel(s, nb);
+            const double ms = time_ms(1, reps);
+            os << (first ? "" : ", ") << "\"" << s << "x" << nb << "\": " << ms;
+            first = false;
+            if (ms < best) {
+                best = ms;
+                bs = s;
+                bn = nb;
+            }
+        }
+    set_single_kernel(bs, bn);
+    os << "}, \"states_per_lane\": " << bs << ", \"bond_batch\": " << bn;
+    if (nvec > 1) {
+        double bestv = 1e300;
+        int bg = vec_group;
+        os << ", \"groups\": {";
+        first = true;
+        for (int g : {1, 2, 4, 8}) {
+            if (g > nvec && g != 1) continue;
+            vec_group = g;
+            const double ms = time_ms(nvec, reps);
+            os << (first ? "" : ", ") << "\"" << g << "\": " << ms;
+            first = false;
+            if (ms < bestv) {
+                bestv = ms;
+                bg = g;
+            }
+        }
+        vec_group = bg;
+        os << "}, \"vec_group\": " << bg;
+    }
+    os << ", \"tuned\": 1}";
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return os.str();
 }
 
 Hamiltonian::~Hamiltonian() {
@@ -889,7 +1055,8 @@ std::string Hamiltonian::info() const {
        << ", \"lds\": " << lds_bytes << ", \"state32\": " << (state32 ? 1 : 0)
        << ", \"max_support\": " << prog.max_support << ", \"exchange_path\": " << (xchg ? 1 : 0)
        << ", \"xbonds\": [" << xparams.n_lo << ", " << xparams.n_hi << ", " << xparams.n_mix
-       << "], \"diag_mixed\": " << xparams.n_dmix << "}";
+       << "], \"diag_mixed\": " << xparams.n_dmix << ", \"states_per_lane\": " << spl
+       << ", \"bond_batch\": " << bond_batch << ", \"vec_group\": " << vec_group << ", \"xgrid\": " << xgrid_s << "}";
     return os.str();
 }
 

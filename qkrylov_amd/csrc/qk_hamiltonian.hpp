@@ -109,7 +109,12 @@ struct Hamiltonian {
     void sz_q(const cplx* psi, cplx* phi, int nq, u64 ld, hipStream_t st) const;
 
     std::string info() const;
+    // Time the exchange-path kernel variants for blocks of nvec vectors and
+    // keep the fastest; returns the timings (JSON).
+    std::string autotune(int nvec);
     void setup_exchange();
+    void set_single_kernel(int states_per_lane, int bond_batch);
+    std::vector<XTile> xtiles_for(u32 span) const;
 
     std::shared_ptr<const Basis> basis;
     SiteKind site;
@@ -124,7 +129,11 @@ struct Hamiltonian {
     DevBuf d_xbonds, d_dlo, d_dhi, d_dmix, d_dmixtab, d_xtiles, d_xtiles_s;   // exchange path
     XParams xparams_s{};        // single-vector kernel, spl states per lane
     int xgrid_s = 0;
-    int spl = 2;
+    int spl = 0;                   // states per lane of the single-vector kernel
+    int bond_batch = 1;            // bonds whose loads one wave issues together
+    int vec_group = 2;             // vectors per block-kernel launch (exchange path)
+    const void* xs_fn = nullptr;   // the single-vector kernel for (spl, bond_batch)
+    std::vector<double> dhi_host;  // high-half diagonal per row
     std::vector<uint2> tiles_host;   // (hi, r0) per tile, launch order
     ApplyParams params{};
     ApplyParams dparams{};

@@ -1,0 +1,49 @@
+// Training and use of the MLP preconditioner for Davidson (qk_mlp.hpp).
+#pragma once
+
+#include "qk_mlp.hpp"
+#include "qk_solvers.hpp"
+
+#include <vector>
+
+namespace qk {
+
+struct MlpTrainConfig {
+    int n_eig = 8;            // Davidson run that generates the pairs
+    int max_subspace = 48;
+    double tol = 1e-8;
+    int max_iter = 40;
+    int max_pairs = 256;
+    int solve_iters = 300;    // MINRES iterations per accurate correction
+    double solve_rtol = 1e-8;
+    int epochs = 30;
+    int batch_pairs = 8;
+    double lr = 1e-3;
+    unsigned seed = 7;
+};
+
+// MINRES solve of (I - uu^H)(H - theta)(I - uu^H) t = -r, t orthogonal to u
+// (all device vectors of H.dim()); returns the final relative residual.
+double jd_correction(const Hamiltonian& H, const cplx* r, const cplx* u, double theta, cplx* t,
+                     int max_it, double rtol, hipStream_t st);
+
+// Generate pairs on Hs with a Jacobi Davidson run, compute their accurate
+// corrections and train `net`; loss_history gets the mean loss before
+// training and after each epoch.
+MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& cfg,
+                        std::vector<double>* loss_history);
+
+// t = (jacobi + net(features)) * r for Davidson on H.
+class MlpPreconditioner : public Preconditioner {
+public:
+    MlpPreconditioner(const Mlp& net, const Hamiltonian& H);
+    void apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda, const cplx* u,
+               cplx* t, hipStream_t st) override;
+
+private:
+    const Mlp& net_;
+    MlpStates states_;
+    DevBuf X_, o_;
+};
+
+}  // namespace qk

@@ -1,0 +1,676 @@
+// MLP preconditioner kernels on gfx950: BF16 MFMA (v_mfma_f32_16x16x32_bf16)
+// with FP32 accumulation for the forward pass, the activation gradients and
+// the weight gradients; FP32 Adam on a master copy; FP64 loss reductions.
+//
+// MFMA operand layout (16x16x32, wave64; lane l, g = l / 16, c = l % 16):
+//   A (16 x 32):  lane holds A[c][8g .. 8g+7]
+//   B (32 x 16):  lane holds B[8g .. 8g+7][c]
+//   D (16 x 16):  lane holds D[4g .. 4g+3][c]
+// so an operand is 8 consecutive BF16 along k, one 16-byte load, when the
+// source is stored k-contiguous (X and activations row-major, weights
+// transposed to [n][k]).
+#include "qk_mlp.hpp"
+
+#include "qk_hamiltonian.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <thread>
+
+namespace qk {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = kMlpIn, kHid = kMlpHid;
+constexpr int kRows = 64;          // rows per workgroup tile (4 waves x 16)
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ u16 d_f2bf(float f) {
+    u32 u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u16(u >> 16);
+}
+__device__ __forceinline__ float d_bf2f(u16 b) { return __uint_as_float(u32(b) << 16); }
+
+__device__ __forceinline__ f32x4 mfma(const u16x8 a, const u16x8 b, const f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+__device__ __forceinline__ u16x8 ld8(const u16* p) { return *reinterpret_cast<const u16x8*>(p); }
+
+// ---------------------------------------------------------------------------
+// forward: o = w3 . relu(W2^T relu(W1^T x + b1) + b2) + b3, per row.
+// Activations are rounded to BF16 (they feed the next MFMA); h2 is rounded
+// too, so backward sees exactly what forward used.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1t,
+              const u16* __restrict__ w2t, const u16* __restrict__ w3b,
+              const float* __restrict__ prm, float* __restrict__ out, u16* __restrict__ h1s,
+              u16* __restrict__ h2s) {
+    __shared__ __align__(16) u16 s_w1t[kHid * kIn];
+    __shared__ __align__(16) u16 s_w2t[kHid * kHid];
+    __shared__ __align__(16) u16 s_h1[kRows * kHid];
+    __shared__ float s_b1[kHid], s_b2[kHid], s_w3[kHid];
+    for (int i = threadIdx.x; i < kHid * kIn / 8; i += kThreads)
+        reinterpret_cast<u16x8*>(s_w1t)[i] = reinterpret_cast<const u16x8*>(w1t)[i];
+    for (int i = threadIdx.x; i < kHid * kHid / 8; i += kThreads)
+        reinterpret_cast<u16x8*>(s_w2t)[i] = reinterpret_cast<const u16x8*>(w2t)[i];
+    for (int i = threadIdx.x; i < kHid; i += kThreads) {
+        s_b1[i] = prm[kMlpB1 + i];
+        s_b2[i] = prm[kMlpB2 + i];
+        s_w3[i] = d_bf2f(w3b[i]);
+    }
+    const float b3 = prm[kMlpB3];
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    u16* h1w = s_h1 + wave * 16 * kHid;
+    for (u64 tile = blockIdx.x; tile * kRows < n; tile += gridDim.x) {
+        const u64 m0 = tile * kRows + u64(wave) * 16;
+        // layer 1: [16 x 32] x [32 x 128]
+        const u64 ra = m0 + c;
+        u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (ra < n) a = ld8(X + ra * kIn + 8 * g);
+        f32x4 acc[8];
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) {
+            acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+            acc[nb] = mfma(a, ld8(s_w1t + (nb * 16 + c) * kIn + 8 * g), acc[nb]);
+        }
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) {
+            const int col = nb * 16 + c;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                h1w[(4 * g + i) * kHid + col] = d_f2bf(fmaxf(acc[nb][i] + s_b1[col], 0.f));
+        }
+        __syncthreads();
+        if (h1s) {
+            for (int q = lane; q < 16 * kHid / 8; q += 64) {
+                const int rr = q / (kHid / 8), cc = (q % (kHid / 8)) * 8;
+                if (m0 + rr < n)
+                    *reinterpret_cast<u16x8*>(h1s + (m0 + rr) * kHid + cc) =
+                        *reinterpret_cast<const u16x8*>(h1w + rr * kHid + cc);
+            }
+        }
+        // layer 2: [16 x 128] x [128 x 128]
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < kHid / 32; ++kb) {
+            const u16x8 a2 = ld8(h1w + c * kHid + kb * 32 + 8 * g);
+#pragma unroll
+            for (int nb = 0; nb < 8; ++nb)
+                acc[nb] = mfma(a2, ld8(s_w2t + (nb * 16 + c) * kHid + kb * 32 + 8 * g), acc[nb]);
+        }
+        // layer 3 and the saved h2
+        float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) {
+            const int col = nb * 16 + c;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const u16 hb = d_f2bf(fmaxf(acc[nb][i] + s_b2[col], 0.f));
+                part[i] += d_bf2f(hb) * s_w3[col];
+                const u64 row = m0 + 4 * g + i;
+                if (h2s && row < n) h2s[row * kHid + col] = hb;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) part[i] += __shfl_xor(part[i], off);
+        if (c == 0) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const u64 row = m0 + 4 * g + i;
+                if (row < n) out[row] = part[i] + b3;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// backward, activation part.  gz2 = go * w3 * [h2 > 0] and
+// gz1 = (gz2 W2^T) * [h1 > 0], both kept as BF16 for the weight-gradient
+// products; per-workgroup FP32 partials of gw3 = h2^T go and gb3 = sum go.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+k_mlp_backward_act(const u16* __restrict__ h1s, const u16* __restrict__ h2s,
+                   const float* __restrict__ go, const u64 n, const u16* __restrict__ w2k,
+                   const u16* __restrict__ w3b, u16* __restrict__ gz2s, u16* __restrict__ gz1s,
+                   float* __restrict__ part3) {
+    __shared__ __align__(16) u16 s_w2k[kHid * kHid];   // W2[k][n]: B operand of gz2 W2^T
+    __shared__ float s_w3[kHid];
+    __shared__ float s_gw3[4][kHid];
+    __shared__ float s_gb3[4];
+    for (int i = threadIdx.x; i < kHid * kHid / 8; i += kThreads)
+        reinterpret_cast<u16x8*>(s_w2k)[i] = reinterpret_cast<const u16x8*>(w2k)[i];
+    for (int i = threadIdx.x; i < kHid; i += kThreads) s_w3[i] = d_bf2f(w3b[i]);
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    // gw3 partial: lane accumulates columns 32kb + 8g + j of its row c
+    float gw3[kHid / 32][8];
+#pragma unroll
+    for (int kb = 0; kb < kHid / 32; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) gw3[kb][j] = 0.f;
+    float gb3 = 0.f;
+    for (u64 tile = blockIdx.x; tile * kRows < n; tile += gridDim.x) {
+        const u64 m0 = tile * kRows + u64(wave) * 16;
+        const u64 ra = m0 + c;
+        const bool rv = ra < n;
+        const float gor = rv ? go[ra] : 0.f;
+        if (g == 0) gb3 += gor;
+        f32x4 acc[8];
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kb = 0; kb < kHid / 32; ++kb) {
+            const int k0 = kb * 32 + 8 * g;
+            u16x8 h2v = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (rv) h2v = ld8(h2s + ra * kHid + k0);
+            u16x8 a;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const float h = d_bf2f(h2v[j]);
+                gw3[kb][j] += gor * h;
+                a[j] = d_f2bf(h > 0.f ? gor * s_w3[k0 + j] : 0.f);
+            }
+            if (rv) *reinterpret_cast<u16x8*>(gz2s + ra * kHid + k0) = a;
+            // gh1[m][k] = sum_n gz2[m][n] W2[k][n]
+#pragma unroll
+            for (int nb = 0; nb < 8; ++nb)
+                acc[nb] = mfma(a, ld8(s_w2k + (nb * 16 + c) * kHid + k0), acc[nb]);
+        }
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) {
+            const int col = nb * 16 + c;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const u64 row = m0 + 4 * g + i;
+                if (row < n) {
+                    const float h1 = d_bf2f(h1s[row * kHid + col]);
+                    gz1s[row * kHid + col] = d_f2bf(h1 > 0.f ? acc[nb][i] : 0.f);
+                }
+            }
+        }
+    }
+    // reduce gw3 over the 16 rows (lanes c) of each column group, then waves
+#pragma unroll
+    for (int kb = 0; kb < kHid / 32; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) gw3[kb][j] += __shfl_xor(gw3[kb][j], off);
+    for (int off = 1; off < 16; off <<= 1) gb3 += __shfl_xor(gb3, off);
+    if (c == 0) {
+#pragma unroll
+        for (int kb = 0; kb < kHid / 32; ++kb)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s_gw3[wave][kb * 32 + 8 * g + j] = gw3[kb][j];
+    }
+    if (lane == 0) s_gb3[wave] = gb3;
+    __syncthreads();
+    float* dst = part3 + size_t(blockIdx.x) * (kHid + 1);
+    for (int i = threadIdx.x; i < kHid; i += kThreads)
+        dst[i] = ((s_gw3[0][i] + s_gw3[1][i]) + s_gw3[2][i]) + s_gw3[3][i];
+    if (threadIdx.x == 0) dst[kHid] = ((s_gb3[0] + s_gb3[1]) + s_gb3[2]) + s_gb3[3];
+}
+
+// ---------------------------------------------------------------------------
+// weight gradients: G[K x 128] = A^T D and colsum[128] = sum_m D[m][:], per
+// workgroup over its rows, A (n x K) and D (n x 128) BF16 row-major.  Tiles
+// of 32 rows are transposed into LDS so both operands are k-contiguous.
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ void __launch_bounds__(kThreads)
+k_mlp_wgrad(const u16* __restrict__ A, const u16* __restrict__ D, const u64 n,
+            float* __restrict__ part) {
+    constexpr int TR = 32;                       // rows per tile = MFMA K
+    constexpr int NBLK = (K / 16) * (kHid / 16); // 16x16 output blocks
+    constexpr int PER = NBLK / 4;                // per wave
+    __shared__ __align__(16) u16 s_at[K * TR];
+    __shared__ __align__(16) u16 s_dt[kHid * TR];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+    f32x4 acc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    float colsum = 0.f;   // thread t < 128 owns column t
+    for (u64 tile = blockIdx.x; tile * TR < n; tile += gridDim.x) {
+        const u64 m0 = tile * TR;
+        __syncthreads();
+        for (int q = threadIdx.x; q < TR * K / 8; q += kThreads) {
+            const int rr = q / (K / 8), kk = (q % (K / 8)) * 8;
+            u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (m0 + rr < n) v = ld8(A + (m0 + rr) * K + kk);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s_at[(kk + j) * TR + rr] = v[j];
+        }
+        for (int q = threadIdx.x; q < TR * kHid / 8; q += kThreads) {
+            const int rr = q / (kHid / 8), kk = (q % (kHid / 8)) * 8;
+            u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (m0 + rr < n) v = ld8(D + (m0 + rr) * kHid + kk);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s_dt[(kk + j) * TR + rr] = v[j];
+        }
+        __syncthreads();
+        if (threadIdx.x < kHid) {
+            float s = 0.f;
+            for (int rr = 0; rr < TR; ++rr) s += d_bf2f(s_dt[threadIdx.x * TR + rr]);
+            colsum += s;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int blk = wave * PER + q;
+            const int kb = blk / (kHid / 16), nb = blk % (kHid / 16);
+            acc[q] = mfma(ld8(s_at + (kb * 16 + c) * TR + 8 * g), ld8(s_dt + (nb * 16 + c) * TR + 8 * g),
+                          acc[q]);
+        }
+    }
+    float* dst = part + size_t(blockIdx.x) * (K * kHid + kHid);
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int blk = wave * PER + q;
+        const int kb = blk / (kHid / 16), nb = blk % (kHid / 16);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dst[(kb * 16 + 4 * g + i) * kHid + nb * 16 + c] = acc[q][i];
+    }
+    if (threadIdx.x < kHid) dst[K * kHid + threadIdx.x] = colsum;
+}
+
+// grads[off + j] = sum over workgroups b (in order) of part[b][j], j < len
+__global__ void k_mlp_sum_parts(const float* __restrict__ part, int nparts, int stride, int len,
+                                float* __restrict__ dst) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    float s = 0.f;
+    for (int b = 0; b < nparts; ++b) s += part[size_t(b) * stride + j];
+    dst[j] = s;
+}
+
+__global__ void k_mlp_adam(float* __restrict__ p, float* __restrict__ m1, float* __restrict__ m2,
+                           const float* __restrict__ g, int n, float lr, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float a = 0.9f * m1[i] + 0.1f * gi;
+    const float b = 0.999f * m2[i] + 0.001f * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    p[i] -= lr * (a / c1) / (sqrtf(b / c2) + 1e-8f);
+}
+
+// BF16 copies: W1^T[n][k], W2^T[n][k], W2[k][n], w3
+__global__ void k_mlp_cast(const float* __restrict__ p, u16* __restrict__ w1t, u16* __restrict__ w2t,
+                           u16* __restrict__ w2k, u16* __restrict__ w3b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kIn * kHid) {
+        const int k = i / kHid, nn = i % kHid;
+        w1t[nn * kIn + k] = d_f2bf(p[kMlpW1 + i]);
+    }
+    if (i < kHid * kHid) {
+        const int k = i / kHid, nn = i % kHid;
+        const u16 b = d_f2bf(p[kMlpW2 + i]);
+        w2t[nn * kHid + k] = b;
+        w2k[i] = b;
+    }
+    if (i < kHid) w3b[i] = d_f2bf(p[kMlpW3 + i]);
+}
+
+int mlp_grid(u64 n, u64 rows_per_tile) {
+    const u64 tiles = (n + rows_per_tile - 1) / rows_per_tile;
+    return int(std::max<u64>(1, std::min<u64>(tiles, u64(device_cu_count()) * 2)));
+}
+
+// ---------------------------------------------------------------------------
+// features, combine, loss
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double jac_den(double d, double theta) {
+    // the reference's clamp of the Jacobi denominator (davidson.cpp:166-177)
+    double den = d - theta;
+    if (fabs(den) < 1e-10) den = den >= 0.0 ? 1e-10 : -1e-10;
+    return den;
+}
+
+__global__ void k_mlp_features(const double* __restrict__ diag, const float* __restrict__ stat,
+                               const u64 n, const double theta, const double inv_sites,
+                               u16* __restrict__ X) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = diag[i];
+    const double den = jac_den(d, theta);
+    float f[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; ++k) f[k] = 0.f;
+    f[0] = float(tanh((d - theta) * 0.25));
+    f[1] = float(fmin(fmax(1.0 / den, -4.0), 4.0) * 0.25);
+    f[2] = float(theta * inv_sites);
+    f[3] = float(d * inv_sites);
+    f[4] = stat[i * 5 + 4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) f[5 + k] = stat[i * 5 + k];
+    f[9] = float(tanh(d - theta));
+    u16x8 v[kIn / 8];
+#pragma unroll
+    for (int q = 0; q < kIn / 8; ++q)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[q][j] = d_f2bf(f[q * 8 + j]);
+#pragma unroll
+    for (int q = 0; q < kIn / 8; ++q) reinterpret_cast<u16x8*>(X + i * kIn)[q] = v[q];
+}
+
+__global__ void k_mlp_combine(const double* __restrict__ diag, const u64 n, const double theta,
+                              const float* __restrict__ o, const cplx* __restrict__ r,
+                              cplx* __restrict__ t) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double w = 1.0 / jac_den(diag[i], theta) + double(o[i]);
+    t[i] = cplx{w * r[i].re, w * r[i].im};
+}
+
+// per (pair, block) partial sums of A = sum conj(ts) w r, c = sum conj(u) w r,
+// S = sum w^2 |r|^2 (w = jacobi + o)
+constexpr int kLossT = 256;
+__global__ void __launch_bounds__(kLossT)
+k_mlp_loss_part(const double* __restrict__ diag, const u64 n, const double* __restrict__ theta,
+                const float* __restrict__ o, const cplx* __restrict__ r, const cplx* __restrict__ ts,
+                const cplx* __restrict__ u, const int nblk, double* __restrict__ part) {
+    const int p = blockIdx.y;
+    const double th = theta[p];
+    double s[5] = {0, 0, 0, 0, 0};
+    for (u64 i = u64(blockIdx.x) * kLossT + threadIdx.x; i < n; i += u64(nblk) * kLossT) {
+        const u64 k = u64(p) * n + i;
+        const double w = 1.0 / jac_den(diag[i], th) + double(o[k]);
+        const cplx rr = r[k], tt = ts[k], uu = u[k];
+        // conj(tt) * rr and conj(uu) * rr
+        const double ar = tt.re * rr.re + tt.im * rr.im, ai = tt.re * rr.im - tt.im * rr.re;
+        const double cr = uu.re * rr.re + uu.im * rr.im, ci = uu.re * rr.im - uu.im * rr.re;
+        s[0] += w * ar;
+        s[1] += w * ai;
+        s[2] += w * cr;
+        s[3] += w * ci;
+        s[4] += w * w * (rr.re * rr.re + rr.im * rr.im);
+    }
+    __shared__ double sh[5][kLossT / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        double v = s[q];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) sh[q][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        double v = 0.0;
+        for (int w = 0; w < kLossT / 64; ++w) v += sh[threadIdx.x][w];
+        part[(size_t(p) * nblk + blockIdx.x) * 5 + threadIdx.x] = v;
+    }
+}
+
+// go[k] = (1/P) dL_p/dw_k with L_p = 1 - |A|^2 / (B C), B = S - |c|^2
+__global__ void k_mlp_loss_grad(const double* __restrict__ diag, const u64 n,
+                                const double* __restrict__ theta, const float* __restrict__ o,
+                                const cplx* __restrict__ r, const cplx* __restrict__ ts,
+                                const cplx* __restrict__ u, const double* __restrict__ coef,
+                                const int P, float* __restrict__ go) {
+    const u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= u64(P) * n) return;
+    const int p = int(k / n);
+    const u64 i = k % n;
+    const double* cf = coef + size_t(p) * 8;   // Ar, Ai, cr, ci, alpha, beta
+    const double w = 1.0 / jac_den(diag[i], theta[p]) + double(o[k]);
+    const cplx rr = r[k], tt = ts[k], uu = u[k];
+    const double ar = tt.re * rr.re + tt.im * rr.im, ai = tt.re * rr.im - tt.im * rr.re;
+    const double cr = uu.re * rr.re + uu.im * rr.im, ci = uu.re * rr.im - uu.im * rr.re;
+    // Re(conj(A) * (conj(ts) r)) and Re(conj(c) * (conj(u) r))
+    const double reA = cf[0] * ar + cf[1] * ai;
+    const double reC = cf[2] * cr + cf[3] * ci;
+    const double r2 = rr.re * rr.re + rr.im * rr.im;
+    const double gw = cf[4] * reA + cf[5] * (w * r2 - reC);
+    go[k] = float(gw / double(P));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Mlp
+// ---------------------------------------------------------------------------
+Mlp::Mlp(unsigned seed) {
+    QK_HIP(hipStreamCreate(&st_));
+    params.resize(sizeof(float) * kMlpParams);
+    m1.resize(sizeof(float) * kMlpParams);
+    m2.resize(sizeof(float) * kMlpParams);
+    w1t.resize(sizeof(u16) * kIn * kHid);
+    w2t.resize(sizeof(u16) * kHid * kHid);
+    w2k.resize(sizeof(u16) * kHid * kHid);
+    w3b.resize(sizeof(u16) * kHid);
+    // He-uniform hidden layers; the output layer starts at zero, so the
+    // untrained network is the Jacobi preconditioner.
+    std::vector<float> h(kMlpParams, 0.0f);
+    std::mt19937 rng(seed);
+    std::uniform_real_distribution<float> u(-1.0f, 1.0f);
+    const float a1 = std::sqrt(6.0f / kIn), a2 = std::sqrt(6.0f / kHid);
+    for (int i = 0; i < kIn * kHid; ++i) h[kMlpW1 + i] = a1 * u(rng);
+    for (int i = 0; i < kHid * kHid; ++i) h[kMlpW2 + i] = a2 * u(rng);
+    set_params(h.data());
+}
+
+Mlp::~Mlp() {
+    if (st_) {
+        (void)hipStreamSynchronize(st_);
+        (void)hipStreamDestroy(st_);
+    }
+}
+
+void Mlp::get_params(float* host) const {
+    QK_HIP(hipMemcpyAsync(host, params.get(), sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st_));
+    QK_HIP(hipStreamSynchronize(st_));
+}
+
+void Mlp::set_params(const float* host) {
+    QK_HIP(hipMemcpyAsync(params.get(), host, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st_));
+    QK_HIP(hipMemsetAsync(m1.get(), 0, sizeof(float) * kMlpParams, st_));
+    QK_HIP(hipMemsetAsync(m2.get(), 0, sizeof(float) * kMlpParams, st_));
+    adam_t = 0;
+    cast();
+    QK_HIP(hipStreamSynchronize(st_));
+}
+
+void Mlp::cast() {
+    hipLaunchKernelGGL(k_mlp_cast, dim3((kHid * kHid + 255) / 256), dim3(256), 0, st_,
+                       params.get<float>(), w1t.get<u16>(), w2t.get<u16>(), w2k.get<u16>(),
+                       w3b.get<u16>());
+    QK_HIP(hipGetLastError());
+}
+
+void Mlp::forward(const u16* X, u64 n, float* o, u16* h1, u16* h2, hipStream_t st) const {
+    if (!n) return;
+    hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_grid(n, kRows)), dim3(kThreads), 0, st, X, n,
+                       w1t.get<u16>(), w2t.get<u16>(), w3b.get<u16>(), params.get<float>(), o, h1, h2);
+    QK_HIP(hipGetLastError());
+}
+
+void Mlp::backward(const u16* X, const u16* h1, const u16* h2, const float* go, u64 n,
+                   float* grads, hipStream_t st) {
+    QK_HIP(hipMemsetAsync(grads, 0, sizeof(float) * kMlpParams, st));
+    if (!n) return;
+    wg_.resize(sizeof(u16) * n * kHid * 2);
+    u16* gz2 = wg_.get<u16>();
+    u16* gz1 = gz2 + n * kHid;
+    const int ga = mlp_grid(n, kRows);
+    const int gw = mlp_grid(n, 32);
+    const size_t s3 = kHid + 1, s1 = size_t(kIn) * kHid + kHid, s2 = size_t(kHid) * kHid + kHid;
+    part_.resize(sizeof(float) * (size_t(ga) * s3 + size_t(gw) * (s1 + s2)));
+    float* p3 = part_.get<float>();
+    float* p1 = p3 + size_t(ga) * s3;
+    float* p2 = p1 + size_t(gw) * s1;
+    hipLaunchKernelGGL(k_mlp_backward_act, dim3(ga), dim3(kThreads), 0, st, h1, h2, go, n,
+                       w2k.get<u16>(), w3b.get<u16>(), gz2, gz1, p3);
+    hipLaunchKernelGGL((k_mlp_wgrad<kIn>), dim3(gw), dim3(kThreads), 0, st, X, gz1, n, p1);
+    hipLaunchKernelGGL((k_mlp_wgrad<kHid>), dim3(gw), dim3(kThreads), 0, st, h1, gz2, n, p2);
+    // W1 and b1 are contiguous in the parameter block, as are W2 and b2
+    hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(s1) + 255) / 256), dim3(256), 0, st, p1, gw,
+                       int(s1), int(s1), grads + kMlpW1);
+    hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(s2) + 255) / 256), dim3(256), 0, st, p2, gw,
+                       int(s2), int(s2), grads + kMlpW2);
+    hipLaunchKernelGGL(k_mlp_sum_parts, dim3(1), dim3(256), 0, st, p3, ga, int(s3), int(s3),
+                       grads + kMlpW3);
+    QK_HIP(hipGetLastError());
+}
+
+void Mlp::adam_step(const float* grads, float lr, hipStream_t st) {
+    ++adam_t;
+    const float c1 = 1.0f - std::pow(0.9f, float(adam_t));
+    const float c2 = 1.0f - std::pow(0.999f, float(adam_t));
+    hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, st,
+                       params.get<float>(), m1.get<float>(), m2.get<float>(), grads, kMlpParams, lr,
+                       c1, c2);
+    hipLaunchKernelGGL(k_mlp_cast, dim3((kHid * kHid + 255) / 256), dim3(256), 0, st,
+                       params.get<float>(), w1t.get<u16>(), w2t.get<u16>(), w2k.get<u16>(),
+                       w3b.get<u16>());
+    QK_HIP(hipGetLastError());
+}
+
+void Mlp::eval_host(const float* Xh, u64 n, const float* goh, float* oh, float* gradh) {
+    std::vector<u16> xb(n * kIn);
+    for (u64 i = 0; i < n * kIn; ++i) xb[i] = f32_to_bf16(Xh[i]);
+    DevBuf X(sizeof(u16) * std::max<u64>(n, 1) * kIn), o(sizeof(float) * std::max<u64>(n, 1)),
+        h1(sizeof(u16) * std::max<u64>(n, 1) * kHid), h2(sizeof(u16) * std::max<u64>(n, 1) * kHid),
+        go(sizeof(float) * std::max<u64>(n, 1)), g(sizeof(float) * kMlpParams);
+    if (n) QK_HIP(hipMemcpyAsync(X.get(), xb.data(), sizeof(u16) * n * kIn, hipMemcpyHostToDevice, st_));
+    forward(X.get<u16>(), n, o.get<float>(), h1.get<u16>(), h2.get<u16>(), st_);
+    if (oh && n) QK_HIP(hipMemcpyAsync(oh, o.get(), sizeof(float) * n, hipMemcpyDeviceToHost, st_));
+    if (goh && gradh) {
+        if (n) QK_HIP(hipMemcpyAsync(go.get(), goh, sizeof(float) * n, hipMemcpyHostToDevice, st_));
+        backward(X.get<u16>(), h1.get<u16>(), h2.get<u16>(), go.get<float>(), n, g.get<float>(), st_);
+        QK_HIP(hipMemcpyAsync(gradh, g.get(), sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st_));
+    }
+    QK_HIP(hipStreamSynchronize(st_));
+}
+
+// ---------------------------------------------------------------------------
+// per-state statistics: occupation fractions and connectivity
+// ---------------------------------------------------------------------------
+void MlpStates::build(const Hamiltonian& H, hipStream_t st) {
+    const Basis& B = *H.basis;
+    n = B.size();
+    nsites = std::max(1, B.nsites());
+    diag.resize(sizeof(double) * std::max<u64>(n, 1));
+    stat.resize(sizeof(float) * 5 * std::max<u64>(n, 1));
+    if (!n) return;
+    DevBuf dc(sizeof(cplx) * n);
+    H.diagonal(dc.get<cplx>(), st);
+    std::vector<cplx> dh(n);
+    QK_HIP(hipMemcpyAsync(dh.data(), dc.get(), sizeof(cplx) * n, hipMemcpyDeviceToHost, st));
+    QK_HIP(hipStreamSynchronize(st));
+    std::vector<double> dr(n);
+    for (u64 i = 0; i < n; ++i) dr[i] = dh[i].re;
+
+    // connectivity: fraction of off-diagonal groups that reach a basis state
+    // with a non-zero element (the row's off-diagonal pattern)
+    std::vector<const Group*> off;
+    for (const Group& g : H.prog.groups)
+        if (g.kind != kDiag) off.push_back(&g);
+    const bool two = B.kind() == SiteKind::Hubbard || B.kind() == SiteKind::TJ;
+    const int ns = B.nsites();
+    std::vector<float> sh(5 * n);
+    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned w = 0; w < nt; ++w)
+        th.emplace_back([&, w]() {
+            const u64 lo = n * w / nt, hi = n * (w + 1) / nt;
+            for (u64 i = lo; i < hi; ++i) {
+                const u64 s = B.state(i);
+                float occ[4] = {0, 0, 0, 0};
+                for (int q = 0; q < ns; ++q) {
+                    const int v = two ? int((s >> (2 * q)) & 3u) : int((s >> q) & 1u);
+                    occ[v] += 1.0f;
+                }
+                int cnt = 0;
+                for (const Group* g : off) {
+                    u32 cfg = 0;
+                    for (size_t k = 0; k < g->pos.size(); ++k) cfg |= u32((s >> g->pos[k]) & 1u) << k;
+                    const cplx a = g->table[cfg];
+                    if (a.re == 0.0 && a.im == 0.0) continue;
+                    if (B.index(s ^ g->flip) >= 0) ++cnt;
+                }
+                for (int k = 0; k < 4; ++k) sh[5 * i + k] = occ[k] / float(ns);
+                sh[5 * i + 4] = off.empty() ? 0.0f : float(cnt) / float(off.size());
+            }
+        });
+    for (auto& t : th) t.join();
+    QK_HIP(hipMemcpyAsync(diag.get(), dr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+    QK_HIP(hipMemcpyAsync(stat.get(), sh.data(), sizeof(float) * 5 * n, hipMemcpyHostToDevice, st));
+    QK_HIP(hipStreamSynchronize(st));
+}
+
+void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st) {
+    if (!s.n) return;
+    hipLaunchKernelGGL(k_mlp_features, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st,
+                       s.diag.get<double>(), s.stat.get<float>(), s.n, theta, 1.0 / s.nsites, X);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_combine(const MlpStates& s, double theta, const float* o, const cplx* r, cplx* t,
+                 hipStream_t st) {
+    if (!s.n) return;
+    hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st,
+                       s.diag.get<double>(), s.n, theta, o, r, t);
+    QK_HIP(hipGetLastError());
+}
+
+double mlp_pair_loss(const MlpStates& s, const cplx* r, const cplx* ts, const cplx* u,
+                     const double* theta_dev, int P, const float* o, float* go, DevBuf& scratch,
+                     hipStream_t st) {
+    if (!s.n || P <= 0) return 0.0;
+    const int nblk = int(std::min<u64>(64, (s.n + kLossT - 1) / kLossT));
+    scratch.resize(sizeof(double) * (size_t(P) * nblk * 5 + size_t(P) * 8));
+    double* part = scratch.get<double>();
+    double* coef = part + size_t(P) * nblk * 5;
+    hipLaunchKernelGGL(k_mlp_loss_part, dim3(nblk, P), dim3(kLossT), 0, st, s.diag.get<double>(), s.n,
+                       theta_dev, o, r, ts, u, nblk, part);
+    std::vector<double> ph(size_t(P) * nblk * 5);
+    QK_HIP(hipMemcpyAsync(ph.data(), part, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, st));
+    QK_HIP(hipStreamSynchronize(st));
+    // |ts|^2 per pair is needed for the loss value only; the gradient
+    // direction does not depend on it, so it is folded into the scale below.
+    std::vector<double> cf(size_t(P) * 8, 0.0);
+    double loss = 0.0;
+    for (int p = 0; p < P; ++p) {
+        double v[5] = {0, 0, 0, 0, 0};
+        for (int b = 0; b < nblk; ++b)
+            for (int q = 0; q < 5; ++q) v[q] += ph[(size_t(p) * nblk + b) * 5 + q];
+        const double A2 = v[0] * v[0] + v[1] * v[1];
+        const double Bv = v[4] - (v[2] * v[2] + v[3] * v[3]);
+        // ts is stored normalised (|ts| = 1), so C = 1
+        const double Bs = std::max(Bv, 1e-300);
+        loss += 1.0 - A2 / Bs;
+        double* c = cf.data() + size_t(p) * 8;
+        c[0] = v[0];
+        c[1] = v[1];
+        c[2] = v[2];
+        c[3] = v[3];
+        c[4] = -2.0 / Bs;
+        c[5] = 2.0 * A2 / (Bs * Bs);
+    }
+    QK_HIP(hipMemcpyAsync(coef, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice, st));
+    const u64 tot = u64(P) * s.n;
+    hipLaunchKernelGGL(k_mlp_loss_grad, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st,
+                       s.diag.get<double>(), s.n, theta_dev, o, r, ts, u, coef, P, go);
+    QK_HIP(hipGetLastError());
+    QK_HIP(hipStreamSynchronize(st));
+    return loss;
+}
+
+}  // namespace qk

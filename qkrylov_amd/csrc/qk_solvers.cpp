@@ -206,7 +206,7 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
 // Davidson (reference src/solvers/davidson.cpp:85-227)
 // ---------------------------------------------------------------------------
 DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace, double tol,
-                               bool want_vectors, Preconditioner* pre) {
+                               bool want_vectors, Preconditioner* pre, int max_iter) {
     DavidsonResult res;
     const u64 d = H.dim();
     if (d == 0) return res;
@@ -261,7 +261,8 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
     int valid_rows = 0;
     std::vector<DevBuf> R;
     DevBuf coef;
-    for (int iter = 0; iter < 100; ++iter) {
+    max_iter = std::max(max_iter, 1);
+    for (int iter = 0; iter < max_iter; ++iter) {
         const int m = int(V.size());
         // H_sub[i][j] = <V_i, HV_j>, j <= i (:143-150); new rows only.
         std::vector<cplx> Hn(size_t(m) * m, cplx{0.0, 0.0});
@@ -309,7 +310,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
                 need.push_back(k);
             }
         }
-        if (all_conv || iter == 99) {
+        if (all_conv || iter == max_iter - 1) {
             res.iterations = iter + 1;
             res.converged = all_conv;
             for (int k = 0; k < ne; ++k) res.eigenvalues.push_back(ev[k]);
@@ -331,11 +332,22 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         }
 
         // corrections t = M(r) (:166-177)
+        DevBuf Ub;
+        if (pre && pre->wants_ritz() && ne > 0) {
+            std::vector<cplx> Sv(size_t(ne) * m);
+            for (int k = 0; k < ne; ++k)
+                for (int i = 0; i < m; ++i) Sv[size_t(k) * m + i] = S[size_t(k) * m + i];
+            coef.resize(sizeof(cplx) * Sv.size());
+            QK_HIP(hipMemcpyAsync(coef.get(), Sv.data(), sizeof(cplx) * Sv.size(), hipMemcpyHostToDevice, st));
+            Ub.resize(sizeof(cplx) * d * ne);
+            vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, Ub.get<cplx>(), d, d, st);
+        }
         std::vector<DevBuf> T;
         for (int k : need) {
             T.emplace_back(sizeof(cplx) * d);
             const cplx* rk = Rb.get<cplx>() + u64(k) * d;
-            if (pre) pre->apply(H, rk, diag.get<cplx>(), ev[k], T.back().get<cplx>(), st);
+            const cplx* uk = Ub.get() ? Ub.get<cplx>() + u64(k) * d : nullptr;
+            if (pre) pre->apply(H, rk, diag.get<cplx>(), ev[k], uk, T.back().get<cplx>(), st);
             else vec::jacobi_precond(rk, diag.get<cplx>(), ev[k], T.back().get<cplx>(), d, st);
         }
 
@@ -377,7 +389,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
             }
         }
     }
-    return DavidsonResult{};   // not reached (iter == 99 returns), as :226
+    return DavidsonResult{};   // not reached (the last iteration returns), as :226
 }
 
 // ---------------------------------------------------------------------------

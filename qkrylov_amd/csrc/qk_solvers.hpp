@@ -32,13 +32,17 @@ struct DavidsonResult {
 // Preconditioner hook: t = M(r; lambda).  When null, Jacobi (reference).
 struct Preconditioner {
     virtual ~Preconditioner() = default;
-    // r, t: device vectors of dim entries; diag: device complex diagonal.
+    // r, t: device vectors of dim entries; diag: device complex diagonal;
+    // u: the Ritz vector of lambda when wants_ritz(), else null.
     virtual void apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda,
-                       cplx* t, hipStream_t st) = 0;
+                       const cplx* u, cplx* t, hipStream_t st) = 0;
+    virtual bool wants_ritz() const { return false; }
 };
 
+// max_iter: the reference stops after 100 iterations (davidson.cpp:141).
 DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace, double tol,
-                               bool want_vectors, Preconditioner* pre = nullptr);
+                               bool want_vectors, Preconditioner* pre = nullptr,
+                               int max_iter = 100);
 
 struct CFResult {
     std::vector<double> alphas, betas;

@@ -1,0 +1,136 @@
+"""Learned Davidson preconditioner on the GPU: BF16-MFMA forward and backward
+kernels against the CPU oracle (oracle/mlp_oracle.py), the features kernel,
+the MINRES correction against a dense solve, training, and Davidson with
+the trained network against the Jacobi method."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(qk, seed=3):
+    net = qk.MLPPreconditioner(11)
+    net.set_params(M.init_params(seed))
+    return net
+
+
+def test_forward_matches_oracle(qk):
+    net = _net(qk)
+    p = net.get_params()
+    rng = np.random.default_rng(0)
+    for n in (1, 63, 64, 1000):          # ragged tiles of 64 rows
+        X = rng.standard_normal((n, 32)).astype(np.float32)
+        o = net.eval(X)
+        ref, _ = M.forward(p, X)
+        scale = np.max(np.abs(ref)) + 1e-3
+        print(f"[mlp forward] n={n} max|diff|/scale={np.max(np.abs(o - ref)) / scale:.2e}", flush=True)
+        # FP32 accumulation order and rare one-ulp BF16 flips of activations
+        assert np.max(np.abs(o - ref)) <= 2e-2 * scale
+        assert np.median(np.abs(o - ref)) <= 1e-4 * scale
+
+
+def test_gradients_match_oracle(qk):
+    net = _net(qk, 4)
+    p = net.get_params()
+    rng = np.random.default_rng(1)
+    for n in (50, 777):
+        X = rng.standard_normal((n, 32)).astype(np.float32)
+        go = rng.standard_normal(n).astype(np.float32)
+        _, g = net.eval(X, go)
+        ref = M.backward(p, X, go)
+        for name, a, b in (("W1", M.W1, M.B1), ("b1", M.B1, M.W2), ("W2", M.W2, M.B2),
+                           ("b2", M.B2, M.W3), ("w3", M.W3, M.B3), ("b3", M.B3, M.NPARAMS)):
+            rel = np.linalg.norm(g[a:b] - ref[a:b]) / (np.linalg.norm(ref[a:b]) + 1e-12)
+            print(f"[mlp grad] n={n} {name} rel={rel:.2e}", flush=True)
+            assert rel < 2e-2, (name, rel)
+
+
+def test_untrained_network_is_jacobi(qk):
+    net = qk.MLPPreconditioner(5)
+    X = np.random.default_rng(2).standard_normal((300, 32)).astype(np.float32)
+    assert np.all(net.eval(X) == 0.0)
+
+
+def _spin_chain(qk, lat, n):
+    b = qk.SpinHalfBasis(n, sz=0)
+    return qk.MatrixFreeHamiltonian(b, qk.SpinHalfSite(), lat.heisenberg_chain(n)), b
+
+
+def test_features_match_oracle(qk, orc, lat):
+    n = 10
+    H, b = _spin_chain(qk, lat, n)
+    ob = orc.Basis("spin", n, {"use_sz": True, "sz2": 0})
+    oh = orc.Ham(ob, "spin", lat.heisenberg_chain(n))
+    A = oh.to_scipy().tocsr()
+    diag = A.diagonal().real
+    off = A.copy()
+    off.setdiag(0)
+    off.eliminate_zeros()
+    ngroups = H.info()["lo"] + H.info()["hi"] + H.info()["mix"]
+    # one group per bond here, so the row's non-zero off-diagonals are its groups
+    conn = (np.diff(off.indptr) / ngroups).astype(np.float32)
+    occ = M.occupations(ob.states(), n, False)
+    for theta in (-4.5, float(diag[3])):
+        f = qk.mlp_features(H, theta)
+        ref = M.features(diag, occ, conn, theta, n)
+        ulp = np.abs(f.view(np.uint32).astype(np.int64) - ref.view(np.uint32).astype(np.int64))
+        assert np.max(ulp) <= 1 << 16, np.max(ulp)   # at most one BF16 ulp (tanh rounding)
+        assert np.mean(f == ref) > 0.99
+
+
+def test_jd_correction_matches_dense_solve(qk, orc, lat):
+    bonds = lat.square_bonds(3, 2)
+    ops = lat.tj_model(bonds, 1.0, 0.4)
+    sec = qk.Sector().set_hubbard_particles(2, 2)
+    H = qk.MatrixFreeHamiltonian(qk.TJBasis(6, sector=sec), qk.TJSite(), ops)
+    ob = orc.Basis("tj", 6, {"use_nup": True, "nup": 2, "use_ndn": True, "ndn": 2})
+    A = orc.Ham(ob, "tj", ops).to_scipy().toarray()
+    d = A.shape[0]
+    rng = np.random.default_rng(3)
+    Q, _ = np.linalg.qr(rng.standard_normal((d, 4)) + 1j * rng.standard_normal((d, 4)))
+    w, S = np.linalg.eigh(Q.conj().T @ A @ Q)
+    u = Q @ S[:, 0]
+    theta = float(w[0])
+    r = A @ u - theta * u
+    t, rr = qk.jd_correction(H, r, u, theta, max_iter=600, rtol=1e-13)
+    P = np.eye(d) - np.outer(u, u.conj())
+    Md = P @ (A - theta * np.eye(d)) @ P + np.outer(u, u.conj())
+    t_ref = np.linalg.solve(Md, -r)
+    print(f"[jd] d={d} minres rel res={rr:.1e} |t-t_ref|/|t_ref|="
+          f"{np.linalg.norm(t - t_ref) / np.linalg.norm(t_ref):.2e}", flush=True)
+    assert abs(np.vdot(u, t)) < 1e-10 * np.linalg.norm(t)
+    assert np.linalg.norm(t - t_ref) <= 1e-8 * np.linalg.norm(t_ref)
+
+
+def test_training_lowers_loss_and_davidson_agrees(qk, lat):
+    def tj(lx, ly, k):
+        ops = lat.tj_model(lat.square_bonds(lx, ly), 1.0, 0.4)
+        sec = qk.Sector().set_hubbard_particles(k, k)
+        return qk.MatrixFreeHamiltonian(qk.TJBasis(lx * ly, sector=sec), qk.TJSite(), ops)
+
+    Hs = tj(2, 4, 3)     # d = 560
+    net = qk.MLPPreconditioner(7)
+    t = time.time()
+    hist = net.train(Hs, n_eig=4, max_subspace=24, max_iter=20, max_pairs=48, epochs=15,
+                     batch_pairs=8, lr=1e-3)
+    print(f"[mlp train] {net.last_train} loss {hist[0]:.4f} -> {hist[-1]:.4f} "
+          f"in {time.time() - t:.1f} s", flush=True)
+    assert net.last_train["pairs"] > 0
+    assert net.last_train["solve_rel_res"] < 1e-6
+    assert hist[-1] < hist[0]
+
+    H = tj(3, 4, 5)      # d = 16,632
+    res = {}
+    for name, pre in (("jacobi", None), ("mlp", net)):
+        t = time.time()
+        r = qk.davidson_lowest_ex(H, 4, 32, 1e-8, 300, pre)
+        res[name] = (r, time.time() - t)
+        print(f"[davidson 3x4] {name}: iterations={r.iterations} converged={r.converged} "
+              f"time={res[name][1]:.2f} s ev={r.eigenvalues}", flush=True)
+    rj, rm = res["jacobi"][0], res["mlp"][0]
+    assert rj.converged and rm.converged
+    np.testing.assert_allclose(rm.eigenvalues, rj.eigenvalues, rtol=0, atol=1e-10)
