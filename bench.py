@@ -148,6 +148,111 @@ def bench_ftlm(qk, lat):
     }
 
 
+def bench_lanczos_j1j2(qk, lat):
+    """J1-J2 Heisenberg on the 6x4 periodic square lattice (J1=1, J2=0.5, Sz=0,
+    d=2,704,156): 300 Lanczos steps with full reorthogonalisation (tol 0),
+    ground energy and the residual of the Ritz vector."""
+    t = time.perf_counter()
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(24, sz=0), qk.SpinHalfSite(),
+                                 lat.j1j2_square(6, 4, 1.0, 0.5))
+    setup = time.perf_counter() - t
+    t = time.perf_counter()
+    r = qk.lanczos_ground_state(H, maxiter=300, tol=0.0)
+    dt = time.perf_counter() - t
+    v = r.eigenvector
+    res = float(np.linalg.norm(H.apply(v) - r.energy * v) / np.linalg.norm(v))
+    log(f"lanczos J1-J2 6x4: E0={r.energy:.12f} iterations={r.iterations} time={dt:.2f} s "
+        f"|H psi - E psi|={res:.2e}")
+    return {"workload": "J1-J2 Heisenberg 6x4 periodic, J1=1, J2=0.5, Sz=0, d=2,704,156; "
+                        "300 Lanczos steps, full reorthogonalisation, reference start vector",
+            "dim": H.dimension, "setup_s": setup, "time_to_ground_state_s": dt,
+            "iterations": r.iterations, "E0": r.energy, "E0_per_site": r.energy / 24,
+            "ritz_residual": res}
+
+
+def bench_davidson_tj(qk, lat, msub=48, max_iter=400):
+    """t-J on the 4x4 periodic square lattice (t=1, J=0.4, 7 up, 7 down,
+    d=411,840), lowest 8 eigenpairs to residual 1e-8: Jacobi Davidson against
+    Davidson with the learned preconditioner trained on 3x4 (d=16,632)."""
+    def tj(lx, ly, k):
+        ops = lat.tj_model(lat.square_bonds(lx, ly), 1.0, 0.4)
+        sec = qk.Sector().set_hubbard_particles(k, k)
+        return qk.MatrixFreeHamiltonian(qk.TJBasis(lx * ly, sector=sec), qk.TJSite(), ops)
+
+    Hs, H = tj(3, 4, 5), tj(4, 4, 7)
+    net = qk.MLPPreconditioner(7)
+    t = time.perf_counter()
+    hist = net.train(Hs, n_eig=8, max_subspace=48, max_iter=30, max_pairs=192, epochs=20,
+                     batch_pairs=8, lr=1e-3)
+    t_train = time.perf_counter() - t
+    log(f"mlp training on 3x4: {t_train:.1f} s {net.last_train}")
+    out = {"workload": "t-J 4x4 periodic, t=1, J=0.4, 7 up 7 down, d=411,840; lowest 8, "
+                       f"residual 1e-8, max_subspace {msub}, at most {max_iter} iterations",
+           "dim": H.dimension, "train_s": t_train, "train": net.last_train,
+           "loss_history": [float(x) for x in hist]}
+    res = {}
+    for name, pre in (("jacobi", None), ("mlp", net)):
+        t = time.perf_counter()
+        r = qk.davidson_lowest_ex(H, 8, msub, 1e-8, max_iter, pre)
+        dt = time.perf_counter() - t
+        res[name] = r
+        out[name] = {"iterations": r.iterations, "converged": r.converged, "time_s": dt,
+                     "eigenvalues": [float(x) for x in r.eigenvalues]}
+        log(f"davidson 4x4 {name}: iterations={r.iterations} converged={r.converged} {dt:.2f} s")
+    out["max_eig_diff"] = float(np.max(np.abs(np.asarray(res["mlp"].eigenvalues)
+                                              - np.asarray(res["jacobi"].eigenvalues))))
+    out["speedup_time"] = out["jacobi"]["time_s"] / out["mlp"]["time_s"]
+    return out
+
+
+def bench_dynamics(qk, torch, H, d):
+    """N=28 chain: ground state by Lanczos (reference start, tol 1e-12), then
+    S^z(q)|0> for the 28 momenta q = 2 pi k / 28 as one block of start vectors,
+    200 continued-fraction steps each, S(q, w) on 2000 points in [0, 5],
+    eta = 0.05."""
+    t = time.perf_counter()
+    gs = qk.lanczos_ground_state(H, maxiter=200, tol=1e-12)
+    t_gs = time.perf_counter() - t
+    log(f"N=28 ground state: E0={gs.energy:.12f} iterations={gs.iterations} {t_gs:.2f} s")
+    psi = torch.from_numpy(gs.eigenvector).to("cuda")
+    nq, M = N_SITES, 200
+    phi = torch.empty((nq, d), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    H.sz_q_device(psi.data_ptr(), phi.data_ptr(), nq, d, torch.cuda.current_stream().cuda_stream)
+    H.sync()
+    torch.cuda.synchronize()
+    t_phi = time.perf_counter() - t
+    del psi
+    gs.eigenvector = None
+    t = time.perf_counter()
+    cf = qk.continued_fraction_block_device(H, phi.data_ptr(), nq, M)
+    t_cf = time.perf_counter() - t
+    del phi
+    torch.cuda.empty_cache()
+    omegas = np.linspace(0.0, 5.0, 2000)
+    t = time.perf_counter()
+    S = np.array([[qk.evaluate_spectral_function(c.alphas, c.betas, c.norm_phi0, w, gs.energy, 0.05)
+                   for w in omegas] for c in cf])
+    t_s = time.perf_counter() - t
+    # sum rule: the weight inside [0, 5] against |phi_q|^2
+    dw = omegas[1] - omegas[0]
+    weight = S.sum(1) * dw
+    norms2 = np.array([c.norm_phi0 ** 2 for c in cf])
+    kpi = nq // 2
+    peak = float(omegas[int(np.argmax(S[kpi]))])
+    log(f"dynamics: phi_q {t_phi:.2f} s, CF block {nq}x{M} {t_cf:.2f} s, S(q,w) {t_s:.2f} s, "
+        f"q=pi peak at w={peak:.3f}")
+    return {"workload": "N=28 chain, Sz=0, d=40,116,600: Lanczos ground state (tol 1e-12), "
+                        "S^z(q)|0> for 28 q as one block, 200 CF steps, S(q,w) 2000 points, eta 0.05",
+            "E0": gs.energy, "E0_per_site": gs.energy / N_SITES, "gs_iterations": gs.iterations,
+            "time_ground_state_s": t_gs, "time_phi_s": t_phi, "time_cf_block_s": t_cf,
+            "time_spectra_s": t_s, "cf_steps": [len(c.alphas) for c in cf],
+            "q_pi_peak_omega": peak,
+            "sum_rule_ratio_q_pi": float(weight[kpi] / norms2[kpi]) if norms2[kpi] > 0 else None,
+            "q0_norm": float(cf[0].norm_phi0)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +263,9 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--apply-only", action="store_true", help="time the b=1 apply only")
     ap.add_argument("--ftlm-only", action="store_true", help="time the FTLM sweep only")
+    ap.add_argument("--skip-lanczos", action="store_true", help="skip the J1-J2 6x4 Lanczos run")
+    ap.add_argument("--skip-davidson", action="store_true", help="skip the t-J 4x4 Davidson runs")
+    ap.add_argument("--skip-dynamics", action="store_true", help="skip the N=28 dynamics run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 5 times (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -206,8 +314,24 @@ def main():
         extra["block"] = {"vectors": nb, "ms_per_block": msb, "vector_applies_per_s": nb * 1e3 / msb,
                           "achieved_GBs": ach_b, "frac": ach_b / HBM_PEAK_GBS}
         log(f"b={nb}: {msb:.3f} ms/block  {nb * 1e3 / msb:.1f} vector-applies/s  {ach_b:.1f} GB/s")
-    if not args.apply_only and not args.skip_ftlm:
-        extra["ftlm"] = bench_ftlm(qk, lat)
+    def leg(name, fn):
+        # a failing side run is reported in the line, it does not lose the headline
+        try:
+            extra[name] = fn()
+        except Exception as e:  # noqa: BLE001
+            log(f"{name} failed: {e!r}")
+            extra[name] = {"error": repr(e)}
+        torch.cuda.empty_cache()
+
+    if not args.apply_only:
+        if not args.skip_ftlm:
+            leg("ftlm", lambda: bench_ftlm(qk, lat))
+        if not args.skip_lanczos:
+            leg("lanczos", lambda: bench_lanczos_j1j2(qk, lat))
+        if not args.skip_davidson:
+            leg("davidson", lambda: bench_davidson_tj(qk, lat))
+        if not args.skip_dynamics:
+            leg("dynamics", lambda: bench_dynamics(qk, torch, H, d))
     cpu = None
     if not args.apply_only and not args.skip_cpu:
         cpu = cpu_baseline(lat)

@@ -195,6 +195,8 @@ const char* qkrylov_last_error(void);
 
 /* Device name and basic properties, for logs. Returns 0 on success. */
 int qkrylov_device_info(char* buf, size_t len);
+/* Number of visible GPUs (0 when the HIP runtime finds none). */
+int qkrylov_device_count(void);
 
 /* Sector: clear every constraint flag. */
 int qkrylov_sector_reset(qkrylov_sector_h sector);
@@ -317,6 +319,13 @@ int qkrylov_continued_fraction_block_f64(qkrylov_hamiltonian_h h, const double* 
                                          int nblock, int n_iter, double* alphas_out,
                                          double* betas_out, double* norms_out, int* counts_out);
 
+/* As qkrylov_continued_fraction_block_f64 with the start vectors already in
+ * device memory: vector b at phi0_dev + b*dim complex doubles. */
+int qkrylov_continued_fraction_block_device_f64(qkrylov_hamiltonian_h h, const void* phi0_dev,
+                                                int nblock, int n_iter, double* alphas_out,
+                                                double* betas_out, double* norms_out,
+                                                int* counts_out);
+
 /* Reference src/solvers/dynamics.cpp:61-89 in FP64. */
 double qkrylov_evaluate_spectral_function_f64(const double* alphas, const double* betas, size_t n,
                                               double norm_phi0, double omega, double E0, double eta);
@@ -341,6 +350,12 @@ int qkrylov_ftlm_sweep_f64(const qkrylov_hamiltonian_h* hs, int nsec, const doub
  * Spin-1/2 bases only. */
 int qkrylov_sz_q_device_f64(qkrylov_hamiltonian_h h, const void* psi_dev, void* phi_dev,
                             int nq, uint64_t ld, void* stream);
+
+/* G[i*nw + j] = sum_k conj(V_i[k]) W_j[k] on the GPU (FP64 MFMA), the
+ * tall-skinny projection Davidson uses for new rows of V^H H V: host
+ * vectors V (m*2*n doubles, m <= 64) and W (nw*2*n doubles, nw <= 16). */
+int qkrylov_block_dot_f64(const double* V_complex, int m, const double* W_complex, int nw,
+                          uint64_t n, double* G_complex_out);
 
 /* Synchronise the handle's stream. */
 int qkrylov_hamiltonian_sync(qkrylov_hamiltonian_h h);

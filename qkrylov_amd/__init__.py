@@ -8,6 +8,7 @@ the HIP kernels; vectors cross the boundary as complex128 numpy arrays.
 from __future__ import annotations
 
 import ctypes as ct
+import enum
 from typing import List, Optional, Sequence, Tuple, Union
 
 import numpy as np
@@ -16,14 +17,16 @@ from . import _lib
 from ._lib import check, last_error, load
 
 __all__ = [
-    "Sector", "Basis", "SpinHalfBasis", "FermionBasis", "HubbardBasis", "TJBasis",
+    "Sector", "Op", "Basis", "SpinHalfBasis", "FermionBasis", "HubbardBasis", "TJBasis",
     "Site", "SpinHalfSite", "FermionSite", "HubbardSite", "TJSite",
     "OpSum", "Sz", "Sp", "Sm", "Sx", "Sy", "CdagUp", "CUp", "CdagDn", "CDn",
     "Nup", "Ndn", "Nupdn", "N", "C", "Cdag", "Id",
     "MatrixFreeHamiltonian", "LanczosResult", "lanczos_ground_state",
     "DavidsonResult", "davidson_lowest", "DynamicsResult", "continued_fraction_coeffs",
     "continued_fraction_block", "evaluate_spectral_function", "FTLMResult", "ftlm",
-    "ftlm_sweep", "device_info",
+    "ftlm_sweep", "device_info", "gpu_count", "find_gpu", "Bdag", "B",
+    "MLPPreconditioner", "davidson_lowest_ex", "jd_correction", "mlp_features",
+    "continued_fraction_block_device",
 ]
 
 
@@ -35,6 +38,16 @@ def device_info() -> str:
     buf = ct.create_string_buffer(256)
     check(load().qkrylov_device_info(buf, 256), "device_info")
     return buf.value.decode()
+
+
+def gpu_count() -> int:
+    """Number of visible GPUs (reference __init__.py:37-39)."""
+    return max(int(load().qkrylov_device_count()), 0)
+
+
+def find_gpu() -> Optional[str]:
+    """'hip' when a GPU is visible, else None (reference __init__.py:31-35)."""
+    return "hip" if gpu_count() > 0 else None
 
 
 # --------------------------------------------------------------------------- sector
@@ -150,7 +163,7 @@ class SpinHalfBasis(Basis):
     site_kind = 0
 
     def __init__(self, N: int, conserve_sz: bool = False, sz: Optional[float] = None,
-                 sector: Optional[Sector] = None):
+                 sector: Optional[Sector] = None, dtype=None):
         if sector is None:
             sector = Sector()
             if sz is not None or conserve_sz:
@@ -163,7 +176,7 @@ class FermionBasis(Basis):
     site_kind = 1
 
     def __init__(self, N: int, conserve_n: bool = False, n: int = 0,
-                 sector: Optional[Sector] = None):
+                 sector: Optional[Sector] = None, dtype=None):
         if sector is None:
             sector = Sector()
             if conserve_n or n != 0:
@@ -173,7 +186,8 @@ class FermionBasis(Basis):
 
 class _TwoSpecies(Basis):
     def __init__(self, N: int, conserve_nup: bool = False, nup: int = 0,
-                 conserve_ndn: bool = False, ndn: int = 0, sector: Optional[Sector] = None):
+                 conserve_ndn: bool = False, ndn: int = 0, sector: Optional[Sector] = None,
+                 dtype=None):
         if sector is None:
             sector = Sector()
             lib = load()
@@ -200,7 +214,7 @@ class Site:
     _create = None
     kind = -1
 
-    def __init__(self):
+    def __init__(self, dtype=None):
         self._h = getattr(load(), self._create)()
 
     def __del__(self):
@@ -301,6 +315,29 @@ class OpSumExpr:
         return OpSumExpr([complex(other) * t for t in self.terms])
 
 
+class Op(str, enum.Enum):
+    """Operator names (reference operators.py:6-22), plus the fermion-site
+    names C, Cdag, Id of the reference sites."""
+    Sz = "Sz"
+    Sp = "Sp"
+    Sm = "Sm"
+    Sx = "Sx"
+    Sy = "Sy"
+    CdagUp = "CdagUp"
+    CUp = "CUp"
+    CdagDn = "CdagDn"
+    CDn = "CDn"
+    Nup = "Nup"
+    Ndn = "Ndn"
+    Nupdn = "Nupdn"
+    Bdag = "Bdag"
+    B = "B"
+    N = "N"
+    C = "C"
+    Cdag = "Cdag"
+    Id = "Id"
+
+
 def _op(name):
     return lambda i: LocalOpExpr(name, i)
 
@@ -309,18 +346,22 @@ Sz, Sp, Sm, Sx, Sy = (_op(n) for n in ("Sz", "Sp", "Sm", "Sx", "Sy"))
 CdagUp, CUp, CdagDn, CDn = (_op(n) for n in ("CdagUp", "CUp", "CdagDn", "CDn"))
 Nup, Ndn, Nupdn, N, C_, Cdag, Id = (_op(n) for n in ("Nup", "Ndn", "Nupdn", "N", "C", "Cdag", "Id"))
 globals()["C"] = C_  # fermion annihilator, named as the reference site op "C"
+# boson names of the reference's operator module (operators.py:126-127); no
+# site defines them, so a Hamiltonian that uses them is rejected at creation
+Bdag, B = _op("Bdag"), _op("B")
 
 
 class OpSum:
     """Append-only list of (coeff, [(op, site), ...]) (reference opsum.hpp)."""
 
-    def __init__(self):
+    def __init__(self, dtype=None):
         self.terms: List[Tuple[complex, List[Tuple[str, int]]]] = []
 
     def add_term(self, coeff: complex, *ops):
         if len(ops) % 2:
             raise ValueError("add_term(coeff, op1, site1, [op2, site2, ...])")
-        facs = [(str(ops[i]), int(ops[i + 1])) for i in range(0, len(ops), 2)]
+        facs = [(ops[i].value if isinstance(ops[i], Op) else str(ops[i]), int(ops[i + 1]))
+                for i in range(0, len(ops), 2)]
         self.terms.append((complex(coeff), facs))
         return self
 
@@ -380,11 +421,22 @@ class MatrixFreeHamiltonian:
     ``apply(x)`` computes y[a] = sum_terms conj(amp(a->b)) x[b] in FP64.
     """
 
-    def __init__(self, basis: Basis, site: Optional[Site] = None, ops: Optional[OpSum] = None):
+    def __init__(self, basis: Basis, site: Optional[Site] = None, ops: Optional[OpSum] = None,
+                 device: str = "hip", dtype=np.float64):
+        """device: accepted for the reference's signature (hamiltonian.py:25);
+        this library always runs on the GPU ("cpu" is refused).  dtype:
+        np.float64 (default here) or np.float32, which narrows the outputs of
+        apply/diagonal to complex64 as the reference's FP32 build returns;
+        the arithmetic is FP64 either way."""
         if ops is None and isinstance(site, OpSum):
             site, ops = None, site
         if site is None:
             site = site_for(basis)
+        if str(device).lower() not in ("hip", "gpu", "cuda", "rocm"):
+            raise ValueError(f"device {device!r}: this build runs on the GPU only (use 'hip' or 'gpu')")
+        if dtype not in (np.float32, np.float64):
+            raise ValueError("dtype must be np.float32 or np.float64")
+        self.device, self.dtype = str(device), dtype
         self.basis, self.site, self.ops = basis, site, ops
         lib = load()
         oh = ops._handle()
@@ -428,9 +480,47 @@ class MatrixFreeHamiltonian:
         if d == 0 or nvec == 0:
             return y
         check(load().qkrylov_hamiltonian_apply_f64(self._h, _ptr(x), _ptr(y), nvec), "apply")
-        return y
+        return y.astype(np.complex64) if self.dtype == np.float32 else y
 
     __matmul__ = apply
+
+    def to(self, device: Optional[str] = None, dtype=None) -> "MatrixFreeHamiltonian":
+        """A new Hamiltonian on the same basis, site and terms (reference
+        hamiltonian.py:111-115)."""
+        return MatrixFreeHamiltonian(self.basis, self.site, self.ops,
+                                     device=device if device is not None else self.device,
+                                     dtype=dtype if dtype is not None else self.dtype)
+
+    def aslinearoperator(self):
+        """scipy LinearOperator whose matvec is apply() (reference hamiltonian.py:120-144)."""
+        import scipy.sparse.linalg as sla
+        cdt = np.complex64 if self.dtype == np.float32 else np.complex128
+        return sla.LinearOperator(shape=(self.dimension, self.dimension),
+                                  matvec=lambda v: self.apply(np.ravel(v)),
+                                  matmat=lambda m: self.apply(np.ascontiguousarray(np.asarray(m).T)).T,
+                                  dtype=cdt)
+
+    def to_sparse(self, block: int = 256):
+        """Explicit CSR matrix for small bases (reference hamiltonian.py:146-177),
+        built from applies to blocks of unit vectors; entries that are exactly
+        zero are dropped, as the reference's np.nonzero does."""
+        import scipy.sparse as sp
+        d = self.dimension
+        rows, cols, vals = [], [], []
+        for c0 in range(0, d, block):
+            nb = min(block, d - c0)
+            E = np.zeros((nb, d), dtype=np.complex128)
+            E[np.arange(nb), c0 + np.arange(nb)] = 1.0
+            Y = np.atleast_2d(self.apply(E))
+            r, c = np.nonzero(Y.T)
+            rows.append(r)
+            cols.append(c + c0)
+            vals.append(Y.T[r, c])
+        cdt = np.complex64 if self.dtype == np.float32 else np.complex128
+        if not rows:
+            return sp.csr_matrix((d, d), dtype=cdt)
+        return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                             shape=(d, d), dtype=cdt)
 
     def apply_device(self, x_ptr: int, y_ptr: int, nvec: int = 1, ld: int = 0, stream: int = 0):
         check(load().qkrylov_hamiltonian_apply_device_f64(self._h, x_ptr, y_ptr, nvec, ld,
@@ -446,7 +536,7 @@ class MatrixFreeHamiltonian:
     def diagonal(self) -> np.ndarray:
         out = np.empty(self.dimension, dtype=np.complex128)
         check(load().qkrylov_hamiltonian_diagonal_f64(self._h, _ptr(out)), "diagonal")
-        return out
+        return out.astype(np.complex64) if self.dtype == np.float32 else out
 
     def __repr__(self):
         return f"MatrixFreeHamiltonian(dim={self.dimension})"
@@ -619,6 +709,24 @@ def continued_fraction_block(H: MatrixFreeHamiltonian, phi0: np.ndarray, n_iter:
                                                       _ptr(b), _ptr(nrm), _ptr(cnt)), "cf")
     out = []
     for j in range(nb):
+        n = int(cnt[j])
+        nbeta = n if n == n_iter and b[j, n - 1] >= 1e-15 else max(n - 1, 0)
+        out.append(DynamicsResult(a[j, :n].copy(), b[j, :nbeta].copy(), float(nrm[j])))
+    return out
+
+
+def continued_fraction_block_device(H: MatrixFreeHamiltonian, phi0_ptr: int, nblock: int,
+                                    n_iter: int = 100):
+    """As continued_fraction_block with the start vectors in device memory
+    (vector b at phi0_ptr + 16*b*dim bytes)."""
+    a = np.zeros((nblock, n_iter))
+    b = np.zeros((nblock, n_iter))
+    nrm = np.zeros(nblock)
+    cnt = np.zeros(nblock, dtype=np.int32)
+    check(load().qkrylov_continued_fraction_block_device_f64(
+        H._h, int(phi0_ptr), int(nblock), int(n_iter), _ptr(a), _ptr(b), _ptr(nrm), _ptr(cnt)), "cf")
+    out = []
+    for j in range(nblock):
         n = int(cnt[j])
         nbeta = n if n == n_iter and b[j, n - 1] >= 1e-15 else max(n - 1, 0)
         out.append(DynamicsResult(a[j, :n].copy(), b[j, :nbeta].copy(), float(nrm[j])))

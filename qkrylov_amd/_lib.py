@@ -109,6 +109,7 @@ SIGNATURES = {
     # extensions
     "qkrylov_last_error": (C.c_char_p, []),
     "qkrylov_device_info": (c_int, [C.c_char_p, c_size_t]),
+    "qkrylov_device_count": (c_int, []),
     "qkrylov_sector_reset": (c_int, [c_void_p]),
     "qkrylov_opsum_add_term_f64": (c_int, [c_void_p, c_double, c_double, c_int, P(C.c_char_p),
                                            P(c_int)]),
@@ -129,6 +130,8 @@ SIGNATURES = {
                                                       c_void_p, P(c_double), P(c_int)]),
     "qkrylov_continued_fraction_block_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p,
                                                      c_void_p, c_void_p, c_void_p]),
+    "qkrylov_continued_fraction_block_device_f64": (c_int, [c_void_p, c_void_p, c_int, c_int,
+                                                            c_void_p, c_void_p, c_void_p, c_void_p]),
     "qkrylov_evaluate_spectral_function_f64": (c_double, [c_void_p, c_void_p, c_size_t, c_double,
                                                           c_double, c_double, c_double]),
     "qkrylov_ftlm_f64": (c_int, [c_void_p, c_double, c_int, c_int, P(FTLMResultF64)]),
@@ -136,6 +139,7 @@ SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p]),
     "qkrylov_sz_q_device_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_u64, c_void_p]),
     "qkrylov_hamiltonian_sync": (c_int, [c_void_p]),
+    "qkrylov_block_dot_f64": (c_int, [c_void_p, c_int, c_void_p, c_int, c_u64, c_void_p]),
     "qkrylov_reference_start_f64": (c_int, [c_u64, C.c_uint, c_void_p]),
     "qkrylov_reference_normals_f64": (c_int, [c_u64, C.c_uint, c_void_p]),
     # learned Davidson preconditioner

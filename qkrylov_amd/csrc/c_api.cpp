@@ -10,6 +10,7 @@
 #include "qk_learned.hpp"
 #include "qk_rng.hpp"
 #include "qk_solvers.hpp"
+#include "qk_vec.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -120,6 +121,11 @@ extern "C" {
 
 const char* qkrylov_last_error(void) { return g_last_error.c_str(); }
 
+int qkrylov_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
 int qkrylov_device_info(char* buf, size_t len) {
     if (!buf || len == 0) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {
@@ -339,6 +345,32 @@ int qkrylov_hamiltonian_autotune(qkrylov_hamiltonian_h h, int nvec, char* buf, s
             std::strncpy(buf, s.c_str(), len - 1);
             buf[len - 1] = 0;
         }
+    });
+}
+int qkrylov_block_dot_f64(const double* V, int m, const double* W, int nw, uint64_t n, double* G) {
+    if (!V || !W || !G || m <= 0 || nw <= 0 || m > 64 || nw > 16) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        hipStream_t st = nullptr;
+        QK_HIP(hipStreamCreate(&st));
+        struct Guard {
+            hipStream_t s;
+            ~Guard() { (void)hipStreamDestroy(s); }
+        } guard{st};
+        const u64 nn = std::max<u64>(n, 1);
+        DevBuf dv(sizeof(cplx) * nn * m), dw(sizeof(cplx) * nn * nw), g(sizeof(cplx) * m * nw),
+            pv(sizeof(void*) * m), pw(sizeof(void*) * nw), scratch;
+        std::vector<const cplx*> hv(m), hw(nw);
+        for (int i = 0; i < m; ++i) hv[i] = dv.get<cplx>() + u64(i) * n;
+        for (int j = 0; j < nw; ++j) hw[j] = dw.get<cplx>() + u64(j) * n;
+        if (n) {
+            QK_HIP(hipMemcpyAsync(dv.get(), V, sizeof(cplx) * n * m, hipMemcpyHostToDevice, st));
+            QK_HIP(hipMemcpyAsync(dw.get(), W, sizeof(cplx) * n * nw, hipMemcpyHostToDevice, st));
+        }
+        QK_HIP(hipMemcpyAsync(pv.get(), hv.data(), sizeof(void*) * m, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(pw.get(), hw.data(), sizeof(void*) * nw, hipMemcpyHostToDevice, st));
+        vec::block_dot(pv.get<const cplx*>(), m, pw.get<const cplx*>(), nw, n, g.get<cplx>(), scratch, st);
+        QK_HIP(hipMemcpyAsync(G, g.get(), sizeof(cplx) * m * nw, hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
     });
 }
 int qkrylov_hamiltonian_sync(qkrylov_hamiltonian_h h) {
@@ -624,6 +656,26 @@ int qkrylov_continued_fraction_block_f64(qkrylov_hamiltonian_h h, const double* 
     return guarded([&] {
         const std::vector<CFResult> r =
             continued_fraction_block(*h->ptr, reinterpret_cast<const cplx*>(phi0), nblock, n_iter);
+        for (int b = 0; b < nblock; ++b) {
+            const CFResult& cf = r[b];
+            norms[b] = cf.norm_phi0;
+            for (int i = 0; i < n_iter; ++i) {
+                alphas[size_t(b) * n_iter + i] = i < int(cf.alphas.size()) ? cf.alphas[i] : 0.0;
+                betas[size_t(b) * n_iter + i] = i < int(cf.betas.size()) ? cf.betas[i] : 0.0;
+            }
+            if (counts) counts[b] = int(cf.alphas.size());
+        }
+    });
+}
+
+int qkrylov_continued_fraction_block_device_f64(qkrylov_hamiltonian_h h, const void* phi0_dev,
+                                                int nblock, int n_iter, double* alphas, double* betas,
+                                                double* norms, int* counts) {
+    if (!hok(h) || !phi0_dev || !alphas || !betas || !norms || n_iter <= 0 || nblock <= 0)
+        return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const std::vector<CFResult> r = continued_fraction_block_dev(
+            *h->ptr, static_cast<const cplx*>(phi0_dev), nblock, n_iter);
         for (int b = 0; b < nblock; ++b) {
             const CFResult& cf = r[b];
             norms[b] = cf.norm_phi0;

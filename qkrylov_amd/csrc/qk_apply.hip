@@ -313,6 +313,114 @@ k_apply_xchg(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y, 
     }
 }
 
+// Multi-sector exchange kernel: the tiles of several sectors of one model in
+// one launch, each tile applied to all vectors of its sector, B at a time.
+// Ranks, bond tests and diagonal of a state are computed once per tile and
+// reused for every vector; only the row offsets (jhi) differ by sector.
+template <int B>
+__global__ void __launch_bounds__(64 * kXWaves)
+k_apply_xchg_multi(const XParams p, const MParams mp, const cplx* __restrict__ x,
+                   cplx* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr u32 NT = 64 * kXWaves;
+    cplx* s_tab = reinterpret_cast<cplx*>(smem);
+    u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ndmix_tab) * sizeof(cplx));
+    for (u32 i = threadIdx.x; i < p.ndmix_tab; i += NT) s_tab[i] = p.dmix_tab[i];
+    if ((p.nlo & 7) == 0) {
+        const uint4* src = reinterpret_cast<const uint4*>(p.jlo);
+        uint4* dst = reinterpret_cast<uint4*>(s_jlo);
+        for (u32 i = threadIdx.x; i < p.nlo / 8; i += NT) dst[i] = src[i];
+    } else {
+        for (u32 i = threadIdx.x; i < p.nlo; i += NT) s_jlo[i] = p.jlo[i];
+    }
+    __syncthreads();
+
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const u32 lane = threadIdx.x & 63;
+    const u32 hi_end = p.n_lo + p.n_hi;
+    const u32 mix_end = hi_end + p.n_mix;
+
+    QK_FOR_TILES_W(mp, wave, t, kXWaves) {
+        const MTile T = mp.tiles[t];
+        const u32 hi = __builtin_amdgcn_readfirstlane(T.hi);
+        const u64* __restrict__ jhi = mp.jhis[T.sec];
+        if (lane >= T.nvalid) continue;
+        const u32 r = T.r0 + lane;
+        const u32 lo = p.lo_list[T.pos0 + lane];
+        const u64 a = T.base + r;
+        double dg = p.dlo[T.pos0 + lane] + T.dh;
+        if (p.n_dmix) {
+            const u32 s = (hi << p.L) | lo;
+            for (u32 g = 0; g < p.n_dmix; ++g) {
+                const DevGroup G = p.dmix[g];
+                dg += s_tab[G.tab + config_of(s, G)].re;
+            }
+        }
+        for (u32 v0 = 0; v0 < T.nvec; v0 += B) {
+            const u32 nv = min(u32(B), T.nvec - v0);
+            const cplx* xs = x + T.xoff + u64(v0) * T.ld;
+            cplx* ys = y + T.xoff + u64(v0) * T.ld;
+            cplx acc[B];
+#pragma unroll
+            for (int v = 0; v < B; ++v) {
+                acc[v] = cplx{0.0, 0.0};
+                if (u32(v) < nv) {
+                    const cplx xa = xs[v * T.ld + a];
+                    acc[v] = cplx{dg * xa.re, dg * xa.im};
+                }
+            }
+            u32 b = 0;
+            for (; b < p.n_lo; ++b) {
+                const XBond X = p.bonds[b];
+                const u32 vi = (lo >> X.bi) & 1, vj = (lo >> X.bj) & 1;
+                if (vi == vj) continue;
+                const double amp = vi ? X.a10 : X.a01;
+                const u64 idx = T.base + s_jlo[lo ^ X.mlo];
+#pragma unroll
+                for (int v = 0; v < B; ++v)
+                    if (u32(v) < nv) {
+                        const cplx xv = xs[v * T.ld + idx];
+                        acc[v].re += amp * xv.re;
+                        acc[v].im += amp * xv.im;
+                    }
+            }
+            for (; b < hi_end; ++b) {
+                const XBond X = p.bonds[b];
+                const u32 vi = (hi >> (X.bi - p.L)) & 1, vj = (hi >> (X.bj - p.L)) & 1;
+                if (vi == vj) continue;                      // uniform
+                const double amp = vi ? X.a10 : X.a01;
+                const u64 idx = jhi[hi ^ X.mhi] + r;
+#pragma unroll
+                for (int v = 0; v < B; ++v)
+                    if (u32(v) < nv) {
+                        const cplx xv = xs[v * T.ld + idx];
+                        acc[v].re += amp * xv.re;
+                        acc[v].im += amp * xv.im;
+                    }
+            }
+            for (; b < mix_end; ++b) {
+                const XBond X = p.bonds[b];
+                const u32 vj = (hi >> (X.bj - p.L)) & 1;   // uniform
+                const u64 base2 = jhi[hi ^ X.mhi];          // uniform
+                const u32 vi = (lo >> X.bi) & 1;
+                if (vi == vj || base2 == ~u64(0)) continue;
+                const double amp = vi ? X.a10 : X.a01;
+                const u64 idx = base2 + s_jlo[lo ^ X.mlo];
+#pragma unroll
+                for (int v = 0; v < B; ++v)
+                    if (u32(v) < nv) {
+                        const cplx xv = xs[v * T.ld + idx];
+                        acc[v].re += amp * xv.re;
+                        acc[v].im += amp * xv.im;
+                    }
+            }
+#pragma unroll
+            for (int v = 0; v < B; ++v)
+                if (u32(v) < nv) store_nt(&ys[v * T.ld + a], acc[v]);
+        }
+    }
+}
+
 // Bond batches of the single-vector exchange kernel.  Every load of a batch
 // (bond descriptors, row offsets, LDS ranks, the SPL gathers of all NB bonds)
 // is issued before the first one is used, so a wave keeps NB*SPL gathers in
@@ -816,10 +924,12 @@ void Hamiltonian::setup_exchange() {
     std::vector<DevGroup> mixdev = pack_groups(mixg, mixtab);
 
     dhi_host = dhi;
+    dlo_host = dlo;
     std::vector<XBond> bonds = lo_b;
     bonds.insert(bonds.end(), hi_b.begin(), hi_b.end());
     bonds.insert(bonds.end(), mix_b.begin(), mix_b.end());
 
+    bonds_host = bonds;
     upload(d_xbonds, bonds.data(), bonds.size(), stream);
     upload(d_dlo, dlo.data(), dlo.size(), stream);
     upload(d_dhi, dhi.data(), dhi.size(), stream);
@@ -995,6 +1105,67 @@ std::string Hamiltonian::autotune(int nvec) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return os.str();
+}
+
+bool MultiApply::build(const std::vector<Group>& groups, hipStream_t st) {
+    ok_ = false;
+    h0_ = nullptr;
+    std::vector<const Group*> gs;
+    for (const Group& g : groups)
+        if (g.h && g.nvec > 0 && g.h->dim() > 0) gs.push_back(&g);
+    if (gs.empty()) return false;
+    const Hamiltonian& h0 = *gs.front()->h;
+    if (!h0.xchg) return false;
+    for (const Group* g : gs) {
+        const Hamiltonian& h = *g->h;
+        if (!h.xchg || h.basis->L() != h0.basis->L() || h.basis->nbits() != h0.basis->nbits() ||
+            h.xparams.nlo != h0.xparams.nlo || h.xparams.n_dmix != 0 || h0.xparams.n_dmix != 0 ||
+            h.dlo_host != h0.dlo_host || h.dhi_host != h0.dhi_host ||
+            h.bonds_host.size() != h0.bonds_host.size() || h.xparams.n_lo != h0.xparams.n_lo ||
+            h.xparams.n_hi != h0.xparams.n_hi || h.xparams.n_mix != h0.xparams.n_mix ||
+            std::memcmp(h.bonds_host.data(), h0.bonds_host.data(), sizeof(XBond) * h.bonds_host.size()) != 0)
+            return false;
+    }
+    std::vector<MTile> mt;
+    std::vector<const u64*> jh;
+    for (size_t s = 0; s < gs.size(); ++s) {
+        const Group& g = *gs[s];
+        jh.push_back(g.h->xparams.jhi);
+        for (const XTile& T : g.h->xtiles_for(64)) {
+            MTile m{};
+            m.base = T.base;
+            m.xoff = g.off;
+            m.ld = g.h->dim();
+            m.pos0 = T.pos0;
+            m.hi = T.hi;
+            m.r0 = T.r0;
+            m.nvalid = T.nvalid;
+            m.sec = u32(s);
+            m.nvec = u32(g.nvec);
+            m.dh = T.dh;
+            mt.push_back(m);
+        }
+    }
+    upload(tiles_, mt.data(), mt.size(), st);
+    upload(jhis_, jh.data(), jh.size(), st);
+    QK_HIP(hipStreamSynchronize(st));
+    mp_.tiles = tiles_.get<MTile>();
+    mp_.jhis = jhis_.get<const u64*>();
+    mp_.ntiles = u32(mt.size());
+    grid_ = grid_for(h0.xlds_bytes, mt.size(), &mp_.tiles_per_xcd, kXWaves);
+    if (h0.xlds_bytes > 64 * 1024)
+        QK_HIP(hipFuncSetAttribute((const void*)k_apply_xchg_multi<4>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, h0.xlds_bytes));
+    h0_ = &h0;
+    ok_ = true;
+    return true;
+}
+
+void MultiApply::launch(const cplx* x, cplx* y, hipStream_t st) const {
+    if (!ok_) fail("MultiApply::launch without a successful build");
+    hipLaunchKernelGGL((k_apply_xchg_multi<4>), dim3(grid_), dim3(64 * kXWaves), h0_->xlds_bytes, st,
+                       h0_->xparams, mp_, x, y);
+    QK_HIP(hipGetLastError());
 }
 
 Hamiltonian::~Hamiltonian() {

@@ -71,6 +71,25 @@ struct XParams {
     u32 tiles_per_xcd;
 };
 
+// Tile of the multi-sector exchange kernel: 64 ranks of one row of sector
+// `sec`, applied to all nvec vectors of that sector (vector v at
+// xoff + v*ld of the concatenated buffers).
+struct MTile {
+    u64 base;      // jhi[hi] of the sector
+    u64 xoff;      // offset of the sector's first vector
+    u64 ld;        // sector dimension (vector stride)
+    u32 pos0, hi, r0, nvalid;
+    u32 sec, nvec;
+    double dh;
+};
+static_assert(sizeof(MTile) == 56, "MTile layout");
+
+struct MParams {
+    const MTile* tiles;
+    const u64* const* jhis;   // per-sector row offsets
+    u32 ntiles, tiles_per_xcd;
+};
+
 // Everything one apply launch needs, passed by value as the kernel argument.
 struct ApplyParams {
     const u16* jlo;
@@ -134,6 +153,8 @@ struct Hamiltonian {
     int vec_group = 2;             // vectors per block-kernel launch (exchange path)
     const void* xs_fn = nullptr;   // the single-vector kernel for (spl, bond_batch)
     std::vector<double> dhi_host;  // high-half diagonal per row
+    std::vector<double> dlo_host;  // low-half diagonal, lo_list order
+    std::vector<XBond> bonds_host;
     std::vector<uint2> tiles_host;   // (hi, r0) per tile, launch order
     ApplyParams params{};
     ApplyParams dparams{};
@@ -149,6 +170,31 @@ struct Hamiltonian {
 
     // scratch for host-buffer entry points
     mutable DevBuf scratch_x, scratch_y;
+};
+
+// y = H_s x for several Hamiltonians (sectors of one model) and several
+// vectors each, in one launch per call: the sectors must be on the exchange
+// path with identical bonds and diagonal tables (same OpSum, site and number
+// of sites; only the sector differs).  Vectors of group g live at
+// off_g + v*dim_g of the buffers passed to launch().
+class MultiApply {
+public:
+    struct Group {
+        const Hamiltonian* h;
+        int nvec;
+        u64 off;
+    };
+    // false when the groups cannot share one launch (callers then apply per group)
+    bool build(const std::vector<Group>& groups, hipStream_t st);
+    void launch(const cplx* x, cplx* y, hipStream_t st) const;
+    bool ok() const { return ok_; }
+
+private:
+    bool ok_ = false;
+    const Hamiltonian* h0_ = nullptr;
+    MParams mp_{};
+    DevBuf tiles_, jhis_;
+    int grid_ = 0;
 };
 
 }  // namespace qk

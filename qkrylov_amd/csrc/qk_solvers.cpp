@@ -16,6 +16,7 @@
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <string>
 #include <thread>
 
 namespace qk {
@@ -268,16 +269,32 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         std::vector<cplx> Hn(size_t(m) * m, cplx{0.0, 0.0});
         for (int i = 0; i < std::min(valid_rows, m); ++i)
             for (int j = 0; j < std::min(valid_rows, m); ++j) Hn[size_t(i) * m + j] = Hs[size_t(i) * valid_rows + j];
-        h.resize(sizeof(cplx) * m);
-        for (int i = valid_rows; i < m; ++i) {
-            vec::mdot(pHV.get<const cplx*>(), i + 1, V[i].get<cplx>(), d, h.get<cplx>(), red, st);
-            std::vector<cplx> hv(i + 1);
-            QK_HIP(hipMemcpyAsync(hv.data(), h.get(), sizeof(cplx) * (i + 1), hipMemcpyDeviceToHost, st));
+        h.resize(sizeof(cplx) * m * std::max(1, m - valid_rows));
+        const int nnew = m - valid_rows;
+        if (nnew > 0 && m <= 64 && nnew <= 16) {
+            // one FP64-MFMA projection G = HV^H V_new for all new rows
+            vec::block_dot(pHV.get<const cplx*>(), m, pV.get<const cplx*>() + valid_rows, nnew, d,
+                           h.get<cplx>(), red, st);
+            std::vector<cplx> G(size_t(m) * nnew);
+            QK_HIP(hipMemcpyAsync(G.data(), h.get(), sizeof(cplx) * G.size(), hipMemcpyDeviceToHost, st));
             QK_HIP(hipStreamSynchronize(st));
-            for (int j = 0; j <= i; ++j) {
-                const cplx vij = cconj(hv[j]);   // <V_i, HV_j>
-                Hn[size_t(i) * m + j] = vij;
-                if (i != j) Hn[size_t(j) * m + i] = cconj(vij);
+            for (int i = valid_rows; i < m; ++i)
+                for (int j = 0; j <= i; ++j) {
+                    const cplx vij = cconj(G[size_t(j) * nnew + (i - valid_rows)]);   // <V_i, HV_j>
+                    Hn[size_t(i) * m + j] = vij;
+                    if (i != j) Hn[size_t(j) * m + i] = cconj(vij);
+                }
+        } else {
+            for (int i = valid_rows; i < m; ++i) {
+                vec::mdot(pHV.get<const cplx*>(), i + 1, V[i].get<cplx>(), d, h.get<cplx>(), red, st);
+                std::vector<cplx> hv(i + 1);
+                QK_HIP(hipMemcpyAsync(hv.data(), h.get(), sizeof(cplx) * (i + 1), hipMemcpyDeviceToHost, st));
+                QK_HIP(hipStreamSynchronize(st));
+                for (int j = 0; j <= i; ++j) {
+                    const cplx vij = cconj(hv[j]);   // <V_i, HV_j>
+                    Hn[size_t(i) * m + j] = vij;
+                    if (i != j) Hn[size_t(j) * m + i] = cconj(vij);
+                }
             }
         }
         Hs = Hn;
@@ -411,6 +428,7 @@ struct BlockRec {
     DevBuf bufU, bufY, bufP, c, cp, bp, alive, alpha, beta, nrm;
     int nsamp = 0;
     hipStream_t st = nullptr;
+    MultiApply multi;   // all groups in one apply launch when they allow it
 
     void setup(const std::vector<std::pair<const Hamiltonian*, int>>& g, hipStream_t stream) {
         st = stream;
@@ -424,6 +442,19 @@ struct BlockRec {
         }
         seg.build(lens, st);
         nsamp = seg.nsamp;
+        // One launch for every group and vector pays off while launches, not
+        // bytes, dominate (small sectors); large bases keep the per-vector
+        // kernels, which the block kernel does not beat there.
+        std::vector<MultiApply::Group> mg;
+        u64 dmax = 0;
+        int ntot = 0;
+        for (const Grp& x : grps) {
+            mg.push_back({x.h, x.nvec, x.off});
+            dmax = std::max(dmax, x.h->dim());
+            ntot += x.nvec;
+        }
+        const char* e = std::getenv("QKRYLOV_MULTI_APPLY");
+        if (!(e && std::string(e) == "0") && ntot > 1 && dmax < (u64(1) << 22)) multi.build(mg, st);
         const size_t bytes = sizeof(cplx) * std::max<u64>(seg.total, 1);
         bufU.resize(bytes);
         bufY.resize(bytes);
@@ -472,7 +503,9 @@ struct BlockRec {
             cplx* Y = bufY.get<cplx>();
             cplx* P = bufP.get<cplx>();
             for (int step = 0; step < M; ++step) {
-                for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
+                if (multi.ok()) multi.launch(U, Y, st);
+                else
+                    for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
                 double* a = alpha.get<double>() + size_t(step) * nsamp;
                 double* b = beta.get<double>() + size_t(step) * nsamp;
                 vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);

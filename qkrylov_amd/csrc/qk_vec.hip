@@ -350,6 +350,90 @@ __global__ void __launch_bounds__(kT) k_seg_norm_fin(const u32* __restrict__ fir
     if (threadIdx.x == 0) nrm[j] = sqrt(s);
 }
 
+
+// ---------------------------------------------------------------------------
+// G = V^H W for m <= 64 vectors V and nb <= 16 vectors W (tall-skinny
+// projection, Davidson's new rows of V^H H V) on FP64 MFMA.
+// v_mfma_f64_16x16x4f64, lane l (r = l % 16, q = l / 16): A[r][q], B[q][r],
+// D[4q + i][r].  The K index q covers two complex entries: q/2 selects the
+// entry, q%2 the real or imaginary part, so
+//   A = (Re v | Im v),  B = (Re w | Im w)    -> Re G = Re v Re w + Im v Im w
+//   A = (Re v | Im v),  B' = (Im w | -Re w)  -> Im G = Re v Im w - Im v Re w.
+// Every vector is read once per launch; per-workgroup partials are summed in
+// a fixed order by k_block_dot_fin.
+// ---------------------------------------------------------------------------
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+constexpr int kBDChunk = 8192;   // complex entries per wave slice
+
+template <int MB>
+__global__ void __launch_bounds__(kT)
+k_block_dot_part(const cplx* const* __restrict__ V, const int m, const cplx* const* __restrict__ W,
+                 const int nw, const u64 n, const u64 per_wave, cplx* __restrict__ part) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4, half = q >> 1, im = q & 1;
+    const cplx* vp[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) vp[b] = (b * 16 + r < m) ? V[b * 16 + r] : nullptr;
+    const cplx* wp = r < nw ? W[r] : nullptr;
+    f64x4 acc_re[MB], acc_im[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        acc_re[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+        acc_im[b] = f64x4{0.0, 0.0, 0.0, 0.0};
+    }
+    const u64 w0 = (u64(blockIdx.x) * 4 + wave) * per_wave;
+    const u64 w1 = min(n, w0 + per_wave);
+    for (u64 c = w0; c < w1; c += 2) {
+        const u64 cc = c + half;
+        const bool in = cc < w1;
+        cplx wv{0.0, 0.0};
+        if (in && wp) wv = wp[cc];
+        const double bre = im ? wv.im : wv.re;
+        const double bsw = im ? -wv.re : wv.im;
+#pragma unroll
+        for (int b = 0; b < MB; ++b) {
+            cplx vv{0.0, 0.0};
+            if (in && vp[b]) vv = vp[b][cc];
+            const double a = im ? vv.im : vv.re;
+            acc_re[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bre, acc_re[b], 0, 0, 0);
+            acc_im[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bsw, acc_im[b], 0, 0, 0);
+        }
+    }
+    // reduce the 4 waves in LDS (fixed order), write this workgroup's partial
+    __shared__ double sh[4][MB * 16 * 16 * 2];
+#pragma unroll
+    for (int b = 0; b < MB; ++b)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 4 * q + i;
+            const int o = ((b * 16 + row) * 16 + r) * 2;
+            sh[wave][o] = acc_re[b][i];
+            sh[wave][o + 1] = acc_im[b][i];
+        }
+    __syncthreads();
+    cplx* dst = part + size_t(blockIdx.x) * (MB * 16 * 16);
+    for (int e = threadIdx.x; e < MB * 16 * 16; e += kT) {
+        const double re = ((sh[0][2 * e] + sh[1][2 * e]) + sh[2][2 * e]) + sh[3][2 * e];
+        const double iv = ((sh[0][2 * e + 1] + sh[1][2 * e + 1]) + sh[2][2 * e + 1]) + sh[3][2 * e + 1];
+        dst[e] = cplx{re, iv};
+    }
+}
+
+// G[i*nw + j] = sum over workgroups (in order) of the partials, i < m, j < nw
+__global__ void k_block_dot_fin(const cplx* __restrict__ part, const int nparts, const int stride,
+                                const int m, const int nw, cplx* __restrict__ G) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * nw) return;
+    const int i = e / nw, j = e % nw;
+    double re = 0.0, iv = 0.0;
+    for (int b = 0; b < nparts; ++b) {
+        const cplx v = part[size_t(b) * stride + i * 16 + j];
+        re += v.re;
+        iv += v.im;
+    }
+    G[e] = cplx{re, iv};
+}
+
 inline u32 nblk(u64 n) { return static_cast<u32>((n + kCH - 1) / kCH); }
 inline u32 ngrid(u64 n) { return static_cast<u32>((n + kT - 1) / kT); }
 
@@ -434,6 +518,30 @@ void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& sc
     hipLaunchKernelGGL(k_mdot_part, dim3(nb), dim3(kT), 0, st, V, k, w, n, scratch.get<cplx>(),
                        nb);
     hipLaunchKernelGGL(k_finish_c, dim3(k), dim3(kT), 0, st, scratch.get<cplx>(), nb, h);
+    QK_HIP(hipGetLastError());
+}
+
+void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n, cplx* G,
+               DevBuf& scratch, hipStream_t st) {
+    if (m <= 0 || nw <= 0) return;
+    if (m > 64 || nw > 16) fail("block_dot: at most 64 x 16 projections per call");
+    const u64 waves = std::max<u64>(1, std::min<u64>((n + kBDChunk - 1) / kBDChunk, 4096));
+    u64 per_wave = (n + waves - 1) / waves;
+    per_wave = (per_wave + 1) & ~u64(1);                // even: pairs of entries
+    const u32 nparts = u32((((n + per_wave - 1) / per_wave) + 3) / 4);
+    const int MB = (m + 15) / 16;
+    const int stride = MB * 16 * 16;
+    scratch.resize(sizeof(cplx) * size_t(std::max<u32>(nparts, 1)) * stride);
+    cplx* part = scratch.get<cplx>();
+    const u32 g = std::max<u32>(nparts, 1);
+    switch (MB) {
+        case 1: hipLaunchKernelGGL((k_block_dot_part<1>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
+        case 2: hipLaunchKernelGGL((k_block_dot_part<2>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
+        case 3: hipLaunchKernelGGL((k_block_dot_part<3>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
+        default: hipLaunchKernelGGL((k_block_dot_part<4>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
+    }
+    hipLaunchKernelGGL(k_block_dot_fin, dim3((m * nw + 255) / 256), dim3(256), 0, st, part, int(g),
+                       stride, m, nw, G);
     QK_HIP(hipGetLastError());
 }
 

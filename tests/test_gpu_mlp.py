@@ -123,14 +123,32 @@ def test_training_lowers_loss_and_davidson_agrees(qk, lat):
     assert net.last_train["solve_rel_res"] < 1e-6
     assert hist[-1] < hist[0]
 
-    H = tj(3, 4, 5)      # d = 16,632
-    res = {}
-    for name, pre in (("jacobi", None), ("mlp", net)):
-        t = time.time()
-        r = qk.davidson_lowest_ex(H, 4, 32, 1e-8, 300, pre)
-        res[name] = (r, time.time() - t)
-        print(f"[davidson 3x4] {name}: iterations={r.iterations} converged={r.converged} "
-              f"time={res[name][1]:.2f} s ev={r.eigenvalues}", flush=True)
-    rj, rm = res["jacobi"][0], res["mlp"][0]
-    assert rj.converged and rm.converged
-    np.testing.assert_allclose(rm.eigenvalues, rj.eigenvalues, rtol=0, atol=1e-10)
+    # Davidson on the training lattice and on 2x5 (d = 3,150): both methods
+    # converge and agree; the iteration counts are printed (the gain on the
+    # 4x4 target lattice is measured by bench.py)
+    for H, tag in ((Hs, "2x4"), (tj(2, 5, 4), "2x5")):
+        res = {}
+        for name, pre in (("jacobi", None), ("mlp", net)):
+            t = time.time()
+            r = qk.davidson_lowest_ex(H, 4, 24, 1e-8, 400, pre)
+            res[name] = (r, time.time() - t)
+            print(f"[davidson {tag}] {name}: iterations={r.iterations} converged={r.converged} "
+                  f"time={res[name][1]:.2f} s ev={r.eigenvalues}", flush=True)
+        rj, rm = res["jacobi"][0], res["mlp"][0]
+        assert rj.converged and rm.converged
+        np.testing.assert_allclose(rm.eigenvalues, rj.eigenvalues, rtol=0, atol=1e-10)
+
+
+def test_block_dot_matches_numpy(qk):
+    import ctypes as ct
+    lib = qk.load()
+    rng = np.random.default_rng(5)
+    for m, nw, n in ((1, 1, 1), (5, 3, 1001), (37, 16, 20000), (64, 7, 99)):
+        V = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+        W = rng.standard_normal((nw, n)) + 1j * rng.standard_normal((nw, n))
+        G = np.empty((m, nw), dtype=np.complex128)
+        assert lib.qkrylov_block_dot_f64(V.ctypes.data, m, W.ctypes.data, nw, n, G.ctypes.data) == 0
+        ref = V.conj() @ W.T
+        print(f"[block_dot] m={m} nw={nw} n={n} max rel={np.max(np.abs(G - ref)) / np.max(np.abs(ref)):.1e}",
+              flush=True)
+        np.testing.assert_allclose(G, ref, rtol=0, atol=1e-13 * np.max(np.abs(ref)) * max(1, np.sqrt(n)))
