@@ -92,7 +92,8 @@ def cpu_baseline(lat, n_rows=4_000_000, reps=15):
         ts.append(time.perf_counter() - t)
     med = float(np.median(ts))
     rows_per_s = n_rows / med
-    return {
+    del part
+    out = {
         "value": rows_per_s / d,
         "unit": "applies/s",
         "cores": int(threads),
@@ -101,6 +102,26 @@ def cpu_baseline(lat, n_rows=4_000_000, reps=15):
                    f"{setup:.1f} s; median of {reps} gather SpMVs after 2 warm-ups = "
                    f"{med * 1e3:.1f} ms, scaled to whole applies"),
     }
+    # the same CPU path for the solver legs: Lanczos on the N=16 chain (whole
+    # run) and FTLM on the N=20 XXZ sectors with 2 of the 64 vectors per sector
+    n16 = orc.Ham(orc.Basis("spin", 16, {"use_sz": True, "sz2": 0}), "spin", lat.heisenberg_chain(16))
+    t = time.perf_counter()
+    e16, it16, _, _ = n16.lanczos(200, 1e-12)
+    out["lanczos_n16"] = {"time_to_ground_state_s": time.perf_counter() - t, "E0": e16,
+                          "iterations": it16}
+    temps = 0.05 * 200.0 ** (np.arange(200) / 199.0)
+    t = time.perf_counter()
+    nsamp = 0
+    for k in range(21):
+        oh = orc.Ham(orc.Basis("spin", 20, {"use_sz": True, "sz2": 2 * k - 20}), "spin",
+                     lat.xxz_chain(20, 1.0, 1.5))
+        oh.ftlm(1.0 / temps, 2, 100)
+        nsamp += 2
+    dt = time.perf_counter() - t
+    out["ftlm"] = {"samples_per_s": nsamp / dt, "sample": "21 sectors x 2 vectors, M=100, 200 "
+                   "temperatures, CSR build included", "time_s": dt}
+    log(f"cpu: lanczos N=16 {out['lanczos_n16']}, ftlm {out['ftlm']}")
+    return out
 
 
 def bench_ftlm(qk, lat):
@@ -151,7 +172,16 @@ def bench_ftlm(qk, lat):
 def bench_lanczos_j1j2(qk, lat):
     """J1-J2 Heisenberg on the 6x4 periodic square lattice (J1=1, J2=0.5, Sz=0,
     d=2,704,156): 300 Lanczos steps with full reorthogonalisation (tol 0),
-    ground energy and the residual of the Ritz vector."""
+    ground energy and the residual of the Ritz vector.  Also the N=16 chain
+    (d=12,870) to tol 1e-12, the reference's CPU-runnable case."""
+    H16 = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(16, sz=0), qk.SpinHalfSite(),
+                                   lat.heisenberg_chain(16))
+    qk.lanczos_ground_state(H16, maxiter=5)       # warm-up
+    t = time.perf_counter()
+    r16 = qk.lanczos_ground_state(H16, maxiter=200, tol=1e-12)
+    n16 = {"time_to_ground_state_s": time.perf_counter() - t, "E0": r16.energy,
+           "iterations": r16.iterations}
+    log(f"lanczos N=16: {n16}")
     t = time.perf_counter()
     H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(24, sz=0), qk.SpinHalfSite(),
                                  lat.j1j2_square(6, 4, 1.0, 0.5))
@@ -167,7 +197,7 @@ def bench_lanczos_j1j2(qk, lat):
                         "300 Lanczos steps, full reorthogonalisation, reference start vector",
             "dim": H.dimension, "setup_s": setup, "time_to_ground_state_s": dt,
             "iterations": r.iterations, "E0": r.energy, "E0_per_site": r.energy / 24,
-            "ritz_residual": res}
+            "ritz_residual": res, "chain_n16": n16}
 
 
 def bench_davidson_tj(qk, lat, msub=48, max_iter=400):

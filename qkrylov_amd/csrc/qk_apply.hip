@@ -746,7 +746,10 @@ static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
     const int cus = device_cu_count();
     const int by_lds = (160 * 1024) / std::max(lds, 1);
     const int by_waves = 32 / waves;   // 8 waves per SIMD, 4 SIMDs per CU
-    const int per_cu = std::max(1, std::min({8, by_lds, by_waves}));
+    int per_cu = std::max(1, std::min({8, by_lds, by_waves}));
+    // fewer resident workgroups keep fewer tiles in flight per XCD (a smaller
+    // L2 working set) at the cost of latency hiding; the autotuner tries it
+    if (const char* e = std::getenv("QKRYLOV_WG_PER_CU")) per_cu = std::max(1, std::min(per_cu, std::atoi(e)));
     const int want = int((ntiles + waves - 1) / waves);
     int blocks = std::min(want, cus * per_cu);
     blocks = std::max(kXcd, (blocks + kXcd - 1) / kXcd * kXcd);
