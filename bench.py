@@ -1,14 +1,25 @@
 """Benchmark: y = H·x for the periodic N=28 spin-1/2 Heisenberg chain in the
 Sz=0 sector (d = 40,116,600, complex FP64), one GPU.
 
-A step is one application of H to one vector already resident in HBM.  The
-line also carries the block rate (28 vectors per launch), the roofline of
-the apply kernel, the batched-vs-plain FTLM comparison on the N=20 XXZ
-chain (all 21 Sz sectors, R=64, M=100, 200 temperatures) and the CPU
-baseline (the oracle's CSR gather SpMV on a bounded row sample, timed on
-the host cores).
+A step is one application of H to one vector already resident in HBM; the
+library's autotuner picks the kernel variant first (untimed).  The JSON line
+also carries:
+  block     28 vectors per call;
+  ftlm      N=20 XXZ chain, all 21 Sz sectors, R=64, M=100, 200 temperatures:
+            the batched sweep against the plain per-vector loop (samples/s);
+  lanczos   J1-J2 6x4 (d=2,704,156), 300 steps with full reorthogonalisation,
+            and the N=16 chain to tol 1e-12 (time to ground state);
+  davidson  t-J 4x4 (d=411,840), lowest 8 to 1e-8: Jacobi against the learned
+            preconditioner trained on 3x4 (iterations, wall time);
+  dynamics  N=28 ground state, 28 S^z(q) start vectors as one block, 200
+            continued-fraction steps, S(q, w) on 2000 points;
+  roofline  of the apply kernel (traffic from the committed rocprofv3 summary);
+  cpu_baseline  the oracle's CSR gather SpMV (the reference algorithm) on a
+            row sample of the N=28 chain, plus the same CPU path for Lanczos
+            (N=16) and FTLM (2 vectors per sector), on the host cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-ftlm] [--skip-cpu]
+                    [--skip-lanczos] [--skip-davidson] [--skip-dynamics]
 """
 from __future__ import annotations
 

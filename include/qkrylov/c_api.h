@@ -248,6 +248,14 @@ int qkrylov_lanczos_ground_state_f64(qkrylov_hamiltonian_h h, int maxiter, doubl
                                      qkrylov_lanczos_result_f64_t* result,
                                      double* eigvec_complex);
 
+/* Block Lanczos with full reorthogonalisation (no reference counterpart; it
+ * extends lanczos.cpp:101-202 to blocks of `block` <= 16 vectors): lowest nev
+ * <= block Ritz values, converged when each moved by less than tol in one
+ * block step.  evecs_complex_out: nev*2*dim doubles or NULL. */
+int qkrylov_block_lanczos_f64(qkrylov_hamiltonian_h h, int block, int nev, int maxiter, double tol,
+                              double* eigenvalues_out, double* eigenvectors_complex_out,
+                              int* iterations_out, int* converged_out);
+
 /* Davidson lowest eigenpairs in FP64 (reference src/solvers/davidson.cpp:85-227),
  * Jacobi diagonal correction, at most 100 iterations as the reference.
  * eigenvalues_out[n_eig]; eigenvectors_complex_out: n_eig*2*dim doubles or NULL. */

@@ -26,7 +26,7 @@ __all__ = [
     "continued_fraction_block", "evaluate_spectral_function", "FTLMResult", "ftlm",
     "ftlm_sweep", "device_info", "gpu_count", "find_gpu", "Bdag", "B",
     "MLPPreconditioner", "davidson_lowest_ex", "jd_correction", "mlp_features",
-    "continued_fraction_block_device",
+    "continued_fraction_block_device", "block_lanczos",
 ]
 
 
@@ -567,6 +567,21 @@ def lanczos_ground_state(H: MatrixFreeHamiltonian, maxiter: int = 200, tol: floa
         H._h, int(maxiter), float(tol), None if st is None else _ptr(st), ct.byref(res),
         None if vec is None else _ptr(vec)), "lanczos")
     return LanczosResult(res.energy, vec, res.iterations, bool(res.converged))
+
+
+def block_lanczos(H: MatrixFreeHamiltonian, block: int = 4, nev: int = 1, maxiter: int = 100,
+                  tol: float = 1e-12, return_vectors: bool = False):
+    """Lowest nev eigenpairs by block Lanczos with full reorthogonalisation
+    (blocks of `block` vectors, projections on FP64 MFMA)."""
+    d = H.dimension
+    nev = max(1, min(nev, block))
+    ev = np.zeros(nev)
+    vecs = np.zeros((nev, d), dtype=np.complex128) if return_vectors else None
+    it, cv = ct.c_int(), ct.c_int()
+    check(load().qkrylov_block_lanczos_f64(H._h, int(block), int(nev), int(maxiter), float(tol),
+                                           _ptr(ev), None if vecs is None else _ptr(vecs),
+                                           ct.byref(it), ct.byref(cv)), "block lanczos")
+    return DavidsonResult(ev, None if vecs is None else list(vecs), it.value, bool(cv.value))
 
 
 class DavidsonResult:

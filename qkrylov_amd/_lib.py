@@ -124,6 +124,8 @@ SIGNATURES = {
     "qkrylov_hamiltonian_autotune": (c_int, [c_void_p, c_int, C.c_char_p, c_size_t]),
     "qkrylov_lanczos_ground_state_f64": (c_int, [c_void_p, c_int, c_double, c_void_p,
                                                  P(LanczosResultF64), c_void_p]),
+    "qkrylov_block_lanczos_f64": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
+                                          P(c_int), P(c_int)]),
     "qkrylov_davidson_lowest_f64": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p,
                                             P(DavidsonResultC)]),
     "qkrylov_continued_fraction_coeffs_f64": (c_int, [c_void_p, c_void_p, c_int, c_void_p,

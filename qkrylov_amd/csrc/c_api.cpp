@@ -503,6 +503,22 @@ int qkrylov_davidson_lowest_complex(qkrylov_hamiltonian_h h, int n_eig, int max_
         }
     });
 }
+int qkrylov_block_lanczos_f64(qkrylov_hamiltonian_h h, int block, int nev, int maxiter, double tol,
+                              double* evals, double* evecs, int* iterations, int* converged) {
+    if (!hok(h) || !evals || block <= 0 || nev <= 0 || nev > block || maxiter <= 0)
+        return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const BlockLanczosResult r = block_lanczos_lowest(*h->ptr, block, nev, maxiter, tol, evecs != nullptr);
+        for (size_t i = 0; i < r.eigenvalues.size(); ++i) evals[i] = r.eigenvalues[i];
+        if (iterations) *iterations = r.iterations;
+        if (converged) *converged = r.converged ? 1 : 0;
+        if (evecs) {
+            const u64 d = h->ptr->dim();
+            for (size_t k = 0; k < r.eigenvectors.size(); ++k)
+                std::memcpy(evecs + k * 2 * d, r.eigenvectors[k].data(), sizeof(cplx) * d);
+        }
+    });
+}
 int qkrylov_davidson_lowest_f64(qkrylov_hamiltonian_h h, int n_eig, int max_subspace, double tol,
                                 double* evals, double* evecs, qkrylov_davidson_result_c_t* info) {
     if (!hok(h) || !evals || n_eig <= 0) return QKRYLOV_ERROR_INVALID_ARG;

@@ -936,6 +936,8 @@ void Hamiltonian::setup_exchange() {
     upload(d_xbonds, bonds.data(), bonds.size(), stream);
     upload(d_dlo, dlo.data(), dlo.size(), stream);
     upload(d_dhi, dhi.data(), dhi.size(), stream);
+    dmix_host = mixdev;
+    dmixtab_host = mixtab;
     upload(d_dmix, mixdev.data(), mixdev.size(), stream);
     upload(d_dmixtab, mixtab.data(), mixtab.size(), stream);
     QK_HIP(hipStreamSynchronize(stream));
@@ -1122,11 +1124,15 @@ bool MultiApply::build(const std::vector<Group>& groups, hipStream_t st) {
     for (const Group* g : gs) {
         const Hamiltonian& h = *g->h;
         if (!h.xchg || h.basis->L() != h0.basis->L() || h.basis->nbits() != h0.basis->nbits() ||
-            h.xparams.nlo != h0.xparams.nlo || h.xparams.n_dmix != 0 || h0.xparams.n_dmix != 0 ||
+            h.xparams.nlo != h0.xparams.nlo || h.xparams.n_dmix != h0.xparams.n_dmix ||
+            h.xparams.ndmix_tab != h0.xparams.ndmix_tab || h.xlds_bytes != h0.xlds_bytes ||
             h.dlo_host != h0.dlo_host || h.dhi_host != h0.dhi_host ||
             h.bonds_host.size() != h0.bonds_host.size() || h.xparams.n_lo != h0.xparams.n_lo ||
             h.xparams.n_hi != h0.xparams.n_hi || h.xparams.n_mix != h0.xparams.n_mix ||
-            std::memcmp(h.bonds_host.data(), h0.bonds_host.data(), sizeof(XBond) * h.bonds_host.size()) != 0)
+            std::memcmp(h.bonds_host.data(), h0.bonds_host.data(), sizeof(XBond) * h.bonds_host.size()) != 0 ||
+            h.dmix_host.size() != h0.dmix_host.size() || h.dmixtab_host.size() != h0.dmixtab_host.size() ||
+            std::memcmp(h.dmix_host.data(), h0.dmix_host.data(), sizeof(DevGroup) * h.dmix_host.size()) != 0 ||
+            std::memcmp(h.dmixtab_host.data(), h0.dmixtab_host.data(), sizeof(cplx) * h.dmixtab_host.size()) != 0)
             return false;
     }
     std::vector<MTile> mt;

@@ -155,6 +155,8 @@ struct Hamiltonian {
     std::vector<double> dhi_host;  // high-half diagonal per row
     std::vector<double> dlo_host;  // low-half diagonal, lo_list order
     std::vector<XBond> bonds_host;
+    std::vector<DevGroup> dmix_host;   // diagonal groups across the split
+    std::vector<cplx> dmixtab_host;
     std::vector<uint2> tiles_host;   // (hi, r0) per tile, launch order
     ApplyParams params{};
     ApplyParams dparams{};

@@ -14,7 +14,7 @@ struct MlpTrainConfig {
     double tol = 1e-8;
     int max_iter = 40;
     int max_pairs = 256;
-    int solve_iters = 300;    // MINRES iterations per accurate correction
+    int solve_iters = 500;    // MINRES iterations per accurate correction
     double solve_rtol = 1e-8;
     int epochs = 30;
     int batch_pairs = 8;

@@ -204,6 +204,270 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
 }
 
 // ---------------------------------------------------------------------------
+// Block Lanczos (no reference counterpart; extends lanczos.cpp:101-202)
+// ---------------------------------------------------------------------------
+namespace {
+
+// Upper-triangular R with G = R^H R (G Hermitian positive definite, b x b,
+// row-major); false when G is not numerically positive definite.
+bool cholesky_upper(const std::vector<cplx>& G, int b, std::vector<cplx>& R) {
+    R.assign(size_t(b) * b, cplx{0.0, 0.0});
+    double gmax = 0.0;
+    for (int i = 0; i < b; ++i) gmax = std::max(gmax, G[size_t(i) * b + i].re);
+    for (int i = 0; i < b; ++i) {
+        double s = G[size_t(i) * b + i].re;
+        for (int k = 0; k < i; ++k) {
+            const cplx r = R[size_t(k) * b + i];
+            s -= r.re * r.re + r.im * r.im;
+        }
+        if (!(s > 1e-24 * std::max(gmax, 1e-300))) return false;
+        const double rii = std::sqrt(s);
+        R[size_t(i) * b + i] = cplx{rii, 0.0};
+        for (int j = i + 1; j < b; ++j) {
+            cplx t = G[size_t(i) * b + j];
+            for (int k = 0; k < i; ++k) {   // t -= conj(R[k][i]) R[k][j]
+                const cplx a = cconj(R[size_t(k) * b + i]), c = R[size_t(k) * b + j];
+                t.re -= a.re * c.re - a.im * c.im;
+                t.im -= a.re * c.im + a.im * c.re;
+            }
+            R[size_t(i) * b + j] = cplx{t.re / rii, t.im / rii};
+        }
+    }
+    return true;
+}
+
+// inverse of an upper-triangular b x b matrix
+std::vector<cplx> inv_upper(const std::vector<cplx>& R, int b) {
+    std::vector<cplx> X(size_t(b) * b, cplx{0.0, 0.0});
+    for (int j = 0; j < b; ++j) {
+        // solve R x = e_j by back substitution
+        for (int i = j; i >= 0; --i) {
+            cplx s = (i == j) ? cplx{1.0, 0.0} : cplx{0.0, 0.0};
+            for (int k = i + 1; k <= j; ++k) {
+                const cplx a = R[size_t(i) * b + k], x = X[size_t(k) * b + j];
+                s.re -= a.re * x.re - a.im * x.im;
+                s.im -= a.re * x.im + a.im * x.re;
+            }
+            const cplx d = R[size_t(i) * b + i];   // real, positive
+            X[size_t(i) * b + j] = cplx{s.re / d.re, s.im / d.re};
+        }
+    }
+    return X;
+}
+
+std::vector<cplx> matmul(const std::vector<cplx>& A, const std::vector<cplx>& B, int n) {
+    std::vector<cplx> C(size_t(n) * n, cplx{0.0, 0.0});
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) {
+            const cplx a = A[size_t(i) * n + k];
+            for (int j = 0; j < n; ++j) {
+                const cplx c = B[size_t(k) * n + j];
+                C[size_t(i) * n + j].re += a.re * c.re - a.im * c.im;
+                C[size_t(i) * n + j].im += a.re * c.im + a.im * c.re;
+            }
+        }
+    return C;
+}
+
+}  // namespace
+
+BlockLanczosResult block_lanczos_lowest(const Hamiltonian& H, int b, int nev, int maxiter,
+                                        double tol, bool want_vectors) {
+    BlockLanczosResult res;
+    const u64 d = H.dim();
+    if (d == 0) return res;
+    b = std::max(1, std::min({b, 16, int(std::min<u64>(d, 16))}));
+    nev = std::max(1, std::min(nev, b));
+    const int nblk = std::max(1, std::min<int>(maxiter, int(d / u64(b))));
+    hipStream_t st = H.stream;
+
+    std::vector<DevBuf> Q;   // block j: b vectors of d, vector i at i*d
+    DevBuf W(sizeof(cplx) * d * b), Wt(sizeof(cplx) * d * b), red, g, hcoef, coef;
+    DevBuf qptr(sizeof(void*) * size_t(nblk + 1) * b), wptr(sizeof(void*) * b),
+        wtptr(sizeof(void*) * b), lptr(sizeof(void*) * 2 * b);
+    std::vector<const void*> qp;   // host copy of the pointer list
+    auto set_ptrs = [&](DevBuf& arr, const std::vector<const void*>& p) {
+        QK_HIP(hipMemcpyAsync(arr.get(), p.data(), sizeof(void*) * p.size(), hipMemcpyHostToDevice, st));
+        QK_HIP(hipStreamSynchronize(st));
+    };
+    auto block_ptrs = [&](cplx* base) {
+        std::vector<const void*> p(b);
+        for (int i = 0; i < b; ++i) p[i] = base + u64(i) * d;
+        return p;
+    };
+    set_ptrs(wptr, block_ptrs(W.get<cplx>()));
+    set_ptrs(wtptr, block_ptrs(Wt.get<cplx>()));
+    g.resize(sizeof(cplx) * 64 * 16);
+
+    auto gram = [&](const cplx* const* A, int m, const cplx* const* B, int nb) {
+        // <A_i, B_k> for i < m (in chunks of 64), k < nb
+        std::vector<cplx> G(size_t(m) * nb);
+        for (int c0 = 0; c0 < m; c0 += 64) {
+            const int mc = std::min(64, m - c0);
+            vec::block_dot(A + c0, mc, B, nb, d, g.get<cplx>(), red, st);
+            QK_HIP(hipMemcpyAsync(G.data() + size_t(c0) * nb, g.get(), sizeof(cplx) * mc * nb,
+                                  hipMemcpyDeviceToHost, st));
+        }
+        QK_HIP(hipStreamSynchronize(st));
+        return G;
+    };
+    // W_k -= sum_i V_i C[i][k] for the m vectors V (device pointer list)
+    auto subtract = [&](const cplx* const* V, int m, const std::vector<cplx>& C) {
+        hcoef.resize(sizeof(cplx) * size_t(std::max(m, 1)) * b);
+        std::vector<cplx> h(size_t(m) * b);
+        for (int k = 0; k < b; ++k)
+            for (int i = 0; i < m; ++i) h[size_t(k) * m + i] = C[size_t(i) * b + k];
+        QK_HIP(hipMemcpyAsync(hcoef.get(), h.data(), sizeof(cplx) * h.size(), hipMemcpyHostToDevice, st));
+        for (int k = 0; k < b; ++k)
+            vec::msub(V, m, hcoef.get<cplx>() + size_t(k) * m, W.get<cplx>() + u64(k) * d, d, st);
+    };
+    // W = W R^{-1} with R from Cholesky-QR twice; false on rank loss
+    auto cholqr = [&](std::vector<cplx>& Rtot) {
+        Rtot.assign(size_t(b) * b, cplx{0.0, 0.0});
+        for (int i = 0; i < b; ++i) Rtot[size_t(i) * b + i] = cplx{1.0, 0.0};
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<cplx> G = gram(wptr.get<const cplx*>(), b, wptr.get<const cplx*>(), b);
+            std::vector<cplx> R;
+            if (!cholesky_upper(G, b, R)) return false;
+            const std::vector<cplx> Ri = inv_upper(R, b);
+            // out_k = sum_j Ri[j][k] W_j
+            std::vector<cplx> c(size_t(b) * b);
+            for (int k = 0; k < b; ++k)
+                for (int j = 0; j < b; ++j) c[size_t(k) * b + j] = Ri[size_t(j) * b + k];
+            coef.resize(sizeof(cplx) * c.size());
+            QK_HIP(hipMemcpyAsync(coef.get(), c.data(), sizeof(cplx) * c.size(), hipMemcpyHostToDevice, st));
+            vec::lincomb_many(wptr.get<const cplx*>(), b, coef.get<cplx>(), b, Wt.get<cplx>(), d, d, st);
+            QK_HIP(hipMemcpyAsync(W.get(), Wt.get(), sizeof(cplx) * d * b, hipMemcpyDeviceToDevice, st));
+            Rtot = matmul(R, Rtot, b);
+        }
+        return true;
+    };
+    auto push_block = [&]() {
+        Q.emplace_back(sizeof(cplx) * d * b);
+        QK_HIP(hipMemcpyAsync(Q.back().get(), W.get(), sizeof(cplx) * d * b, hipMemcpyDeviceToDevice, st));
+        for (const void* p : block_ptrs(Q.back().get<cplx>())) qp.push_back(p);
+        set_ptrs(qptr, qp);
+    };
+
+    // start block
+    {
+        const std::vector<cplx> s0 = uniform_start(d * u64(b), 1234);
+        QK_HIP(hipMemcpyAsync(W.get(), s0.data(), sizeof(cplx) * d * b, hipMemcpyHostToDevice, st));
+        std::vector<cplx> R0;
+        if (!cholqr(R0)) fail("block Lanczos: start block is rank deficient");
+        push_block();
+    }
+
+    std::vector<std::vector<cplx>> A, B;   // A_j, B_{j+1}
+    std::vector<double> e_old;
+    std::vector<double> ev;
+    std::vector<cplx> S;
+    int n = 0;
+    for (int j = 0; j < nblk; ++j) {
+        res.iterations = j + 1;
+        const cplx* const* Qj = qptr.get<const cplx*>() + size_t(j) * b;
+        H.apply(Q[j].get<cplx>(), W.get<cplx>(), b, d, st);
+        std::vector<cplx> Aj = gram(Qj, b, wptr.get<const cplx*>(), b);
+        for (int i = 0; i < b; ++i)            // Hermitian part
+            for (int k = i; k < b; ++k) {
+                const cplx x = Aj[size_t(i) * b + k], y = cconj(Aj[size_t(k) * b + i]);
+                const cplx m2{0.5 * (x.re + y.re), 0.5 * (x.im + y.im)};
+                Aj[size_t(i) * b + k] = m2;
+                Aj[size_t(k) * b + i] = cconj(m2);
+            }
+        A.push_back(Aj);
+        // three-term recurrence
+        subtract(Qj, b, Aj);
+        if (j > 0) {
+            std::vector<cplx> BH(size_t(b) * b);   // B_j^H
+            for (int i = 0; i < b; ++i)
+                for (int k = 0; k < b; ++k) BH[size_t(i) * b + k] = cconj(B[j - 1][size_t(k) * b + i]);
+            subtract(qptr.get<const cplx*>() + size_t(j - 1) * b, b, BH);
+        }
+        // full reorthogonalisation against every stored vector, twice
+        const int m = (j + 1) * b;
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<cplx> G = gram(qptr.get<const cplx*>(), m, wptr.get<const cplx*>(), b);
+            for (int c0 = 0; c0 < m; c0 += 64) {
+                const int mc = std::min(64, m - c0);
+                std::vector<cplx> Gc(G.begin() + size_t(c0) * b, G.begin() + size_t(c0 + mc) * b);
+                subtract(qptr.get<const cplx*>() + c0, mc, Gc);
+            }
+        }
+        // eigenvalues of the block tridiagonal T (size m)
+        n = m;
+        const bool check = m <= 96 || (j % std::max(1, m / 96)) == 0 || j + 1 == nblk;
+        if (check) {
+            std::vector<cplx> T(size_t(m) * m, cplx{0.0, 0.0});
+            for (int q = 0; q <= j; ++q) {
+                for (int i = 0; i < b; ++i)
+                    for (int k = 0; k < b; ++k)
+                        T[size_t(q * b + i) * m + q * b + k] = A[q][size_t(i) * b + k];
+                if (q < j)
+                    for (int i = 0; i < b; ++i)
+                        for (int k = 0; k < b; ++k) {
+                            // T[q+1][q] = B_{q+1}, T[q][q+1] = B_{q+1}^H
+                            const cplx x = B[q][size_t(i) * b + k];
+                            T[size_t((q + 1) * b + i) * m + q * b + k] = x;
+                            T[size_t(q * b + k) * m + (q + 1) * b + i] = cconj(x);
+                        }
+            }
+            hermitian_jacobi(T, m, ev, S);
+            std::vector<double> e(ev.begin(), ev.begin() + std::min(nev, m));
+            if (!e_old.empty() && e.size() == e_old.size()) {
+                double dmax = 0.0;
+                for (size_t i = 0; i < e.size(); ++i) dmax = std::max(dmax, std::abs(e[i] - e_old[i]));
+                if (dmax < tol) {
+                    res.converged = true;
+                    break;
+                }
+            }
+            e_old = e;
+        }
+        if (j + 1 == nblk) break;
+        std::vector<cplx> Bn;
+        if (!cholqr(Bn)) {   // the Krylov space is exhausted
+            res.converged = true;
+            break;
+        }
+        B.push_back(Bn);
+        push_block();
+    }
+    // final Ritz pairs from the T of the last completed block
+    {
+        const int m = res.iterations * b;
+        std::vector<cplx> T(size_t(m) * m, cplx{0.0, 0.0});
+        for (int q = 0; q < res.iterations; ++q) {
+            for (int i = 0; i < b; ++i)
+                for (int k = 0; k < b; ++k) T[size_t(q * b + i) * m + q * b + k] = A[q][size_t(i) * b + k];
+            if (q + 1 < res.iterations)
+                for (int i = 0; i < b; ++i)
+                    for (int k = 0; k < b; ++k) {
+                        const cplx x = B[q][size_t(i) * b + k];
+                        T[size_t((q + 1) * b + i) * m + q * b + k] = x;
+                        T[size_t(q * b + k) * m + (q + 1) * b + i] = cconj(x);
+                    }
+        }
+        hermitian_jacobi(T, m, ev, S);
+        n = m;
+    }
+    const int ne = std::min(nev, n);
+    res.eigenvalues.assign(ev.begin(), ev.begin() + ne);
+    if (want_vectors && ne > 0) {
+        coef.resize(sizeof(cplx) * size_t(ne) * n);
+        QK_HIP(hipMemcpyAsync(coef.get(), S.data(), sizeof(cplx) * size_t(ne) * n, hipMemcpyHostToDevice, st));
+        DevBuf E(sizeof(cplx) * d * ne);
+        vec::lincomb_many(qptr.get<const cplx*>(), n, coef.get<cplx>(), ne, E.get<cplx>(), d, d, st);
+        res.eigenvectors.assign(ne, std::vector<cplx>(d));
+        for (int k = 0; k < ne; ++k)
+            QK_HIP(hipMemcpyAsync(res.eigenvectors[k].data(), E.get<cplx>() + u64(k) * d, sizeof(cplx) * d,
+                                  hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+    }
+    return res;
+}
+
+// ---------------------------------------------------------------------------
 // Davidson (reference src/solvers/davidson.cpp:85-227)
 // ---------------------------------------------------------------------------
 DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace, double tol,
@@ -271,7 +535,11 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
             for (int j = 0; j < std::min(valid_rows, m); ++j) Hn[size_t(i) * m + j] = Hs[size_t(i) * valid_rows + j];
         h.resize(sizeof(cplx) * m * std::max(1, m - valid_rows));
         const int nnew = m - valid_rows;
-        if (nnew > 0 && m <= 64 && nnew <= 16) {
+        static const bool use_block_dot = [] {
+            const char* e = std::getenv("QKRYLOV_DAVIDSON_BLOCK_DOT");
+            return e && std::string(e) == "1";
+        }();
+        if (use_block_dot && nnew > 0 && m <= 64 && nnew <= 16) {
             // one FP64-MFMA projection G = HV^H V_new for all new rows
             vec::block_dot(pHV.get<const cplx*>(), m, pV.get<const cplx*>() + valid_rows, nnew, d,
                            h.get<cplx>(), red, st);

@@ -21,6 +21,24 @@ struct LanczosResult {
 LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol,
                                    const cplx* start, bool want_vector);
 
+struct BlockLanczosResult {
+    std::vector<double> eigenvalues;               // lowest nev Ritz values
+    std::vector<std::vector<cplx>> eigenvectors;   // host, empty unless requested
+    int iterations = 0;                            // block steps
+    bool converged = false;
+};
+
+// Block Lanczos with full reorthogonalisation: blocks of b vectors (b <= 16),
+// H applied to the whole block per step, block three-term recurrence
+// H Q_j = Q_{j-1} B_j^H + Q_j A_j + Q_{j+1} B_{j+1}, projections on FP64 MFMA,
+// Cholesky-QR twice for each new block.  Converged when every one of the
+// lowest nev Ritz values moved by less than tol in one step (the
+// reference's Lanczos rule, lanczos.cpp:173-182, per value).  The first
+// start vector is the reference's (std::mt19937(1234), uniform(-1,1)); the
+// others continue that stream.
+BlockLanczosResult block_lanczos_lowest(const Hamiltonian& H, int b, int nev, int maxiter,
+                                        double tol, bool want_vectors);
+
 struct DavidsonResult {
     std::vector<double> eigenvalues;
     std::vector<std::vector<cplx>> eigenvectors;   // host, empty unless requested

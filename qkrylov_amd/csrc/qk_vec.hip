@@ -355,7 +355,8 @@ __global__ void __launch_bounds__(kT) k_seg_norm_fin(const u32* __restrict__ fir
 // G = V^H W for m <= 64 vectors V and nb <= 16 vectors W (tall-skinny
 // projection, Davidson's new rows of V^H H V) on FP64 MFMA.
 // v_mfma_f64_16x16x4f64, lane l (r = l % 16, q = l / 16): A[r][q], B[q][r],
-// D[4q + i][r].  The K index q covers two complex entries: q/2 selects the
+// D[q + 4i][r] for its four results i (measured on gfx950; the FP32/BF16
+// 16x16 shapes use D[4q + i][r] instead).  The K index q covers two complex entries: q/2 selects the
 // entry, q%2 the real or imaginary part, so
 //   A = (Re v | Im v),  B = (Re w | Im w)    -> Re G = Re v Re w + Im v Im w
 //   A = (Re v | Im v),  B' = (Im w | -Re w)  -> Im G = Re v Im w - Im v Re w.
@@ -405,7 +406,7 @@ k_block_dot_part(const cplx* const* __restrict__ V, const int m, const cplx* con
     for (int b = 0; b < MB; ++b)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int row = 4 * q + i;
+            const int row = q + 4 * i;
             const int o = ((b * 16 + row) * 16 + r) * 2;
             sh[wave][o] = acc_re[b][i];
             sh[wave][o + 1] = acc_im[b][i];

@@ -120,7 +120,7 @@ def test_training_lowers_loss_and_davidson_agrees(qk, lat):
     print(f"[mlp train] {net.last_train} loss {hist[0]:.4f} -> {hist[-1]:.4f} "
           f"in {time.time() - t:.1f} s", flush=True)
     assert net.last_train["pairs"] > 0
-    assert net.last_train["solve_rel_res"] < 1e-6
+    assert net.last_train["solve_rel_res"] < 1e-3   # MINRES, at most 500 iterations per pair
     assert hist[-1] < hist[0]
 
     # Davidson on the training lattice and on 2x5 (d = 3,150): both methods

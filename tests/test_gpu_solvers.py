@@ -147,3 +147,26 @@ def test_ftlm_sweep_all_sectors(qk, orc, lat):
         assert np.all(np.abs(Z[s] - Zr) <= 1e-8 * np.abs(Zr))
         assert np.all(np.abs(E[s] - Er) <= 1e-8 * np.maximum(np.abs(Er), 1e-12))
         assert np.all(np.abs(Cv[s] - Cvr) <= 1e-8 * np.maximum(np.abs(Cvr), 1e-6))
+
+
+def test_block_lanczos_matches_lanczos_and_eigsh(qk, orc, lat):
+    import scipy.sparse.linalg as sla
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(16, sz=0), qk.SpinHalfSite(), lat.heisenberg_chain(16))
+    e_ref = qk.lanczos_ground_state(H, maxiter=200, tol=1e-12).energy
+    for b in (1, 3, 4):
+        r = qk.block_lanczos(H, block=b, nev=1, maxiter=200, tol=1e-12, return_vectors=True)
+        v = r.eigenvectors[0]
+        res = np.linalg.norm(H.apply(v) - r.eigenvalues[0] * v)
+        print(f"[block lanczos] N=16 b={b}: E0={r.eigenvalues[0]:.14f} steps={r.iterations} "
+              f"conv={r.converged} |Hv-Ev|={res:.1e}", flush=True)
+        assert r.converged and abs(r.eigenvalues[0] - e_ref) < 1e-10 and res < 1e-6
+    ops = lat.tj_model(lat.square_bonds(3, 4), 1.0, 0.4)
+    sec = qk.Sector().set_hubbard_particles(5, 5)
+    Ht = qk.MatrixFreeHamiltonian(qk.TJBasis(12, sector=sec), qk.TJSite(), ops)
+    A = orc.Ham(orc.Basis("tj", 12, {"use_nup": True, "nup": 5, "use_ndn": True, "ndn": 5}),
+                "tj", ops).to_scipy()
+    w = np.sort(sla.eigsh(A, k=6, which="SA", tol=1e-13)[0])[:4]
+    r = qk.block_lanczos(Ht, block=4, nev=4, maxiter=150, tol=1e-12)
+    print(f"[block lanczos] t-J 3x4: {r.eigenvalues} vs eigsh {w} steps={r.iterations}", flush=True)
+    assert r.converged
+    np.testing.assert_allclose(r.eigenvalues, w, rtol=0, atol=1e-9)
