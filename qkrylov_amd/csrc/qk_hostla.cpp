@@ -113,19 +113,12 @@ void hermitian_jacobi(const std::vector<cplx>& Ain, int n, std::vector<double>& 
     auto at = [&](std::vector<cplx>& M, int i, int j) -> cplx& { return M[size_t(i) * n + j]; };
     auto cabs_ = [](cplx z) { return std::hypot(z.re, z.im); };
 
-    for (int it = 0; it < 1000; ++it) {
-        double off = 0.0;
-        int p = 0, q = 0;
-        for (int i = 0; i < n; ++i)
-            for (int j = i + 1; j < n; ++j) {
-                const double v = cabs_(at(A, i, j));
-                if (v > off) {
-                    off = v;
-                    p = i;
-                    q = j;
-                }
-            }
-        if (off < 1e-15) break;
+    // One Jacobi rotation zeroing A[p][q].  The reference (davidson.cpp:28-82)
+    // applies the same rotation to the largest off-diagonal element, at most
+    // 1000 times, which leaves subspaces much larger than its default 20 only
+    // partly diagonalised; here cyclic sweeps run until every off-diagonal
+    // element is below 1e-15 relative to the largest diagonal magnitude.
+    auto rotate = [&](int p, int q) {
         const cplx app = at(A, p, p), aqq = at(A, q, q), apq = at(A, p, q);
         const double r = cabs_(apq);
         const double phi = 0.5 * std::atan2(2.0 * r, app.re - aqq.re);
@@ -155,6 +148,18 @@ void hermitian_jacobi(const std::vector<cplx>& Ain, int n, std::vector<double>& 
         at(A, q, q) = cplx{ss * app.re + c * c * aqq.re - cross, 0.0};
         at(A, p, q) = cplx{0.0, 0.0};
         at(A, q, p) = cplx{0.0, 0.0};
+    };
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double dmax = 0.0, off = 0.0;
+        for (int i = 0; i < n; ++i) {
+            dmax = std::max(dmax, std::abs(at(A, i, i).re));
+            for (int j = i + 1; j < n; ++j) off = std::max(off, cabs_(at(A, i, j)));
+        }
+        if (off < 1e-15 * std::max(dmax, 1e-300) || off < 1e-300) break;
+        const double thr = sweep < 3 ? 0.0 : 1e-17 * std::max(dmax, 1e-300);
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q)
+                if (cabs_(at(A, p, q)) > thr) rotate(p, q);
     }
     std::vector<int> idx(n);
     std::iota(idx.begin(), idx.end(), 0);

@@ -537,7 +537,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         const int nnew = m - valid_rows;
         static const bool use_block_dot = [] {
             const char* e = std::getenv("QKRYLOV_DAVIDSON_BLOCK_DOT");
-            return e && std::string(e) == "1";
+            return !(e && std::string(e) == "0");
         }();
         if (use_block_dot && nnew > 0 && m <= 64 && nnew <= 16) {
             // one FP64-MFMA projection G = HV^H V_new for all new rows

@@ -159,7 +159,8 @@ def test_block_lanczos_matches_lanczos_and_eigsh(qk, orc, lat):
         res = np.linalg.norm(H.apply(v) - r.eigenvalues[0] * v)
         print(f"[block lanczos] N=16 b={b}: E0={r.eigenvalues[0]:.14f} steps={r.iterations} "
               f"conv={r.converged} |Hv-Ev|={res:.1e}", flush=True)
-        assert r.converged and abs(r.eigenvalues[0] - e_ref) < 1e-10 and res < 1e-6
+        # energy to 1e-10; the Ritz vector's residual is ~ sqrt of the energy change
+        assert r.converged and abs(r.eigenvalues[0] - e_ref) < 1e-10 and res < 1e-5
     ops = lat.tj_model(lat.square_bonds(3, 4), 1.0, 0.4)
     sec = qk.Sector().set_hubbard_particles(5, 5)
     Ht = qk.MatrixFreeHamiltonian(qk.TJBasis(12, sector=sec), qk.TJSite(), ops)
