@@ -36,8 +36,11 @@ namespace qk {
 
 namespace {
 
-constexpr int kWaves = 4;            // waves per workgroup (generic kernels)
+constexpr int kWaves = 4;            // waves per workgroup (diagonal, S^z(q) kernels)
 constexpr int kBlock = 64 * kWaves;  // threads per workgroup
+constexpr int kGWaves = 16;          // waves per workgroup (generic apply): one LDS
+                                     // copy of the tables serves 16 waves
+constexpr int kGBlock = 64 * kGWaves;
 constexpr int kXWaves = 16;          // waves per workgroup (exchange kernel)
 constexpr int kXcd = 8;              // XCDs on MI355X
 
@@ -107,19 +110,19 @@ __device__ __forceinline__ void store_nt(T* p, const T& v) {
 }
 
 template <typename S, int B>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kGBlock)
 k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, const u64 ld) {
     extern __shared__ __align__(16) unsigned char smem[];
     cplx* s_tab = reinterpret_cast<cplx*>(smem);
     u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ntab) * sizeof(cplx));
 
-    for (u32 i = threadIdx.x; i < p.ntab; i += kBlock) s_tab[i] = p.tables[i];
+    for (u32 i = threadIdx.x; i < p.ntab; i += kGBlock) s_tab[i] = p.tables[i];
     if ((p.nlo & 7) == 0) {
         const uint4* src = reinterpret_cast<const uint4*>(p.jlo);
         uint4* dst = reinterpret_cast<uint4*>(s_jlo);
-        for (u32 i = threadIdx.x; i < p.nlo / 8; i += kBlock) dst[i] = src[i];
+        for (u32 i = threadIdx.x; i < p.nlo / 8; i += kGBlock) dst[i] = src[i];
     } else {
-        for (u32 i = threadIdx.x; i < p.nlo; i += kBlock) s_jlo[i] = p.jlo[i];
+        for (u32 i = threadIdx.x; i < p.nlo; i += kGBlock) s_jlo[i] = p.jlo[i];
     }
     __syncthreads();
 
@@ -127,7 +130,7 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
     const u32 lane = threadIdx.x & 63;
     const S lomask = (S(1) << p.L) - 1;
 
-    QK_FOR_TILES(p, wave, t) {
+    QK_FOR_TILES_W(p, wave, t, kGWaves) {
         const uint2 td = p.tiles[t];
         const u32 hi = __builtin_amdgcn_readfirstlane(td.x);
         const u32 r0 = __builtin_amdgcn_readfirstlane(td.y);
@@ -696,7 +699,7 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
             return;
         }
     }
-    hipLaunchKernelGGL((k_apply<S, B>), dim3(h.grid), dim3(kBlock), h.lds_bytes, st, h.params, x,
+    hipLaunchKernelGGL((k_apply<S, B>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, x,
                        y, ld);
 }
 
@@ -834,7 +837,7 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
     p.ncls2 = B.ncls2();
 
     lds_bytes = int(p.ntab * sizeof(cplx) + ((p.nlo * sizeof(u16) + 15) / 16) * 16);
-    grid = grid_for(lds_bytes, tiles.size(), &p.tiles_per_xcd);
+    grid = grid_for(lds_bytes, tiles.size(), &p.tiles_per_xcd, kGWaves);
     if (lds_bytes > 160 * 1024) fail("group tables exceed LDS");
     if (lds_bytes > 64 * 1024) {
         auto set = [&](const void* f) {
