@@ -34,6 +34,19 @@ def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data
 
 
+def _release(obj, destroy: str) -> None:
+    """Free obj's library handle; at interpreter exit the module globals may
+    already be gone, and then the process releases everything anyway."""
+    h = getattr(obj, "_h", None)
+    if not h:
+        return
+    obj._h = None
+    try:
+        getattr(load(), destroy)(h)
+    except Exception:  # noqa: BLE001
+        pass
+
+
 def device_info() -> str:
     buf = ct.create_string_buffer(256)
     check(load().qkrylov_device_info(buf, 256), "device_info")
@@ -60,9 +73,10 @@ class Sector:
             raise RuntimeError(last_error())
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            load().qkrylov_sector_destroy(self._h)
-            self._h = None
+        try:
+            _release(self, "qkrylov_sector_destroy")
+        except Exception:  # noqa: BLE001  (interpreter exit)
+            pass
 
     def set_sz(self, sz2: int):
         check(load().qkrylov_sector_set_sz(self._h, int(sz2)), "set_sz")
@@ -116,9 +130,10 @@ class Basis:
             raise ValueError(last_error())
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            load().qkrylov_basis_destroy(self._h)
-            self._h = None
+        try:
+            _release(self, "qkrylov_basis_destroy")
+        except Exception:  # noqa: BLE001  (interpreter exit)
+            pass
 
     @property
     def size(self) -> int:
@@ -218,9 +233,10 @@ class Site:
         self._h = getattr(load(), self._create)()
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            load().qkrylov_site_destroy(self._h)
-            self._h = None
+        try:
+            _release(self, "qkrylov_site_destroy")
+        except Exception:  # noqa: BLE001  (interpreter exit)
+            pass
 
 
 class SpinHalfSite(Site):
@@ -448,9 +464,10 @@ class MatrixFreeHamiltonian:
             raise RuntimeError(f"Hamiltonian create failed: {last_error()}")
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            load().qkrylov_hamiltonian_destroy(self._h)
-            self._h = None
+        try:
+            _release(self, "qkrylov_hamiltonian_destroy")
+        except Exception:  # noqa: BLE001  (interpreter exit)
+            pass
 
     @property
     def dimension(self) -> int:
@@ -619,9 +636,10 @@ class MLPPreconditioner:
         self.last_train = None
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            load().qkrylov_mlp_destroy(self._h)
-            self._h = None
+        try:
+            _release(self, "qkrylov_mlp_destroy")
+        except Exception:  # noqa: BLE001  (interpreter exit)
+            pass
 
     @staticmethod
     def num_params() -> int:

@@ -1165,9 +1165,14 @@ bool MultiApply::build(const std::vector<Group>& groups, hipStream_t st) {
     mp_.jhis = jhis_.get<const u64*>();
     mp_.ntiles = u32(mt.size());
     grid_ = grid_for(h0.xlds_bytes, mt.size(), &mp_.tiles_per_xcd, kXWaves);
-    if (h0.xlds_bytes > 64 * 1024)
+    vb_ = 8;   // vectors per pass over a tile
+    if (const char* e = std::getenv("QKRYLOV_MULTI_B")) vb_ = std::atoi(e) == 4 ? 4 : 8;
+    if (h0.xlds_bytes > 64 * 1024) {
         QK_HIP(hipFuncSetAttribute((const void*)k_apply_xchg_multi<4>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, h0.xlds_bytes));
+        QK_HIP(hipFuncSetAttribute((const void*)k_apply_xchg_multi<8>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, h0.xlds_bytes));
+    }
     h0_ = &h0;
     ok_ = true;
     return true;
@@ -1175,8 +1180,12 @@ bool MultiApply::build(const std::vector<Group>& groups, hipStream_t st) {
 
 void MultiApply::launch(const cplx* x, cplx* y, hipStream_t st) const {
     if (!ok_) fail("MultiApply::launch without a successful build");
-    hipLaunchKernelGGL((k_apply_xchg_multi<4>), dim3(grid_), dim3(64 * kXWaves), h0_->xlds_bytes, st,
-                       h0_->xparams, mp_, x, y);
+    if (vb_ == 4)
+        hipLaunchKernelGGL((k_apply_xchg_multi<4>), dim3(grid_), dim3(64 * kXWaves), h0_->xlds_bytes,
+                           st, h0_->xparams, mp_, x, y);
+    else
+        hipLaunchKernelGGL((k_apply_xchg_multi<8>), dim3(grid_), dim3(64 * kXWaves), h0_->xlds_bytes,
+                           st, h0_->xparams, mp_, x, y);
     QK_HIP(hipGetLastError());
 }
 

@@ -197,6 +197,7 @@ private:
     MParams mp_{};
     DevBuf tiles_, jhis_;
     int grid_ = 0;
+    int vb_ = 8;
 };
 
 }  // namespace qk

@@ -171,3 +171,22 @@ def test_block_lanczos_matches_lanczos_and_eigsh(qk, orc, lat):
     print(f"[block lanczos] t-J 3x4: {r.eigenvalues} vs eigsh {w} steps={r.iterations}", flush=True)
     assert r.converged
     np.testing.assert_allclose(r.eigenvalues, w, rtol=0, atol=1e-9)
+
+
+def test_lanczos_at_scale_j1j2(qk, lat):
+    """J1-J2 6x4 (d = 2,704,156): 300 Lanczos steps with full
+    reorthogonalisation (tol 0) against block Lanczos, an independent
+    recurrence on the same H; both Ritz vectors must be eigenvectors."""
+    H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(24, sz=0), qk.SpinHalfSite(),
+                                 lat.j1j2_square(6, 4, 1.0, 0.5))
+    r = qk.lanczos_ground_state(H, maxiter=300, tol=0.0)
+    assert r.iterations == 300
+    v = r.eigenvector
+    res = np.linalg.norm(H.apply(v) - r.energy * v) / np.linalg.norm(v)
+    b = qk.block_lanczos(H, block=4, nev=1, maxiter=120, tol=1e-12, return_vectors=True)
+    vb = b.eigenvectors[0]
+    resb = np.linalg.norm(H.apply(vb) - b.eigenvalues[0] * vb)
+    print(f"[lanczos 6x4] E0={r.energy:.13f} res={res:.1e}; block E0={b.eigenvalues[0]:.13f} "
+          f"res={resb:.1e} steps={b.iterations}", flush=True)
+    assert res < 1e-9 and resb < 1e-5
+    assert abs(r.energy - b.eigenvalues[0]) < 1e-10
