@@ -265,17 +265,20 @@ int qkrylov_davidson_lowest_f64(qkrylov_hamiltonian_h h, int n_eig, int max_subs
 
 /* ---- Learned Davidson preconditioner.  No counterpart in the reference,
  * whose Davidson corrects with the Jacobi diagonal (davidson.cpp:166-177).
- * A 3-layer MLP (32 -> 128 -> 128 -> 1, ReLU, BF16 weights, FP32
- * accumulation on MFMA) maps per-state features (diagonal, theta,
- * connectivity, local occupations) to o_i, and the correction becomes
- * t_i = (1/(H_ii - theta) + o_i) r_i; an untrained network (last layer 0)
- * is the Jacobi method. ---- */
+ * With J_i = 1/(H_ii - theta) (clamped as the reference does), z = J r and
+ * w = J (H z - diag(H) z), the correction is t_i = s1_i z_i + s2_i w_i with
+ * s1 = 1 + 0.5 tanh(a), s2 = -0.5 + 0.25 tanh(b), where (a_i, b_i) is the
+ * output of a 3-layer MLP (32 -> 128 -> 128 -> 2, ReLU, BF16 weights, FP32
+ * accumulation on MFMA) on per-state features (diagonal, theta,
+ * connectivity, local occupations).  An untrained network (last layer 0)
+ * gives the damped first-order Neumann correction t = z - w/2.  Each
+ * correction costs one extra apply of H. ---- */
 typedef struct qkrylov_mlp_t* qkrylov_mlp_h;
 
 typedef struct {
     int pairs;             /* (residual, accurate correction) pairs generated */
     int epochs;
-    double loss_first;     /* mean 1 - cos^2 before training */
+    double loss_first;     /* mean loss (qkrylov_mlp_pair_loss_f64) before training */
     double loss_last;      /* after training */
     double solve_rel_res;  /* worst relative residual of the accurate corrections */
 } qkrylov_mlp_train_info_t;
@@ -283,19 +286,36 @@ typedef struct {
 /* Random hidden layers from std::mt19937(seed), output layer zero. */
 qkrylov_mlp_h qkrylov_mlp_create(unsigned seed);
 void qkrylov_mlp_destroy(qkrylov_mlp_h m);
-/* Parameter count and FP32 layout: W1[32][128], b1[128], W2[128][128],
- * b2[128], w3[128], b3 (row index = input). */
+/* Parameter count (20994) and FP32 layout: W1[32][128], b1[128],
+ * W2[128][128], b2[128], W3[128][2], b3[2] (row index = input). */
 int qkrylov_mlp_num_params(void);
 int qkrylov_mlp_get_params(qkrylov_mlp_h m, float* params_out);
 int qkrylov_mlp_set_params(qkrylov_mlp_h m, const float* params);
-/* o = net(features) for n rows of 32 features (rounded to BF16 on entry);
- * with grad_out and grad_params non-NULL also grad_params = d(sum_i
- * grad_out[i] o[i]) / d(params).  Host buffers. */
+/* o = net(features) (n x 2: a, b per row) for n rows of 32 features
+ * (rounded to BF16 on entry); with grad_out (n x 2) and grad_params
+ * non-NULL also grad_params = d(sum_ik grad_out[i][k] o[i][k]) / d(params).
+ * Host buffers. */
 int qkrylov_mlp_eval(qkrylov_mlp_h m, const float* features, uint64_t n, const float* grad_out,
                      float* o_out, float* grad_params);
 /* Features of every basis state of h for theta: dim x 32 floats (the BF16
  * values the network reads). */
 int qkrylov_mlp_features(qkrylov_hamiltonian_h h, double theta, float* features_out);
+/* The learned correction t of a host residual r (complex, dim) at theta, as
+ * Davidson applies it; z_out / w_out (may be NULL) receive the two terms. */
+int qkrylov_mlp_correction_f64(qkrylov_mlp_h m, qkrylov_hamiltonian_h h, const double* r_complex,
+                               double theta, double* t_complex_out, double* z_complex_out,
+                               double* w_complex_out);
+/* Training loss of npairs pairs on h's basis (host buffers, pair-major, dim
+ * states each): o (npairs*dim x 2 floats, network output), z, w, b, u
+ * (complex), theta (npairs doubles).  With K = P_u (H - theta) P_u,
+ * P_u = 1 - u u^H (u unit) and t = P_u (s1 z + s2 w):
+ * loss = sum over pairs of 1 - |<b, K t>|^2 / |K t|^2, b = K t* / |K t*|
+ * for the accurate correction t*; grad_o (npairs*dim x 2, may be NULL) =
+ * d(loss / npairs) / d(o). */
+int qkrylov_mlp_pair_loss_f64(qkrylov_hamiltonian_h h, int npairs, const float* o,
+                              const double* z_complex, const double* w_complex,
+                              const double* b_complex, const double* u_complex, const double* theta,
+                              float* grad_o, double* loss_out);
 /* Accurate Jacobi-Davidson correction: MINRES on (I-uu^H)(H-theta)(I-uu^H) t
  * = -r, t orthogonal to u (host complex vectors of dim). */
 int qkrylov_jd_correction_f64(qkrylov_hamiltonian_h h, const double* r_complex,

@@ -5,28 +5,35 @@ preconditioner has no counterpart in the reference, whose Davidson corrects
 with the Jacobi diagonal (src/solvers/davidson.cpp:166-177): this file
 restates the library's own definition in numpy, with the BF16 rounding at
 the same points as the GPU kernels, so that the kernels' forward pass,
-parameter gradients, features and pair loss can be checked.
+parameter gradients, features, correction terms and pair loss can be checked.
 
 Pinned by: float32->bfloat16 rounding against torch's conversion; the exact
 (float64, unrounded) backward against central finite differences of the
-exact forward; the pair-loss gradient against finite differences; the
-network with a zero output layer equals the Jacobi method.
+exact forward; the pair-loss gradient against finite differences and the
+loss at its two ends (0 for t along t*, 1 for K t orthogonal to K t*); the
+correction terms against a dense solve (z - w is the first-order Neumann
+expansion of (H - theta)^-1 r, so its error falls as the square of the
+off-diagonal scale); the network with a zero output layer giving z - w/2.
 
-Network: x (32) -> relu(x W1 + b1) (128) -> relu(h1 W2 + b2) (128) -> h2 w3 + b3.
+Network: x (32) -> relu(x W1 + b1) (128) -> relu(h1 W2 + b2) (128) -> h2 W3 + b3 (2).
 Parameter block (float32): W1[32][128], b1[128], W2[128][128], b2[128],
-w3[128], b3, row index = input.
+W3[128][2], b3[2], row index = input.
+
+Correction: z = J r, w = J (H z - diag z), J = 1/(H_ii - theta) clamped as the
+reference clamps it; t = s1 z + s2 w, s1 = 1 + tanh(a)/2, s2 = -1/2 + tanh(b)/4,
+(a, b) = network output per state.
 """
 from __future__ import annotations
 
 import numpy as np
 
-IN, HID = 32, 128
+IN, HID, OUT = 32, 128, 2
 W1, B1 = 0, IN * HID
 W2 = B1 + HID
 B2 = W2 + HID * HID
 W3 = B2 + HID
-B3 = W3 + HID
-NPARAMS = B3 + 1
+B3 = W3 + HID * OUT
+NPARAMS = B3 + OUT
 
 
 def bf16(x) -> np.ndarray:
@@ -43,7 +50,7 @@ def bf16(x) -> np.ndarray:
 def unpack(p: np.ndarray):
     p = np.asarray(p)
     return (p[W1:B1].reshape(IN, HID), p[B1:W2], p[W2:B2].reshape(HID, HID), p[B2:W3],
-            p[W3:B3], p[B3])
+            p[W3:B3].reshape(HID, OUT), p[B3:NPARAMS])
 
 
 def init_params(seed: int) -> np.ndarray:
@@ -55,20 +62,20 @@ def init_params(seed: int) -> np.ndarray:
     p[B1:W2] = rng.uniform(-0.1, 0.1, HID)
     p[W2:B2] = rng.uniform(-1, 1, HID * HID) * np.sqrt(6.0 / HID)
     p[B2:W3] = rng.uniform(-0.1, 0.1, HID)
-    p[W3:B3] = rng.uniform(-0.2, 0.2, HID)
-    p[B3] = 0.05
+    p[W3:B3] = rng.uniform(-0.2, 0.2, HID * OUT)
+    p[B3:NPARAMS] = (0.05, -0.03)
     return p
 
 
 def forward(p: np.ndarray, X: np.ndarray, exact: bool = False):
-    """o and the activations.  exact=False rounds inputs, weights and both
-    activations to BF16 (as the kernels); exact=True is plain float64."""
+    """o (n x 2) and the activations.  exact=False rounds inputs, weights and
+    both activations to BF16 (as the kernels); exact=True is plain float64."""
     w1, b1, w2, b2, w3, b3 = unpack(p)
     q = (lambda a: np.asarray(a, dtype=np.float64)) if exact else (lambda a: bf16(a).astype(np.float64))
     # biases stay FP32 in the kernels
     fb = (lambda a: np.asarray(a, dtype=np.float64)) if exact else \
         (lambda a: np.asarray(a, dtype=np.float32).astype(np.float64))
-    b1, b2, b3 = fb(b1), fb(b2), float(fb(b3))
+    b1, b2, b3 = fb(b1), fb(b2), fb(b3)
     X = q(X)
     z1 = X @ q(w1) + b1
     h1 = np.maximum(z1, 0.0)
@@ -81,26 +88,26 @@ def forward(p: np.ndarray, X: np.ndarray, exact: bool = False):
 
 
 def backward(p: np.ndarray, X: np.ndarray, go: np.ndarray, exact: bool = False) -> np.ndarray:
-    """d(sum go*o)/d(params) with the kernels' BF16 points (gz2 and gz1 are
-    rounded because they feed MFMA) or exactly in float64."""
+    """d(sum go*o)/d(params), go (n x 2), with the kernels' BF16 points (gz2
+    and gz1 are rounded because they feed MFMA) or exactly in float64."""
     w1, b1, w2, b2, w3, b3 = unpack(p)
     _, (Xq, h1, h2) = forward(p, X, exact)
-    go = np.asarray(go, dtype=np.float64)
+    go = np.asarray(go, dtype=np.float64).reshape(-1, OUT)
     if exact:
         w2q, w3q = w2.astype(np.float64), w3.astype(np.float64)
-        gz2 = go[:, None] * w3q[None, :] * (h2 > 0)
+        gz2 = (go @ w3q.T) * (h2 > 0)
         gz1 = (gz2 @ w2q.T) * (h1 > 0)
     else:
         w2q, w3q = bf16(w2).astype(np.float64), bf16(w3).astype(np.float64)
-        gz2 = bf16((go[:, None] * w3q[None, :] * (h2 > 0)).astype(np.float32)).astype(np.float64)
+        gz2 = bf16(((go @ w3q.T) * (h2 > 0)).astype(np.float32)).astype(np.float64)
         gz1 = bf16(((gz2 @ w2q.T) * (h1 > 0)).astype(np.float32)).astype(np.float64)
     g = np.zeros(NPARAMS, dtype=np.float64)
     g[W1:B1] = (Xq.T @ gz1).ravel()
     g[B1:W2] = gz1.sum(0)
     g[W2:B2] = (h1.T @ gz2).ravel()
     g[B2:W3] = gz2.sum(0)
-    g[W3:B3] = h2.T @ go
-    g[B3] = go.sum()
+    g[W3:B3] = (h2.T @ go).ravel()
+    g[B3:NPARAMS] = go.sum(0)
     return g
 
 
@@ -137,18 +144,42 @@ def occupations(states: np.ndarray, nsites: int, two_bit: bool) -> np.ndarray:
     return (occ / nsites).astype(np.float32)
 
 
-def pair_loss(diag, theta, o, r, ts, u):
-    """L = 1 - |<ts, P_u t>|^2 / (|P_u t|^2 |ts|^2), t = (1/den + o) r, and
-    dL/do.  ts must be orthogonal to u."""
-    w = 1.0 / jacobi_den(diag, theta) + np.asarray(o, dtype=np.float64)
-    t = w * r
-    A = np.vdot(ts, t)
-    c = np.vdot(u, t)
-    S = np.sum(w * w * np.abs(r) ** 2)
-    B = S - abs(c) ** 2
-    C = np.vdot(ts, ts).real
-    L = 1.0 - abs(A) ** 2 / (B * C)
-    reA = (np.conj(A) * np.conj(ts) * r).real
-    reC = (np.conj(c) * np.conj(u) * r).real
-    g = -2.0 * reA / (B * C) + 2.0 * abs(A) ** 2 / (B * B * C) * (w * np.abs(r) ** 2 - reC)
-    return L, g
+def neumann_terms(diag, theta, r, apply_h):
+    """z = J r and w = J (H z - diag z); apply_h(x) = H x."""
+    J = 1.0 / jacobi_den(diag, theta)
+    z = J * r
+    w = J * (apply_h(z) - np.asarray(diag, dtype=np.float64) * z)
+    return z, w
+
+
+def coefficients(o):
+    """(s1, s2) per state from the network output o (n x 2)."""
+    o = np.asarray(o, dtype=np.float64).reshape(-1, OUT)
+    return 1.0 + 0.5 * np.tanh(o[:, 0]), -0.5 + 0.25 * np.tanh(o[:, 1])
+
+
+def combine(o, z, w):
+    s1, s2 = coefficients(o)
+    return s1 * z + s2 * w
+
+
+def pair_loss(o, z, w, b, u, theta, apply_h):
+    """Training loss of one pair and dL/do (n x 2).  With P = 1 - u u^H (|u| = 1),
+    K = P (H - theta) P and t = P (s1 z + s2 w):
+        L = 1 - |<b, K t>|^2 / (|K t|^2 |b|^2),
+    b = K t* for the accurate correction t*.  apply_h(x) = H x."""
+    o = np.asarray(o, dtype=np.float64).reshape(-1, OUT)
+    ta, tb = np.tanh(o[:, 0]), np.tanh(o[:, 1])
+    P = lambda x: x - u * np.vdot(u, x)
+    K = lambda x: P(apply_h(x) - theta * x)
+    t = P(combine(o, z, w))
+    q = K(t)
+    a = np.vdot(b, q)
+    Q = np.vdot(q, q).real
+    Bn = np.vdot(b, b).real
+    L = 1.0 - abs(a) ** 2 / (Q * Bn)
+    g = -2.0 * a * b / (Q * Bn) + 2.0 * abs(a) ** 2 * q / (Q * Q * Bn)
+    h = K(P(g))
+    grad = np.stack([(np.conj(h) * z).real * 0.5 * (1.0 - ta * ta),
+                     (np.conj(h) * w).real * 0.25 * (1.0 - tb * tb)], axis=1)
+    return L, grad

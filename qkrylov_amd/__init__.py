@@ -622,11 +622,13 @@ def davidson_lowest(H: MatrixFreeHamiltonian, n_eig: int = 1, max_subspace: int 
 
 
 class MLPPreconditioner:
-    """Learned correction for Davidson: t_i = (1/(H_ii - theta) + net(f_i)) r_i.
+    """Learned correction for Davidson: t_i = s1_i z_i + s2_i w_i.
 
-    The network (32 -> 128 -> 128 -> 1, ReLU, BF16 weights, FP32 accumulation on
-    MFMA) starts with a zero output layer, i.e. as the Jacobi method.  There is no
-    counterpart in the reference, whose Davidson uses the Jacobi diagonal
+    z = J r and w = J (H z - diag(H) z) with J_i = 1/(H_ii - theta); the network
+    (32 -> 128 -> 128 -> 2, ReLU, BF16 weights, FP32 accumulation on MFMA) gives
+    (a_i, b_i) and s1 = 1 + 0.5 tanh(a), s2 = -0.5 + 0.25 tanh(b).  It starts with
+    a zero output layer, i.e. as the damped Neumann correction z - w/2.  There is
+    no counterpart in the reference, whose Davidson uses the Jacobi diagonal
     (src/solvers/davidson.cpp:166-177)."""
 
     def __init__(self, seed: int = 7):
@@ -657,15 +659,17 @@ class MLPPreconditioner:
         check(load().qkrylov_mlp_set_params(self._h, _ptr(p)), "mlp params")
 
     def eval(self, features: np.ndarray, grad_out: Optional[np.ndarray] = None):
-        """Network output for rows of 32 features; with grad_out also the
-        parameter gradient of sum(grad_out * o)."""
+        """Network output (n, 2) for rows of 32 features; with grad_out (n, 2)
+        also the parameter gradient of sum(grad_out * o)."""
         X = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, 32)
         n = X.shape[0]
-        o = np.empty(max(n, 1), dtype=np.float32)
+        o = np.empty((max(n, 1), 2), dtype=np.float32)
         g = None
         go = None
         if grad_out is not None:
             go = np.ascontiguousarray(grad_out, dtype=np.float32).reshape(-1)
+            if go.size != 2 * n:
+                raise ValueError("grad_out must have shape (n, 2)")
             g = np.empty(self.num_params(), dtype=np.float32)
         check(load().qkrylov_mlp_eval(self._h, _ptr(X), n, None if go is None else _ptr(go),
                                       _ptr(o), None if g is None else _ptr(g)), "mlp eval")
@@ -693,6 +697,36 @@ def mlp_features(H: MatrixFreeHamiltonian, theta: float) -> np.ndarray:
     out = np.empty((max(H.dimension, 1), 32), dtype=np.float32)
     check(load().qkrylov_mlp_features(H._h, float(theta), _ptr(out)), "mlp features")
     return out[:H.dimension]
+
+
+def mlp_correction(net: MLPPreconditioner, H: MatrixFreeHamiltonian, r: np.ndarray, theta: float):
+    """The learned correction (t, z, w) of residual r at Ritz value theta."""
+    r = np.ascontiguousarray(r, dtype=np.complex128)
+    if r.size != H.dimension:
+        raise ValueError("residual length must equal the dimension")
+    t, z, w = np.empty_like(r), np.empty_like(r), np.empty_like(r)
+    check(load().qkrylov_mlp_correction_f64(net._h, H._h, _ptr(r), float(theta), _ptr(t), _ptr(z),
+                                            _ptr(w)), "mlp correction")
+    return t, z, w
+
+
+def mlp_pair_loss(H: MatrixFreeHamiltonian, o: np.ndarray, z: np.ndarray, w: np.ndarray,
+                  b: np.ndarray, u: np.ndarray, theta, grad: bool = True):
+    """Summed training loss of pairs on H's basis (rows of z, w, b, u; o is
+    (pairs, n, 2); theta one value per pair) and, with grad, its gradient
+    d(loss / pairs)/d(o)."""
+    n = H.dimension
+    z = np.ascontiguousarray(np.reshape(z, (-1, n)), dtype=np.complex128)
+    P = z.shape[0]
+    w, b, u = (np.ascontiguousarray(np.reshape(a, (P, n)), dtype=np.complex128) for a in (w, b, u))
+    o = np.ascontiguousarray(np.reshape(o, (P, n, 2)), dtype=np.float32)
+    th = np.ascontiguousarray(np.reshape(theta, (P,)), dtype=np.float64)
+    g = np.empty_like(o) if grad else None
+    L = ct.c_double()
+    check(load().qkrylov_mlp_pair_loss_f64(H._h, P, _ptr(o), _ptr(z), _ptr(w), _ptr(b), _ptr(u),
+                                           _ptr(th), None if g is None else _ptr(g), ct.byref(L)),
+          "mlp pair loss")
+    return (L.value, g) if grad else L.value
 
 
 def jd_correction(H: MatrixFreeHamiltonian, r: np.ndarray, u: np.ndarray, theta: float,

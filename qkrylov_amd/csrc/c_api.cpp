@@ -576,6 +576,54 @@ int qkrylov_mlp_features(qkrylov_hamiltonian_h h, double theta, float* out) {
         for (size_t i = 0; i < xb.size(); ++i) out[i] = bf16_to_f32(xb[i]);
     });
 }
+int qkrylov_mlp_correction_f64(qkrylov_mlp_h m, qkrylov_hamiltonian_h h, const double* r,
+                               double theta, double* t_out, double* z_out, double* w_out) {
+    if (!m || !m->ptr || !hok(h) || !r || !t_out) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const Hamiltonian& H = *h->ptr;
+        const u64 d = H.dim();
+        if (!d) return;
+        MlpPreconditioner pre(*m->ptr, H);
+        DevBuf b(sizeof(cplx) * d * 2);
+        cplx* rd = b.get<cplx>();
+        cplx* td = rd + d;
+        QK_HIP(hipMemcpyAsync(rd, r, sizeof(cplx) * d, hipMemcpyHostToDevice, H.stream));
+        pre.apply(H, rd, nullptr, theta, nullptr, td, H.stream);
+        QK_HIP(hipMemcpyAsync(t_out, td, sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
+        if (z_out) QK_HIP(hipMemcpyAsync(z_out, pre.z(), sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
+        if (w_out) QK_HIP(hipMemcpyAsync(w_out, pre.w(), sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
+        QK_HIP(hipStreamSynchronize(H.stream));
+    });
+}
+int qkrylov_mlp_pair_loss_f64(qkrylov_hamiltonian_h h, int npairs, const float* o, const double* z,
+                              const double* w, const double* b, const double* u, const double* theta,
+                              float* grad_o, double* loss) {
+    if (!hok(h) || !o || !z || !w || !b || !u || !theta || !loss || npairs <= 0)
+        return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const Hamiltonian& H = *h->ptr;
+        hipStream_t st = H.stream;
+        const u64 n = H.dim();
+        const u64 tot = u64(npairs) * n;
+        if (!n) {
+            *loss = 0.0;
+            return;
+        }
+        DevBuf vb(sizeof(cplx) * tot * 4), ob(sizeof(float) * tot * kMlpOut),
+            gb(sizeof(float) * tot * kMlpOut);
+        cplx* zd = vb.get<cplx>();
+        const double* src[4] = {z, w, b, u};
+        for (int q = 0; q < 4; ++q)
+            QK_HIP(hipMemcpyAsync(zd + u64(q) * tot, src[q], sizeof(cplx) * tot, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(ob.get(), o, sizeof(float) * tot * kMlpOut, hipMemcpyHostToDevice, st));
+        MlpLossWork wk;
+        *loss = mlp_pair_loss(H, zd, zd + tot, zd + 2 * tot, zd + 3 * tot, theta, npairs, ob.get<float>(),
+                              grad_o ? gb.get<float>() : nullptr, wk, st);
+        if (grad_o)
+            QK_HIP(hipMemcpyAsync(grad_o, gb.get(), sizeof(float) * tot * kMlpOut, hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+    });
+}
 int qkrylov_jd_correction_f64(qkrylov_hamiltonian_h h, const double* r, const double* u, double theta,
                               int max_iter, double rtol, double* t_out, double* rel_res) {
     if (!hok(h) || !r || !u || !t_out || max_iter <= 0) return QKRYLOV_ERROR_INVALID_ARG;

@@ -170,35 +170,42 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
     out.pairs = P;
     if (P == 0) return out;
 
-    // 2. accurate corrections, normalised; everything packed pair-major
-    DevBuf R(sizeof(cplx) * n * P), T(sizeof(cplx) * n * P), U(sizeof(cplx) * n * P),
-        th(sizeof(double) * P);
+    // 2. accurate corrections t*, stored as the unit vector K t* (the loss
+    // target, K = P_u (H - theta) P_u), and the two correction terms
+    // z = J r, w = J (H z - diag z) of every pair; everything pair-major
+    MlpStates states;
+    states.build(Hs, st);
+    DevBuf Z(sizeof(cplx) * n * P), W(sizeof(cplx) * n * P), T(sizeof(cplx) * n * P),
+        U(sizeof(cplx) * n * P), hz(sizeof(cplx) * n), ts(sizeof(cplx) * n);
     Blas b{st, n};
     double worst = 0.0;
     for (int p = 0; p < P; ++p) {
         cplx* tp = T.get<cplx>() + u64(p) * n;
-        const double rr = jd_correction(Hs, rec.r[p].get<cplx>(), rec.u[p].get<cplx>(),
-                                        rec.theta[p], tp, cfg.solve_iters, cfg.solve_rtol, st);
+        cplx* zp = Z.get<cplx>() + u64(p) * n;
+        const cplx* up = rec.u[p].get<cplx>();
+        const double rr = jd_correction(Hs, rec.r[p].get<cplx>(), up, rec.theta[p], ts.get<cplx>(),
+                                        cfg.solve_iters, cfg.solve_rtol, st);
         worst = std::max(worst, rr);
+        proj_op(Hs, b, up, rec.theta[p], ts.get<cplx>(), tp);
         const double nt = b.norm(tp);
         if (nt > 0.0) b.scale(tp, 1.0 / nt);
-        QK_HIP(hipMemcpyAsync(R.get<cplx>() + u64(p) * n, rec.r[p].get(), sizeof(cplx) * n,
-                              hipMemcpyDeviceToDevice, st));
+        mlp_jacobi(states, rec.theta[p], rec.r[p].get<cplx>(), zp, st);
+        Hs.apply(zp, hz.get<cplx>(), 1, n, st);
+        mlp_neumann(states, rec.theta[p], hz.get<cplx>(), zp, W.get<cplx>() + u64(p) * n, st);
         QK_HIP(hipMemcpyAsync(U.get<cplx>() + u64(p) * n, rec.u[p].get(), sizeof(cplx) * n,
                               hipMemcpyDeviceToDevice, st));
     }
-    QK_HIP(hipMemcpyAsync(th.get(), rec.theta.data(), sizeof(double) * P, hipMemcpyHostToDevice, st));
+    QK_HIP(hipStreamSynchronize(st));
     rec.r.clear();
     rec.u.clear();
     out.solve_rel_res = worst;
 
     // 3. features, then Adam over mini-batches of consecutive pairs
-    MlpStates states;
-    states.build(Hs, st);
     const int bp = std::max(1, std::min(cfg.batch_pairs, P));
-    DevBuf X(sizeof(u16) * n * P * kMlpIn), o(sizeof(float) * n * bp), go(sizeof(float) * n * bp),
-        h1(sizeof(u16) * n * bp * kMlpHid), h2(sizeof(u16) * n * bp * kMlpHid),
-        grads(sizeof(float) * kMlpParams), scratch;
+    DevBuf X(sizeof(u16) * n * P * kMlpIn), o(sizeof(float) * n * bp * kMlpOut),
+        go(sizeof(float) * n * bp * kMlpOut), h1(sizeof(u16) * n * bp * kMlpHid),
+        h2(sizeof(u16) * n * bp * kMlpHid), grads(sizeof(float) * kMlpParams);
+    MlpLossWork wk;
     for (int p = 0; p < P; ++p) mlp_features(states, rec.theta[p], X.get<u16>() + u64(p) * n * kMlpIn, st);
 
     auto eval_all = [&]() {
@@ -206,9 +213,9 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
         for (int p0 = 0; p0 < P; p0 += bp) {
             const int cnt = std::min(bp, P - p0);
             net.forward(X.get<u16>() + u64(p0) * n * kMlpIn, n * cnt, o.get<float>(), nullptr, nullptr, st);
-            L += mlp_pair_loss(states, R.get<cplx>() + u64(p0) * n, T.get<cplx>() + u64(p0) * n,
-                               U.get<cplx>() + u64(p0) * n, th.get<double>() + p0, cnt, o.get<float>(),
-                               go.get<float>(), scratch, st);
+            L += mlp_pair_loss(Hs, Z.get<cplx>() + u64(p0) * n, W.get<cplx>() + u64(p0) * n,
+                               T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
+                               rec.theta.data() + p0, cnt, o.get<float>(), nullptr, wk, st);
         }
         return L / P;
     };
@@ -223,9 +230,9 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
             const int cnt = std::min(bp, P - p0);
             const u16* Xb = X.get<u16>() + u64(p0) * n * kMlpIn;
             net.forward(Xb, n * cnt, o.get<float>(), h1.get<u16>(), h2.get<u16>(), st);
-            (void)mlp_pair_loss(states, R.get<cplx>() + u64(p0) * n, T.get<cplx>() + u64(p0) * n,
-                                U.get<cplx>() + u64(p0) * n, th.get<double>() + p0, cnt,
-                                o.get<float>(), go.get<float>(), scratch, st);
+            (void)mlp_pair_loss(Hs, Z.get<cplx>() + u64(p0) * n, W.get<cplx>() + u64(p0) * n,
+                                T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
+                                rec.theta.data() + p0, cnt, o.get<float>(), go.get<float>(), wk, st);
             net.backward(Xb, h1.get<u16>(), h2.get<u16>(), go.get<float>(), n * cnt,
                          grads.get<float>(), st);
             net.adam_step(grads.get<float>(), float(cfg.lr), st);
@@ -243,15 +250,31 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
 // ---------------------------------------------------------------------------
 MlpPreconditioner::MlpPreconditioner(const Mlp& net, const Hamiltonian& H) : net_(net) {
     states_.build(H, H.stream);
-    X_.resize(sizeof(u16) * std::max<u64>(states_.n, 1) * kMlpIn);
-    o_.resize(sizeof(float) * std::max<u64>(states_.n, 1));
+    const u64 n1 = std::max<u64>(states_.n, 1);
+    X_.resize(sizeof(u16) * n1 * kMlpIn);
+    o_.resize(sizeof(float) * n1 * kMlpOut);
+    z_.resize(sizeof(cplx) * n1);
+    w_.resize(sizeof(cplx) * n1);
+    c_.resize(sizeof(cplx));
 }
 
-void MlpPreconditioner::apply(const Hamiltonian&, const cplx* r, const cplx*, double lambda,
-                              const cplx*, cplx* t, hipStream_t st) {
+void MlpPreconditioner::apply(const Hamiltonian& H, const cplx* r, const cplx*, double lambda,
+                              const cplx* u, cplx* t, hipStream_t st) {
+    const u64 n = states_.n;
+    if (!n) return;
+    mlp_jacobi(states_, lambda, r, z_.get<cplx>(), st);
+    H.apply(z_.get<cplx>(), t, 1, n, st);   // t holds H z until the combine
+    mlp_neumann(states_, lambda, t, z_.get<cplx>(), w_.get<cplx>(), st);
     mlp_features(states_, lambda, X_.get<u16>(), st);
-    net_.forward(X_.get<u16>(), states_.n, o_.get<float>(), nullptr, nullptr, st);
-    mlp_combine(states_, lambda, o_.get<float>(), r, t, st);
+    net_.forward(X_.get<u16>(), n, o_.get<float>(), nullptr, nullptr, st);
+    mlp_combine(states_, o_.get<float>(), z_.get<cplx>(), w_.get<cplx>(), t, st);
+    if (u) {
+        // t approximates (H - lambda)^-1 r, which is u itself for an exact Ritz
+        // pair: remove the u component here, in one pass, before Davidson
+        // orthogonalises t against its basis
+        vec::dot(u, t, n, c_.get<cplx>(), red_, st);
+        mlp_project(u, c_.get<cplx>(), t, n, st);
+    }
 }
 
 }  // namespace qk

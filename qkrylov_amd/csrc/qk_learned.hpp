@@ -33,17 +33,23 @@ double jd_correction(const Hamiltonian& H, const cplx* r, const cplx* u, double 
 MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& cfg,
                         std::vector<double>* loss_history);
 
-// t = (jacobi + net(features)) * r for Davidson on H.
+// t = s1 z + s2 w with z = J r, w = J (H z - diag z) and (s1, s2) from the
+// network (qk_mlp.hpp), then the component along the Ritz vector u removed;
+// one extra apply of H per correction.
 class MlpPreconditioner : public Preconditioner {
 public:
     MlpPreconditioner(const Mlp& net, const Hamiltonian& H);
     void apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda, const cplx* u,
                cplx* t, hipStream_t st) override;
+    bool wants_ritz() const override { return true; }
+    // the two correction terms of the last apply (device)
+    const cplx* z() const { return z_.get<cplx>(); }
+    const cplx* w() const { return w_.get<cplx>(); }
 
 private:
     const Mlp& net_;
     MlpStates states_;
-    DevBuf X_, o_;
+    DevBuf X_, o_, z_, w_, c_, red_;
 };
 
 }  // namespace qk

@@ -46,7 +46,7 @@ __device__ __forceinline__ f32x4 mfma(const u16x8 a, const u16x8 b, const f32x4 
 __device__ __forceinline__ u16x8 ld8(const u16* p) { return *reinterpret_cast<const u16x8*>(p); }
 
 // ---------------------------------------------------------------------------
-// forward: o = w3 . relu(W2^T relu(W1^T x + b1) + b2) + b3, per row.
+// forward: o = W3^T relu(W2^T relu(W1^T x + b1) + b2) + b3, per row (2 outputs).
 // Activations are rounded to BF16 (they feed the next MFMA); h2 is rounded
 // too, so backward sees exactly what forward used.
 // ---------------------------------------------------------------------------
@@ -58,7 +58,7 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
     __shared__ __align__(16) u16 s_w1t[kHid * kIn];
     __shared__ __align__(16) u16 s_w2t[kHid * kHid];
     __shared__ __align__(16) u16 s_h1[kRows * kHid];
-    __shared__ float s_b1[kHid], s_b2[kHid], s_w3[kHid];
+    __shared__ float s_b1[kHid], s_b2[kHid], s_w3[kHid * kMlpOut];
     for (int i = threadIdx.x; i < kHid * kIn / 8; i += kThreads)
         reinterpret_cast<u16x8*>(s_w1t)[i] = reinterpret_cast<const u16x8*>(w1t)[i];
     for (int i = threadIdx.x; i < kHid * kHid / 8; i += kThreads)
@@ -66,9 +66,9 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
     for (int i = threadIdx.x; i < kHid; i += kThreads) {
         s_b1[i] = prm[kMlpB1 + i];
         s_b2[i] = prm[kMlpB2 + i];
-        s_w3[i] = d_bf2f(w3b[i]);
     }
-    const float b3 = prm[kMlpB3];
+    for (int i = threadIdx.x; i < kHid * kMlpOut; i += kThreads) s_w3[i] = d_bf2f(w3b[i]);
+    const float b30 = prm[kMlpB3], b31 = prm[kMlpB3 + 1];
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
@@ -111,15 +111,17 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
             for (int nb = 0; nb < 8; ++nb)
                 acc[nb] = mfma(a2, ld8(s_w2t + (nb * 16 + c) * kHid + kb * 32 + 8 * g), acc[nb]);
         }
-        // layer 3 and the saved h2
-        float part[4] = {0.f, 0.f, 0.f, 0.f};
+        // layer 3 (two outputs) and the saved h2
+        float part0[4] = {0.f, 0.f, 0.f, 0.f}, part1[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) {
             const int col = nb * 16 + c;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const u16 hb = d_f2bf(fmaxf(acc[nb][i] + s_b2[col], 0.f));
-                part[i] += d_bf2f(hb) * s_w3[col];
+                const float h = d_bf2f(hb);
+                part0[i] += h * s_w3[col * kMlpOut];
+                part1[i] += h * s_w3[col * kMlpOut + 1];
                 const u64 row = m0 + 4 * g + i;
                 if (h2s && row < n) h2s[row * kHid + col] = hb;
             }
@@ -127,12 +129,18 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int off = 1; off < 16; off <<= 1) part[i] += __shfl_xor(part[i], off);
+            for (int off = 1; off < 16; off <<= 1) {
+                part0[i] += __shfl_xor(part0[i], off);
+                part1[i] += __shfl_xor(part1[i], off);
+            }
         if (c == 0) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const u64 row = m0 + 4 * g + i;
-                if (row < n) out[row] = part[i] + b3;
+                if (row < n) {
+                    out[row * kMlpOut] = part0[i] + b30;
+                    out[row * kMlpOut + 1] = part1[i] + b31;
+                }
             }
         }
         __syncthreads();
@@ -140,7 +148,7 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
 }
 
 // ---------------------------------------------------------------------------
-// backward, activation part.  gz2 = go * w3 * [h2 > 0] and
+// backward, activation part.  gz2 = (W3 go) * [h2 > 0] and
 // gz1 = (gz2 W2^T) * [h1 > 0], both kept as BF16 for the weight-gradient
 // products; per-workgroup FP32 partials of gw3 = h2^T go and gb3 = sum go.
 // ---------------------------------------------------------------------------
@@ -150,28 +158,35 @@ k_mlp_backward_act(const u16* __restrict__ h1s, const u16* __restrict__ h2s,
                    const u16* __restrict__ w3b, u16* __restrict__ gz2s, u16* __restrict__ gz1s,
                    float* __restrict__ part3) {
     __shared__ __align__(16) u16 s_w2k[kHid * kHid];   // W2[k][n]: B operand of gz2 W2^T
-    __shared__ float s_w3[kHid];
-    __shared__ float s_gw3[4][kHid];
-    __shared__ float s_gb3[4];
+    __shared__ float s_w3[kHid * kMlpOut];
+    __shared__ float s_gw3[4][kHid * kMlpOut];
+    __shared__ float s_gb3[4][kMlpOut];
     for (int i = threadIdx.x; i < kHid * kHid / 8; i += kThreads)
         reinterpret_cast<u16x8*>(s_w2k)[i] = reinterpret_cast<const u16x8*>(w2k)[i];
-    for (int i = threadIdx.x; i < kHid; i += kThreads) s_w3[i] = d_bf2f(w3b[i]);
+    for (int i = threadIdx.x; i < kHid * kMlpOut; i += kThreads) s_w3[i] = d_bf2f(w3b[i]);
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
     // gw3 partial: lane accumulates columns 32kb + 8g + j of its row c
-    float gw3[kHid / 32][8];
+    float gw3[kHid / 32][8][kMlpOut];
 #pragma unroll
     for (int kb = 0; kb < kHid / 32; ++kb)
 #pragma unroll
-        for (int j = 0; j < 8; ++j) gw3[kb][j] = 0.f;
-    float gb3 = 0.f;
+        for (int j = 0; j < 8; ++j) {
+            gw3[kb][j][0] = 0.f;
+            gw3[kb][j][1] = 0.f;
+        }
+    float gb30 = 0.f, gb31 = 0.f;
     for (u64 tile = blockIdx.x; tile * kRows < n; tile += gridDim.x) {
         const u64 m0 = tile * kRows + u64(wave) * 16;
         const u64 ra = m0 + c;
         const bool rv = ra < n;
-        const float gor = rv ? go[ra] : 0.f;
-        if (g == 0) gb3 += gor;
+        const float go0 = rv ? go[ra * kMlpOut] : 0.f;
+        const float go1 = rv ? go[ra * kMlpOut + 1] : 0.f;
+        if (g == 0) {
+            gb30 += go0;
+            gb31 += go1;
+        }
         f32x4 acc[8];
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -184,8 +199,10 @@ k_mlp_backward_act(const u16* __restrict__ h1s, const u16* __restrict__ h2s,
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const float h = d_bf2f(h2v[j]);
-                gw3[kb][j] += gor * h;
-                a[j] = d_f2bf(h > 0.f ? gor * s_w3[k0 + j] : 0.f);
+                gw3[kb][j][0] += go0 * h;
+                gw3[kb][j][1] += go1 * h;
+                const float gz = go0 * s_w3[(k0 + j) * kMlpOut] + go1 * s_w3[(k0 + j) * kMlpOut + 1];
+                a[j] = d_f2bf(h > 0.f ? gz : 0.f);
             }
             if (rv) *reinterpret_cast<u16x8*>(gz2s + ra * kHid + k0) = a;
             // gh1[m][k] = sum_n gz2[m][n] W2[k][n]
@@ -212,20 +229,35 @@ k_mlp_backward_act(const u16* __restrict__ h1s, const u16* __restrict__ h2s,
 #pragma unroll
         for (int j = 0; j < 8; ++j)
 #pragma unroll
-            for (int off = 1; off < 16; off <<= 1) gw3[kb][j] += __shfl_xor(gw3[kb][j], off);
-    for (int off = 1; off < 16; off <<= 1) gb3 += __shfl_xor(gb3, off);
+            for (int o = 0; o < kMlpOut; ++o)
+#pragma unroll
+                for (int off = 1; off < 16; off <<= 1)
+                    gw3[kb][j][o] += __shfl_xor(gw3[kb][j][o], off);
+    for (int off = 1; off < 16; off <<= 1) {
+        gb30 += __shfl_xor(gb30, off);
+        gb31 += __shfl_xor(gb31, off);
+    }
     if (c == 0) {
 #pragma unroll
         for (int kb = 0; kb < kHid / 32; ++kb)
 #pragma unroll
-            for (int j = 0; j < 8; ++j) s_gw3[wave][kb * 32 + 8 * g + j] = gw3[kb][j];
+            for (int j = 0; j < 8; ++j)
+#pragma unroll
+                for (int o = 0; o < kMlpOut; ++o)
+                    s_gw3[wave][(kb * 32 + 8 * g + j) * kMlpOut + o] = gw3[kb][j][o];
     }
-    if (lane == 0) s_gb3[wave] = gb3;
+    if (lane == 0) {
+        s_gb3[wave][0] = gb30;
+        s_gb3[wave][1] = gb31;
+    }
     __syncthreads();
-    float* dst = part3 + size_t(blockIdx.x) * (kHid + 1);
-    for (int i = threadIdx.x; i < kHid; i += kThreads)
+    // W3 then b3, as in the parameter block
+    float* dst = part3 + size_t(blockIdx.x) * (kHid * kMlpOut + kMlpOut);
+    for (int i = threadIdx.x; i < kHid * kMlpOut; i += kThreads)
         dst[i] = ((s_gw3[0][i] + s_gw3[1][i]) + s_gw3[2][i]) + s_gw3[3][i];
-    if (threadIdx.x == 0) dst[kHid] = ((s_gb3[0] + s_gb3[1]) + s_gb3[2]) + s_gb3[3];
+    if (threadIdx.x < kMlpOut)
+        dst[kHid * kMlpOut + threadIdx.x] =
+            ((s_gb3[0][threadIdx.x] + s_gb3[1][threadIdx.x]) + s_gb3[2][threadIdx.x]) + s_gb3[3][threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------
@@ -325,7 +357,7 @@ __global__ void k_mlp_cast(const float* __restrict__ p, u16* __restrict__ w1t, u
         w2t[nn * kHid + k] = b;
         w2k[i] = b;
     }
-    if (i < kHid) w3b[i] = d_f2bf(p[kMlpW3 + i]);
+    if (i < kHid * kMlpOut) w3b[i] = d_f2bf(p[kMlpW3 + i]);
 }
 
 int mlp_grid(u64 n, u64 rows_per_tile) {
@@ -370,75 +402,113 @@ __global__ void k_mlp_features(const double* __restrict__ diag, const float* __r
     for (int q = 0; q < kIn / 8; ++q) reinterpret_cast<u16x8*>(X + i * kIn)[q] = v[q];
 }
 
-__global__ void k_mlp_combine(const double* __restrict__ diag, const u64 n, const double theta,
-                              const float* __restrict__ o, const cplx* __restrict__ r,
-                              cplx* __restrict__ t) {
+__global__ void k_mlp_jacobi(const double* __restrict__ diag, const u64 n, const double theta,
+                             const cplx* __restrict__ r, cplx* __restrict__ z) {
     const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double w = 1.0 / jac_den(diag[i], theta) + double(o[i]);
-    t[i] = cplx{w * r[i].re, w * r[i].im};
+    const double j = 1.0 / jac_den(diag[i], theta);
+    z[i] = cplx{j * r[i].re, j * r[i].im};
 }
 
-// per (pair, block) partial sums of A = sum conj(ts) w r, c = sum conj(u) w r,
-// S = sum w^2 |r|^2 (w = jacobi + o)
+__global__ void k_mlp_neumann(const double* __restrict__ diag, const u64 n, const double theta,
+                              const cplx* __restrict__ hz, const cplx* __restrict__ z,
+                              cplx* __restrict__ w) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double d = diag[i];
+    const double j = 1.0 / jac_den(d, theta);
+    w[i] = cplx{j * (hz[i].re - d * z[i].re), j * (hz[i].im - d * z[i].im)};
+}
+
+// the two coefficients of a state from the network output (a, b)
+__device__ __forceinline__ void mlp_coef(const float* o, u64 k, double& s1, double& s2, double& ta,
+                                         double& tb) {
+    ta = tanh(double(o[k * kMlpOut]));
+    tb = tanh(double(o[k * kMlpOut + 1]));
+    s1 = 1.0 + 0.5 * ta;
+    s2 = -0.5 + 0.25 * tb;
+}
+
+__global__ void k_mlp_combine(const u64 n, const float* __restrict__ o, const cplx* __restrict__ z,
+                              const cplx* __restrict__ w, cplx* __restrict__ t) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s1, s2, ta, tb;
+    mlp_coef(o, i, s1, s2, ta, tb);
+    t[i] = cplx{s1 * z[i].re + s2 * w[i].re, s1 * z[i].im + s2 * w[i].im};
+}
+
+__global__ void k_mlp_project(const cplx* __restrict__ u, const cplx* __restrict__ c,
+                              cplx* __restrict__ t, const u64 n) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const cplx cc = *c, uu = u[i];
+    t[i].re -= cc.re * uu.re - cc.im * uu.im;
+    t[i].im -= cc.re * uu.im + cc.im * uu.re;
+}
+
+// ---- training loss ----------------------------------------------------------
+// Per-pair reductions and updates over P pairs of n states stored pair-major
+// (grid.y = pair).
 constexpr int kLossT = 256;
-__global__ void __launch_bounds__(kLossT)
-k_mlp_loss_part(const double* __restrict__ diag, const u64 n, const double* __restrict__ theta,
-                const float* __restrict__ o, const cplx* __restrict__ r, const cplx* __restrict__ ts,
-                const cplx* __restrict__ u, const int nblk, double* __restrict__ part) {
-    const int p = blockIdx.y;
-    const double th = theta[p];
-    double s[5] = {0, 0, 0, 0, 0};
-    for (u64 i = u64(blockIdx.x) * kLossT + threadIdx.x; i < n; i += u64(nblk) * kLossT) {
-        const u64 k = u64(p) * n + i;
-        const double w = 1.0 / jac_den(diag[i], th) + double(o[k]);
-        const cplx rr = r[k], tt = ts[k], uu = u[k];
-        // conj(tt) * rr and conj(uu) * rr
-        const double ar = tt.re * rr.re + tt.im * rr.im, ai = tt.re * rr.im - tt.im * rr.re;
-        const double cr = uu.re * rr.re + uu.im * rr.im, ci = uu.re * rr.im - uu.im * rr.re;
-        s[0] += w * ar;
-        s[1] += w * ai;
-        s[2] += w * cr;
-        s[3] += w * ci;
-        s[4] += w * w * (rr.re * rr.re + rr.im * rr.im);
-    }
-    __shared__ double sh[5][kLossT / 64];
+
+template <int NS>
+__device__ __forceinline__ void block_sums(double (&s)[NS], double* dst) {
+    __shared__ double sh[NS][kLossT / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
+    for (int q = 0; q < NS; ++q) {
         double v = s[q];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         if (lane == 0) sh[q][wv] = v;
     }
     __syncthreads();
-    if (threadIdx.x < 5) {
+    if (threadIdx.x < NS) {
         double v = 0.0;
-        for (int w = 0; w < kLossT / 64; ++w) v += sh[threadIdx.x][w];
-        part[(size_t(p) * nblk + blockIdx.x) * 5 + threadIdx.x] = v;
+        for (int q = 0; q < kLossT / 64; ++q) v += sh[threadIdx.x][q];
+        dst[threadIdx.x] = v;
     }
 }
 
-// go[k] = (1/P) dL_p/dw_k with L_p = 1 - |A|^2 / (B C), B = S - |c|^2
-__global__ void k_mlp_loss_grad(const double* __restrict__ diag, const u64 n,
-                                const double* __restrict__ theta, const float* __restrict__ o,
-                                const cplx* __restrict__ r, const cplx* __restrict__ ts,
-                                const cplx* __restrict__ u, const double* __restrict__ coef,
-                                const int P, float* __restrict__ go) {
+// per (pair, block): sum conj(x) y (re, im) and sum |y|^2
+__global__ void __launch_bounds__(kLossT)
+k_pair_dot(const cplx* __restrict__ x, const cplx* __restrict__ y, const u64 n, const int nblk,
+           double* __restrict__ part) {
+    const int p = blockIdx.y;
+    double s[3] = {0, 0, 0};
+    for (u64 i = u64(blockIdx.x) * kLossT + threadIdx.x; i < n; i += u64(nblk) * kLossT) {
+        const u64 k = u64(p) * n + i;
+        const cplx a = x[k], b = y[k];
+        s[0] += a.re * b.re + a.im * b.im;
+        s[1] += a.re * b.im - a.im * b.re;
+        s[2] += b.re * b.re + b.im * b.im;
+    }
+    block_sums<3>(s, part + (size_t(p) * nblk + blockIdx.x) * 3);
+}
+
+// out[k] = ca[p] a[k] + cb[p] b[k]  (out may alias b)
+__global__ void k_pair_lincomb(const cplx* __restrict__ ca, const cplx* a, const cplx* __restrict__ cb,
+                               const cplx* b, cplx* out, const u64 n, const int P) {
     const u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= u64(P) * n) return;
     const int p = int(k / n);
-    const u64 i = k % n;
-    const double* cf = coef + size_t(p) * 8;   // Ar, Ai, cr, ci, alpha, beta
-    const double w = 1.0 / jac_den(diag[i], theta[p]) + double(o[k]);
-    const cplx rr = r[k], tt = ts[k], uu = u[k];
-    const double ar = tt.re * rr.re + tt.im * rr.im, ai = tt.re * rr.im - tt.im * rr.re;
-    const double cr = uu.re * rr.re + uu.im * rr.im, ci = uu.re * rr.im - uu.im * rr.re;
-    // Re(conj(A) * (conj(ts) r)) and Re(conj(c) * (conj(u) r))
-    const double reA = cf[0] * ar + cf[1] * ai;
-    const double reC = cf[2] * cr + cf[3] * ci;
-    const double r2 = rr.re * rr.re + rr.im * rr.im;
-    const double gw = cf[4] * reA + cf[5] * (w * r2 - reC);
-    go[k] = float(gw / double(P));
+    const cplx x = ca[p], y = cb[p], av = a[k], bv = b[k];
+    out[k] = cplx{x.re * av.re - x.im * av.im + y.re * bv.re - y.im * bv.im,
+                  x.re * av.im + x.im * av.re + y.re * bv.im + y.im * bv.re};
+}
+
+// go = d(loss / P)/d(a, b) from h = K g:  dL/ds1 = Re(conj(h) z), dL/ds2 = Re(conj(h) w)
+__global__ void k_mlp_grad_coef(const float* __restrict__ o, const cplx* __restrict__ z,
+                                const cplx* __restrict__ w, const cplx* __restrict__ h, const u64 cnt,
+                                const double invP, float* __restrict__ go) {
+    const u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    double s1, s2, ta, tb;
+    mlp_coef(o, k, s1, s2, ta, tb);
+    const cplx hh = h[k], zz = z[k], ww = w[k];
+    const double g1 = hh.re * zz.re + hh.im * zz.im, g2 = hh.re * ww.re + hh.im * ww.im;
+    go[k * kMlpOut] = float(g1 * 0.5 * (1.0 - ta * ta) * invP);
+    go[k * kMlpOut + 1] = float(g2 * 0.25 * (1.0 - tb * tb) * invP);
 }
 
 }  // namespace
@@ -454,9 +524,9 @@ Mlp::Mlp(unsigned seed) {
     w1t.resize(sizeof(u16) * kIn * kHid);
     w2t.resize(sizeof(u16) * kHid * kHid);
     w2k.resize(sizeof(u16) * kHid * kHid);
-    w3b.resize(sizeof(u16) * kHid);
+    w3b.resize(sizeof(u16) * kHid * kMlpOut);
     // He-uniform hidden layers; the output layer starts at zero, so the
-    // untrained network is the Jacobi preconditioner.
+    // untrained network gives the damped Neumann correction z - w/2.
     std::vector<float> h(kMlpParams, 0.0f);
     std::mt19937 rng(seed);
     std::uniform_real_distribution<float> u(-1.0f, 1.0f);
@@ -510,7 +580,7 @@ void Mlp::backward(const u16* X, const u16* h1, const u16* h2, const float* go, 
     u16* gz1 = gz2 + n * kHid;
     const int ga = mlp_grid(n, kRows);
     const int gw = mlp_grid(n, 32);
-    const size_t s3 = kHid + 1, s1 = size_t(kIn) * kHid + kHid, s2 = size_t(kHid) * kHid + kHid;
+    const size_t s3 = size_t(kHid) * kMlpOut + kMlpOut, s1 = size_t(kIn) * kHid + kHid, s2 = size_t(kHid) * kHid + kHid;
     part_.resize(sizeof(float) * (size_t(ga) * s3 + size_t(gw) * (s1 + s2)));
     float* p3 = part_.get<float>();
     float* p1 = p3 + size_t(ga) * s3;
@@ -524,8 +594,8 @@ void Mlp::backward(const u16* X, const u16* h1, const u16* h2, const float* go, 
                        int(s1), int(s1), grads + kMlpW1);
     hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(s2) + 255) / 256), dim3(256), 0, st, p2, gw,
                        int(s2), int(s2), grads + kMlpW2);
-    hipLaunchKernelGGL(k_mlp_sum_parts, dim3(1), dim3(256), 0, st, p3, ga, int(s3), int(s3),
-                       grads + kMlpW3);
+    hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(s3) + 255) / 256), dim3(256), 0, st, p3, ga,
+                       int(s3), int(s3), grads + kMlpW3);
     QK_HIP(hipGetLastError());
 }
 
@@ -545,14 +615,16 @@ void Mlp::adam_step(const float* grads, float lr, hipStream_t st) {
 void Mlp::eval_host(const float* Xh, u64 n, const float* goh, float* oh, float* gradh) {
     std::vector<u16> xb(n * kIn);
     for (u64 i = 0; i < n * kIn; ++i) xb[i] = f32_to_bf16(Xh[i]);
-    DevBuf X(sizeof(u16) * std::max<u64>(n, 1) * kIn), o(sizeof(float) * std::max<u64>(n, 1)),
-        h1(sizeof(u16) * std::max<u64>(n, 1) * kHid), h2(sizeof(u16) * std::max<u64>(n, 1) * kHid),
-        go(sizeof(float) * std::max<u64>(n, 1)), g(sizeof(float) * kMlpParams);
+    const u64 n1 = std::max<u64>(n, 1);
+    DevBuf X(sizeof(u16) * n1 * kIn), o(sizeof(float) * n1 * kMlpOut), h1(sizeof(u16) * n1 * kHid),
+        h2(sizeof(u16) * n1 * kHid), go(sizeof(float) * n1 * kMlpOut), g(sizeof(float) * kMlpParams);
     if (n) QK_HIP(hipMemcpyAsync(X.get(), xb.data(), sizeof(u16) * n * kIn, hipMemcpyHostToDevice, st_));
     forward(X.get<u16>(), n, o.get<float>(), h1.get<u16>(), h2.get<u16>(), st_);
-    if (oh && n) QK_HIP(hipMemcpyAsync(oh, o.get(), sizeof(float) * n, hipMemcpyDeviceToHost, st_));
+    if (oh && n)
+        QK_HIP(hipMemcpyAsync(oh, o.get(), sizeof(float) * n * kMlpOut, hipMemcpyDeviceToHost, st_));
     if (goh && gradh) {
-        if (n) QK_HIP(hipMemcpyAsync(go.get(), goh, sizeof(float) * n, hipMemcpyHostToDevice, st_));
+        if (n)
+            QK_HIP(hipMemcpyAsync(go.get(), goh, sizeof(float) * n * kMlpOut, hipMemcpyHostToDevice, st_));
         backward(X.get<u16>(), h1.get<u16>(), h2.get<u16>(), go.get<float>(), n, g.get<float>(), st_);
         QK_HIP(hipMemcpyAsync(gradh, g.get(), sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, st_));
     }
@@ -622,52 +694,131 @@ void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st) {
     QK_HIP(hipGetLastError());
 }
 
-void mlp_combine(const MlpStates& s, double theta, const float* o, const cplx* r, cplx* t,
-                 hipStream_t st) {
+void mlp_jacobi(const MlpStates& s, double theta, const cplx* r, cplx* z, hipStream_t st) {
     if (!s.n) return;
-    hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st,
-                       s.diag.get<double>(), s.n, theta, o, r, t);
+    hipLaunchKernelGGL(k_mlp_jacobi, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st,
+                       s.diag.get<double>(), s.n, theta, r, z);
     QK_HIP(hipGetLastError());
 }
 
-double mlp_pair_loss(const MlpStates& s, const cplx* r, const cplx* ts, const cplx* u,
-                     const double* theta_dev, int P, const float* o, float* go, DevBuf& scratch,
-                     hipStream_t st) {
-    if (!s.n || P <= 0) return 0.0;
-    const int nblk = int(std::min<u64>(64, (s.n + kLossT - 1) / kLossT));
-    scratch.resize(sizeof(double) * (size_t(P) * nblk * 5 + size_t(P) * 8));
-    double* part = scratch.get<double>();
-    double* coef = part + size_t(P) * nblk * 5;
-    hipLaunchKernelGGL(k_mlp_loss_part, dim3(nblk, P), dim3(kLossT), 0, st, s.diag.get<double>(), s.n,
-                       theta_dev, o, r, ts, u, nblk, part);
-    std::vector<double> ph(size_t(P) * nblk * 5);
-    QK_HIP(hipMemcpyAsync(ph.data(), part, sizeof(double) * ph.size(), hipMemcpyDeviceToHost, st));
+void mlp_neumann(const MlpStates& s, double theta, const cplx* hz, const cplx* z, cplx* w,
+                 hipStream_t st) {
+    if (!s.n) return;
+    hipLaunchKernelGGL(k_mlp_neumann, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st,
+                       s.diag.get<double>(), s.n, theta, hz, z, w);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_combine(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
+                 hipStream_t st) {
+    if (!s.n) return;
+    hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st, s.n, o,
+                       z, w, t);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_project(const cplx* u, const cplx* c, cplx* t, u64 n, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_mlp_project, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, u, c, t, n);
+    QK_HIP(hipGetLastError());
+}
+
+namespace {
+
+// the P per-pair sums conj(x) y and |y|^2, on the host
+void pair_dot(const cplx* x, const cplx* y, u64 n, int P, std::vector<cplx>& xy,
+              std::vector<double>* yy, MlpLossWork& wk, hipStream_t st) {
+    const int nblk = int(std::min<u64>(64, (n + kLossT - 1) / kLossT));
+    wk.part.resize(sizeof(double) * size_t(P) * nblk * 3);
+    hipLaunchKernelGGL(k_pair_dot, dim3(nblk, P), dim3(kLossT), 0, st, x, y, n, nblk,
+                       wk.part.get<double>());
+    std::vector<double> h(size_t(P) * nblk * 3);
+    QK_HIP(hipMemcpyAsync(h.data(), wk.part.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     QK_HIP(hipStreamSynchronize(st));
-    // |ts|^2 per pair is needed for the loss value only; the gradient
-    // direction does not depend on it, so it is folded into the scale below.
-    std::vector<double> cf(size_t(P) * 8, 0.0);
+    xy.assign(P, cplx{0.0, 0.0});
+    if (yy) yy->assign(P, 0.0);
+    for (int p = 0; p < P; ++p)
+        for (int b = 0; b < nblk; ++b) {
+            const double* v = h.data() + (size_t(p) * nblk + b) * 3;
+            xy[p].re += v[0];
+            xy[p].im += v[1];
+            if (yy) (*yy)[p] += v[2];
+        }
+}
+
+// out_p = ca_p a_p + cb_p b_p with host coefficients
+void pair_lincomb(const std::vector<cplx>& ca, const cplx* a, const std::vector<cplx>& cb, const cplx* b,
+                  cplx* out, u64 n, int P, MlpLossWork& wk, hipStream_t st) {
+    wk.coef.resize(sizeof(cplx) * 2 * size_t(P));
+    std::vector<cplx> c(ca);
+    c.insert(c.end(), cb.begin(), cb.end());
+    QK_HIP(hipMemcpyAsync(wk.coef.get(), c.data(), sizeof(cplx) * c.size(), hipMemcpyHostToDevice, st));
+    const u64 tot = u64(P) * n;
+    hipLaunchKernelGGL(k_pair_lincomb, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st,
+                       wk.coef.get<cplx>(), a, wk.coef.get<cplx>() + P, b, out, n, P);
+    QK_HIP(hipGetLastError());
+    // the coefficient buffer is reused by the next call
+    QK_HIP(hipStreamSynchronize(st));
+}
+
+// x_p -= <u_p, x_p> u_p
+void pair_project(const cplx* u, cplx* x, u64 n, int P, MlpLossWork& wk, hipStream_t st) {
+    std::vector<cplx> c;
+    pair_dot(u, x, n, P, c, nullptr, wk, st);
+    for (cplx& v : c) v = cplx{-v.re, -v.im};
+    pair_lincomb(c, u, std::vector<cplx>(P, cplx{1.0, 0.0}), x, x, n, P, wk, st);
+}
+
+// y = P_u (H - theta) x for x orthogonal to u (P pairs)
+void pair_k(const Hamiltonian& H, const cplx* u, const double* theta, const cplx* x, cplx* y, int P,
+            MlpLossWork& wk, hipStream_t st) {
+    const u64 n = H.dim();
+    H.apply(x, y, P, n, st);
+    std::vector<cplx> mt(P);
+    for (int p = 0; p < P; ++p) mt[p] = cplx{-theta[p], 0.0};
+    pair_lincomb(mt, x, std::vector<cplx>(P, cplx{1.0, 0.0}), y, y, n, P, wk, st);
+    pair_project(u, y, n, P, wk, st);
+}
+
+}  // namespace
+
+double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* b, const cplx* u,
+                     const double* theta, int P, const float* o, float* go, MlpLossWork& wk,
+                     hipStream_t st) {
+    const u64 n = H.dim();
+    if (!n || P <= 0) return 0.0;
+    const u64 tot = u64(P) * n;
+    wk.t.resize(sizeof(cplx) * tot);
+    wk.q.resize(sizeof(cplx) * tot);
+    wk.g.resize(sizeof(cplx) * tot);
+    cplx* t = wk.t.get<cplx>();
+    cplx* q = wk.q.get<cplx>();
+    cplx* g = wk.g.get<cplx>();
+    // t = P_u (s1 z + s2 w), q = K t
+    hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, tot, o, z, w, t);
+    QK_HIP(hipGetLastError());
+    pair_project(u, t, n, P, wk, st);
+    pair_k(H, u, theta, t, q, P, wk, st);
+    // a = <b, q>, Q = |q|^2 (b unit)
+    std::vector<cplx> a;
+    std::vector<double> Q;
+    pair_dot(b, q, n, P, a, &Q, wk, st);
     double loss = 0.0;
+    std::vector<cplx> ca(P), cq(P);
     for (int p = 0; p < P; ++p) {
-        double v[5] = {0, 0, 0, 0, 0};
-        for (int b = 0; b < nblk; ++b)
-            for (int q = 0; q < 5; ++q) v[q] += ph[(size_t(p) * nblk + b) * 5 + q];
-        const double A2 = v[0] * v[0] + v[1] * v[1];
-        const double Bv = v[4] - (v[2] * v[2] + v[3] * v[3]);
-        // ts is stored normalised (|ts| = 1), so C = 1
-        const double Bs = std::max(Bv, 1e-300);
-        loss += 1.0 - A2 / Bs;
-        double* c = cf.data() + size_t(p) * 8;
-        c[0] = v[0];
-        c[1] = v[1];
-        c[2] = v[2];
-        c[3] = v[3];
-        c[4] = -2.0 / Bs;
-        c[5] = 2.0 * A2 / (Bs * Bs);
+        const double A2 = a[p].re * a[p].re + a[p].im * a[p].im;
+        const double Qs = std::max(Q[p], 1e-300);
+        loss += 1.0 - A2 / Qs;
+        // g = -2 a b / Q + 2 |a|^2 q / Q^2
+        ca[p] = cplx{-2.0 * a[p].re / Qs, -2.0 * a[p].im / Qs};
+        cq[p] = cplx{2.0 * A2 / (Qs * Qs), 0.0};
     }
-    QK_HIP(hipMemcpyAsync(coef, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice, st));
-    const u64 tot = u64(P) * s.n;
-    hipLaunchKernelGGL(k_mlp_loss_grad, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st,
-                       s.diag.get<double>(), s.n, theta_dev, o, r, ts, u, coef, P, go);
+    if (!go) return loss;
+    pair_lincomb(ca, b, cq, q, g, n, P, wk, st);
+    pair_project(u, g, n, P, wk, st);
+    pair_k(H, u, theta, g, t, P, wk, st);   // t = h = K g
+    hipLaunchKernelGGL(k_mlp_grad_coef, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, o, z, w, t,
+                       tot, 1.0 / double(P), go);
     QK_HIP(hipGetLastError());
     QK_HIP(hipStreamSynchronize(st));
     return loss;

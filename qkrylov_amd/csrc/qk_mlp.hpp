@@ -1,15 +1,23 @@
 // Learned preconditioner for the Davidson correction equation (H - theta) t = -r.
 //
-// A three-layer MLP (32 -> 128 -> 128 -> 1, ReLU) reads per-state features
-// (diagonal element, theta, connectivity, local occupations) and returns a
-// per-state correction w_i to the Jacobi multiplier 1/(H_ii - theta):
-//     t_i = (1/(H_ii - theta) + o_i) * r_i .
-// With the last layer at zero the network is exactly the Jacobi method.
+// With J_i = 1/(H_ii - theta) (the reference's clamped Jacobi factor),
+// z = J r and w = J (O z), O = H - diag(H) (one extra apply), the correction is
+//     t_i = s1_i z_i + s2_i w_i,
+//     s1_i = 1 + 0.5 tanh(a_i),  s2_i = -0.5 + 0.25 tanh(b_i),
+// where (a_i, b_i) is the output of a three-layer MLP (32 -> 128 -> 128 -> 2,
+// ReLU) reading per-state features (diagonal element, theta, connectivity,
+// local occupations).  An untrained network (zero output layer) gives
+// t = z - w/2: the first-order Neumann expansion of (D + O - theta)^-1 r,
+// damped by one half; the network moves each state's two coefficients within
+// those bounds.  The component along the Ritz vector u is removed from t.
 // Weights are kept in FP32 (master copy, Adam moments) and cast to BF16 for
 // the kernels; every matrix product runs on gfx950 MFMA (BF16 inputs, FP32
 // accumulation).  Training minimises, per (residual, accurate correction)
-// pair, 1 - cos^2 between P_u t and the accurate correction t*, where P_u
-// projects out the Ritz vector u (Davidson orthogonalises against it).
+// pair, 1 - cos^2 between t and the accurate correction t* in the norm of
+// the correction operator K = P_u (H - theta) P_u (mlp_pair_loss): a plain
+// cosine weighs every error component alike and prefers undamped Neumann
+// terms, whose high-energy errors stall Davidson; K's norm weighs them by
+// their energy.
 #pragma once
 
 #include "qk_device.hpp"
@@ -22,16 +30,18 @@ struct Hamiltonian;
 
 constexpr int kMlpIn = 32;     // input features (zero-padded)
 constexpr int kMlpHid = 128;   // hidden width
+constexpr int kMlpOut = 2;     // (a, b) per state
 
-// FP32 parameter block, offsets in floats.  W1[k][n] and W2[k][n] are stored
-// input-major (row k = input feature / hidden unit of the previous layer).
+// FP32 parameter block, offsets in floats.  W1[k][n], W2[k][n] and W3[k][n]
+// are stored input-major (row k = input feature / hidden unit of the
+// previous layer).
 constexpr int kMlpW1 = 0;
 constexpr int kMlpB1 = kMlpW1 + kMlpIn * kMlpHid;
 constexpr int kMlpW2 = kMlpB1 + kMlpHid;
 constexpr int kMlpB2 = kMlpW2 + kMlpHid * kMlpHid;
 constexpr int kMlpW3 = kMlpB2 + kMlpHid;
-constexpr int kMlpB3 = kMlpW3 + kMlpHid;
-constexpr int kMlpParams = kMlpB3 + 1;
+constexpr int kMlpB3 = kMlpW3 + kMlpHid * kMlpOut;
+constexpr int kMlpParams = kMlpB3 + kMlpOut;
 
 // Features written per state (the rest of the 32 inputs are zero).
 constexpr int kMlpFeatures = 10;
@@ -69,11 +79,11 @@ public:
     void get_params(float* host) const;       // kMlpParams floats
     void set_params(const float* host);
 
-    // o = net(X) for n rows of kMlpIn BF16 features (device).  With h1/h2
-    // non-null the BF16 activations are kept for backward().
+    // o[m*2 + k] = net(X)_k for n rows of kMlpIn BF16 features (device).
+    // With h1/h2 non-null the BF16 activations are kept for backward().
     void forward(const u16* X, u64 n, float* o, u16* h1, u16* h2, hipStream_t st) const;
-    // grads (device, kMlpParams floats) = d(sum_m go[m] o[m]) / d(params),
-    // from the activations of forward().
+    // grads (device, kMlpParams floats) = d(sum_m,k go[m*2+k] o[m*2+k]) /
+    // d(params), from the activations of forward().
     void backward(const u16* X, const u16* h1, const u16* h2, const float* go, u64 n,
                   float* grads, hipStream_t st);
     void adam_step(const float* grads, float lr, hipStream_t st);
@@ -81,13 +91,11 @@ public:
     // Host-buffer forward/backward on given features (gradient checks).
     void eval_host(const float* X, u64 n, const float* go, float* o, float* grads);
 
-    // Per-state features for (H, theta) into X (device, n x kMlpIn BF16).
-    // `stat` holds the per-state statistics of H (see MlpStates).
     hipStream_t stream() const { return st_; }
 
     DevBuf params;            // FP32 master
     DevBuf m1, m2;            // Adam moments
-    DevBuf w1t, w2t, w2k, w3b;   // BF16: W1^T [n][k], W2^T [n][k], W2 [k][n], w3
+    DevBuf w1t, w2t, w2k, w3b;   // BF16: W1^T [n][k], W2^T [n][k], W2 [k][n], W3 [k][2]
     int adam_t = 0;
 
 private:
@@ -107,16 +115,33 @@ struct MlpStates {
 
 // X[i] = features of state i for theta (device).
 void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st);
-// t = (jacobi(theta) + o) * r  (device; the Jacobi clamp is the reference's)
-void mlp_combine(const MlpStates& s, double theta, const float* o, const cplx* r, cplx* t,
+// z = J r (device; J the reference's clamped Jacobi factor for theta)
+void mlp_jacobi(const MlpStates& s, double theta, const cplx* r, cplx* z, hipStream_t st);
+// w = J (hz - diag z), hz = H z  (device)
+void mlp_neumann(const MlpStates& s, double theta, const cplx* hz, const cplx* z, cplx* w,
                  hipStream_t st);
+// t = s1(o) z + s2(o) w  (device)
+void mlp_combine(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
+                 hipStream_t st);
+// t -= c u, c a device scalar  (device)
+void mlp_project(const cplx* u, const cplx* c, cplx* t, u64 n, hipStream_t st);
 
-// Loss and per-state gradient of a batch of P pairs that share one basis:
-// rows of pair p are [p*n, (p+1)*n).  r, ts (accurate corrections), u (Ritz
-// vectors) are complex FP64, theta per pair.  Returns the summed loss over
-// the pairs and writes go = d(mean loss)/d(o) (FP32, P*n).
-double mlp_pair_loss(const MlpStates& s, const cplx* r, const cplx* ts, const cplx* u,
-                     const double* theta_dev, int P, const float* o, float* go, DevBuf& scratch,
+// Work buffers of mlp_pair_loss (grown on demand).
+struct MlpLossWork {
+    DevBuf t, q, g, part, coef;
+};
+
+// Training loss of a batch of P pairs of H's basis, rows of pair p at
+// [p*n, (p+1)*n).  With K = P_u (H - theta) P_u (the operator of the
+// correction equation, P_u = 1 - u u^H) and t = P_u (s1 z + s2 w):
+//     L_p = 1 - |<b, K t>|^2 / |K t|^2,   b = K t* / |K t*|,
+// the cosine between the learned and the accurate correction t* measured in
+// K's norm (K t* = -r up to the solver's tolerance).  z, w, b, u (unit) are
+// complex FP64, theta on the host.  Returns the summed loss; with go
+// non-null also go = d(mean loss)/d(o) (FP32, P*n*2): two batched applies of
+// H per call.
+double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* b, const cplx* u,
+                     const double* theta, int P, const float* o, float* go, MlpLossWork& wk,
                      hipStream_t st);
 
 }  // namespace qk

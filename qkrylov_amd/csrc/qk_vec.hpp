@@ -46,11 +46,11 @@ void lanczos_sub(cplx* w, const cplx* v, const cplx* u, u64 n, const double* alp
 // h[j] = sum conj(V_j) w, j < k; V: device array of k vector pointers.
 void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& scratch,
           hipStream_t st);
-// w -= sum_j V_j h[j]
 // G[i*nw + j] = <V_i, W_j> = sum conj(V_i) W_j for m <= 64, nw <= 16 vectors
 // (FP64 MFMA; each vector read once; deterministic two-stage reduction).
 void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n, cplx* G,
                DevBuf& scratch, hipStream_t st);
+// w -= sum_j V_j h[j]
 void msub(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, hipStream_t st);
 // out = sum_j c[j] V_j  (c on device)
 void lincomb(const cplx* const* V, int k, const cplx* c, cplx* out, u64 n, hipStream_t st);

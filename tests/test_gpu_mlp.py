@@ -1,7 +1,8 @@
 """Learned Davidson preconditioner on the GPU: BF16-MFMA forward and backward
 kernels against the CPU oracle (oracle/mlp_oracle.py), the features kernel,
-the MINRES correction against a dense solve, training, and Davidson with
-the trained network against the Jacobi method."""
+the correction terms and the pair loss against the oracle, the MINRES
+correction against a dense solve, training, and Davidson with the trained
+network against the Jacobi method."""
 import time
 
 import numpy as np
@@ -39,20 +40,82 @@ def test_gradients_match_oracle(qk):
     rng = np.random.default_rng(1)
     for n in (50, 777):
         X = rng.standard_normal((n, 32)).astype(np.float32)
-        go = rng.standard_normal(n).astype(np.float32)
+        go = rng.standard_normal((n, 2)).astype(np.float32)
         _, g = net.eval(X, go)
         ref = M.backward(p, X, go)
         for name, a, b in (("W1", M.W1, M.B1), ("b1", M.B1, M.W2), ("W2", M.W2, M.B2),
-                           ("b2", M.B2, M.W3), ("w3", M.W3, M.B3), ("b3", M.B3, M.NPARAMS)):
+                           ("b2", M.B2, M.W3), ("W3", M.W3, M.B3), ("b3", M.B3, M.NPARAMS)):
             rel = np.linalg.norm(g[a:b] - ref[a:b]) / (np.linalg.norm(ref[a:b]) + 1e-12)
             print(f"[mlp grad] n={n} {name} rel={rel:.2e}", flush=True)
             assert rel < 2e-2, (name, rel)
 
 
-def test_untrained_network_is_jacobi(qk):
+def test_untrained_network_is_damped_neumann(qk):
     net = qk.MLPPreconditioner(5)
     X = np.random.default_rng(2).standard_normal((300, 32)).astype(np.float32)
-    assert np.all(net.eval(X) == 0.0)
+    o = net.eval(X)
+    assert o.shape == (300, 2) and np.all(o == 0.0)
+
+
+def _tj_small(qk, orc, lat, lx=3, ly=2, k=2):
+    n = lx * ly
+    ops = lat.tj_model(lat.square_bonds(lx, ly), 1.0, 0.4)
+    sec = qk.Sector().set_hubbard_particles(k, k)
+    H = qk.MatrixFreeHamiltonian(qk.TJBasis(n, sector=sec), qk.TJSite(), ops)
+    ob = orc.Basis("tj", n, {"use_nup": True, "nup": k, "use_ndn": True, "ndn": k})
+    A = orc.Ham(ob, "tj", ops).to_scipy().tocsr()
+    return H, A
+
+
+def test_correction_matches_oracle(qk, orc, lat):
+    for lx, ly, k in ((3, 2, 2), (4, 3, 3)):
+        H, A = _tj_small(qk, orc, lat, lx, ly, k)
+        d = A.shape[0]
+        diag = A.diagonal().real
+        rng = np.random.default_rng(lx * ly)
+        net = _net(qk, 6)
+        r = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+        for theta in (float(diag.min()) - 0.7, float(np.median(diag))):
+            t, z, w = qk.mlp_correction(net, H, r, theta)
+            z_ref, w_ref = M.neumann_terms(diag, theta, r, lambda x: A @ x)
+            ez = np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref)
+            ew = np.linalg.norm(w - w_ref) / np.linalg.norm(w_ref)
+            # the combine on the network's own output is exact arithmetic
+            o = net.eval(qk.mlp_features(H, theta))
+            et = np.linalg.norm(t - M.combine(o, z, w)) / np.linalg.norm(t)
+            # and the network output agrees with the oracle's forward pass
+            o_ref, _ = M.forward(net.get_params(), qk.mlp_features(H, theta))
+            eo = np.linalg.norm(t - M.combine(o_ref, z_ref, w_ref)) / np.linalg.norm(t)
+            print(f"[mlp correction] {lx}x{ly} d={d} theta={theta:.3f} z {ez:.1e} w {ew:.1e} "
+                  f"t {et:.1e} t(oracle net) {eo:.1e}", flush=True)
+            assert ez < 1e-12 and ew < 1e-12 and et < 1e-12
+            assert eo < 1e-2
+
+
+def test_pair_loss_matches_oracle(qk, orc, lat):
+    for lx, ly, k in ((3, 2, 2), (4, 3, 3)):
+        H, A = _tj_small(qk, orc, lat, lx, ly, k)
+        n = A.shape[0]
+        rng = np.random.default_rng(12)
+        P = 3
+        c = lambda: rng.standard_normal((P, n)) + 1j * rng.standard_normal((P, n))
+        z, w, u, b = c(), c(), c(), c()
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        b /= np.linalg.norm(b, axis=1, keepdims=True)
+        theta = A.diagonal().real.min() - np.array([0.3, 1.1, 0.05])
+        o = (rng.standard_normal((P, n, 2)) * 0.7).astype(np.float32)
+        L, g = qk.mlp_pair_loss(H, o, z, w, b, u, theta)
+        Lref, gref = 0.0, []
+        for p in range(P):
+            lp, gp = M.pair_loss(o[p], z[p], w[p], b[p], u[p], theta[p], lambda x: A @ x)
+            Lref += lp
+            gref.append(gp / P)
+        gref = np.array(gref)
+        eg = np.max(np.abs(g - gref)) / np.max(np.abs(gref))
+        print(f"[mlp loss] {lx}x{ly} d={n} L={L:.12f} ref={Lref:.12f} grad max rel={eg:.1e}", flush=True)
+        assert abs(L - Lref) <= 1e-10 * P
+        assert eg < 1e-6      # the gradient is stored in FP32
+        assert qk.mlp_pair_loss(H, o, z, w, b, u, theta, grad=False) == L
 
 
 def _spin_chain(qk, lat, n):
@@ -123,20 +186,34 @@ def test_training_lowers_loss_and_davidson_agrees(qk, lat):
     assert net.last_train["solve_rel_res"] < 1e-3   # MINRES, at most 500 iterations per pair
     assert hist[-1] < hist[0]
 
-    # Davidson on the training lattice and on 2x5 (d = 3,150): both methods
-    # converge and agree; the iteration counts are printed (the gain on the
-    # 4x4 target lattice is measured by bench.py)
-    for H, tag in ((Hs, "2x4"), (tj(2, 5, 4), "2x5")):
+    # Davidson on the training lattice and on 2x5 (d = 3,150).  Every variant
+    # converges, and every converged Ritz value is an eigenvalue of H (dense
+    # spectrum).  On 2x4 Jacobi (the reference's method) and the trained
+    # network both find the lowest four; on 2x5 the reference's start (unit
+    # vectors on the lowest diagonal entries) misses the fourth with Jacobi
+    # too, so only the eigenvalue check applies there.  The gain on the 4x4
+    # target lattice is measured by bench.py.
+    from oracle import oracle as orc
+    for (lx, ly, k), tag in (((2, 4, 3), "2x4"), ((2, 5, 4), "2x5")):
+        H = tj(lx, ly, k)
+        ob = orc.Basis("tj", lx * ly, {"use_nup": True, "nup": k, "use_ndn": True, "ndn": k})
+        A = orc.Ham(ob, "tj", lat.tj_model(lat.square_bonds(lx, ly), 1.0, 0.4)).to_scipy().toarray()
+        spec = np.linalg.eigvalsh(A)
         res = {}
-        for name, pre in (("jacobi", None), ("mlp", net)):
+        for name, pre in (("jacobi", None), ("neumann", qk.MLPPreconditioner(3)), ("mlp", net)):
             t = time.time()
             r = qk.davidson_lowest_ex(H, 4, 24, 1e-8, 400, pre)
-            res[name] = (r, time.time() - t)
+            res[name] = r
+            ev = np.asarray(r.eigenvalues)
+            dist = np.max(np.min(np.abs(ev[:, None] - spec[None, :]), axis=1))
             print(f"[davidson {tag}] {name}: iterations={r.iterations} converged={r.converged} "
-                  f"time={res[name][1]:.2f} s ev={r.eigenvalues}", flush=True)
-        rj, rm = res["jacobi"][0], res["mlp"][0]
-        assert rj.converged and rm.converged
-        np.testing.assert_allclose(rm.eigenvalues, rj.eigenvalues, rtol=0, atol=1e-10)
+                  f"time={time.time() - t:.2f} s ev={ev} lowest={spec[:4]} "
+                  f"max distance to the spectrum={dist:.1e}", flush=True)
+            assert r.converged, name
+            assert dist < 1e-10, name
+        if tag == "2x4":
+            for name in ("jacobi", "mlp"):
+                np.testing.assert_allclose(res[name].eigenvalues, spec[:4], rtol=0, atol=1e-10)
 
 
 def test_block_dot_matches_numpy(qk):

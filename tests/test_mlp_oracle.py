@@ -1,6 +1,7 @@
 """Pins of the MLP-preconditioner oracle (CPU only): BF16 rounding against
-torch, backward against finite differences, the pair-loss gradient, and the
-zero-output-layer network reducing to the Jacobi correction."""
+torch, backward against finite differences, the pair-loss gradient, the
+correction terms against a dense solve, and the zero-output-layer network
+giving the damped Neumann correction z - w/2."""
 import numpy as np
 import pytest
 import torch
@@ -28,14 +29,14 @@ def test_exact_backward_matches_finite_differences():
     p = M.init_params(3).astype(np.float64)
     rng = np.random.default_rng(1)
     X = rng.standard_normal((7, M.IN))
-    go = rng.standard_normal(7)
+    go = rng.standard_normal((7, M.OUT))
     g = M.backward(p, X, go, exact=True)
-    f = lambda q: float(go @ M.forward(q, X, exact=True)[0])
+    f = lambda q: float(np.sum(go * M.forward(q, X, exact=True)[0]))
     idx = np.concatenate([rng.choice(np.arange(M.W1, M.B1), 12, replace=False),
                           rng.choice(np.arange(M.B1, M.W2), 6, replace=False),
                           rng.choice(np.arange(M.W2, M.B2), 12, replace=False),
                           rng.choice(np.arange(M.B2, M.W3), 6, replace=False),
-                          rng.choice(np.arange(M.W3, M.B3), 6, replace=False), [M.B3]])
+                          rng.choice(np.arange(M.W3, M.B3), 8, replace=False), [M.B3, M.B3 + 1]])
     h = 1e-6
     for i in idx:
         e = np.zeros_like(p)
@@ -48,49 +49,94 @@ def test_rounded_backward_is_close_to_exact():
     p = M.init_params(5)
     rng = np.random.default_rng(2)
     X = rng.standard_normal((300, M.IN)).astype(np.float32)
-    go = rng.standard_normal(300).astype(np.float32)
+    go = rng.standard_normal((300, M.OUT)).astype(np.float32)
     ge = M.backward(p, M.bf16(X), go, exact=True)
     gq = M.backward(p, X, go)
     # BF16 weights and activations: a few per cent, not more
     assert np.linalg.norm(gq - ge) / np.linalg.norm(ge) < 0.1
 
 
-def test_zero_output_layer_is_jacobi():
+def test_zero_output_layer_is_damped_neumann():
     p = M.init_params(9)
-    p[M.W3:M.B3] = 0.0
-    p[M.B3] = 0.0
+    p[M.W3:] = 0.0
     rng = np.random.default_rng(4)
     X = rng.standard_normal((50, M.IN)).astype(np.float32)
     o, _ = M.forward(p, X)
-    assert np.all(o == 0.0)
+    assert o.shape == (50, 2) and np.all(o == 0.0)
+    z = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    w = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    assert np.array_equal(M.combine(o, z, w), z - 0.5 * w)
+    # the coefficients stay inside their bounds for any output
+    s1, s2 = M.coefficients(np.array([[1e30, -1e30], [-1e30, 1e30]]))
+    assert np.allclose(s1, [1.5, 0.5]) and np.allclose(s2, [-0.75, -0.25])
+
+
+def test_neumann_terms_against_dense_solve():
+    """z - w is the first-order expansion of (H - theta)^-1 r in the
+    off-diagonal part: scaling the off-diagonal by eps leaves an error of
+    order eps^2 (a sign or index slip in w leaves order eps)."""
+    rng = np.random.default_rng(8)
+    n = 30
+    D = np.diag(rng.uniform(1.0, 3.0, n))
+    O = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    O = O + O.conj().T
+    np.fill_diagonal(O, 0.0)
+    theta = -1.0
+    r = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    errs = []
+    for eps in (1e-2, 1e-3):
+        H = D + eps * O
+        z, w = M.neumann_terms(np.diag(H).real, theta, r, lambda x: H @ x)
+        exact = np.linalg.solve(H - theta * np.eye(n), r)
+        errs.append(np.linalg.norm(z - w - exact) / np.linalg.norm(exact))
+        # Jacobi alone is first order
+        assert np.linalg.norm(z - exact) / np.linalg.norm(exact) > 10 * errs[-1]
+    assert errs[1] < errs[0] / 50.0, errs
 
 
 def test_pair_loss_gradient_matches_finite_differences():
     rng = np.random.default_rng(6)
     n = 40
-    diag = rng.uniform(-3, 1, n)
-    theta = -4.0
-    r = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    c = lambda: rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.conj().T
+    H = lambda x: A @ x
+    z, w, u, b = c(), c(), c(), c()
     u /= np.linalg.norm(u)
-    ts = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    ts -= u * np.vdot(u, ts)
-    o = rng.standard_normal(n) * 0.1
-    L, g = M.pair_loss(diag, theta, o, r, ts, u)
-    assert 0.0 <= L <= 1.0
+    theta = -2.5
+    o = rng.standard_normal((n, 2)) * 0.5
+    L, g = M.pair_loss(o, z, w, b, u, theta, H)
+    assert 0.0 <= L <= 1.0 and g.shape == (n, 2)
     h = 1e-7
     for i in range(n):
-        e = np.zeros(n)
-        e[i] = h
-        fd = (M.pair_loss(diag, theta, o + e, r, ts, u)[0] - M.pair_loss(diag, theta, o - e, r, ts, u)[0]) / (2 * h)
-        assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(fd))
-    # the loss vanishes when the preconditioned residual is parallel to ts
-    w = 1.0 / M.jacobi_den(diag, theta)
-    o_star = (ts / r).real - w   # only exact when ts/r is real
-    ts2 = (w + o_star) * r
-    ts2 = ts2 - u * np.vdot(u, ts2)
-    L2, _ = M.pair_loss(diag, theta, o_star, r, ts2, u)
-    assert L2 < 1e-12
+        for k in range(2):
+            e = np.zeros((n, 2))
+            e[i, k] = h
+            fd = (M.pair_loss(o + e, z, w, b, u, theta, H)[0]
+                  - M.pair_loss(o - e, z, w, b, u, theta, H)[0]) / (2 * h)
+            assert abs(fd - g[i, k]) < 1e-6 * max(1.0, abs(fd)), (i, k, fd, g[i, k])
+    P = lambda x: x - u * np.vdot(u, x)
+    K = lambda x: P(A @ x - theta * x)
+    # the loss vanishes when t is the accurate correction up to scale (and a
+    # component along u, which the projection removes) ...
+    t = P(M.combine(o, z, w))
+    L0, _ = M.pair_loss(o, z, w, K(3.0 * t), u, theta, H)
+    assert L0 < 1e-12
+    # ... and is 1 when K t is orthogonal to K t*
+    q = K(t)
+    b1 = c()
+    b1 -= q * np.vdot(q, b1) / np.vdot(q, q)
+    L1, _ = M.pair_loss(o, z, w, b1, u, theta, H)
+    assert abs(L1 - 1.0) < 1e-12
+    # the accurate correction of a Ritz pair gives b = -r (K t* = -r)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3)))
+    ev, S = np.linalg.eigh(Q.conj().T @ A @ Q)
+    uu, th = Q @ S[:, 0], ev[0]
+    r = A @ uu - th * uu
+    Pm = np.eye(n) - np.outer(uu, uu.conj())
+    ts = np.linalg.solve(Pm @ (A - th * np.eye(n)) @ Pm + np.outer(uu, uu.conj()), -r)
+    Kt = Pm @ (A @ ts - th * ts)
+    assert np.linalg.norm(Kt + r) < 1e-10 * np.linalg.norm(r)
 
 
 def test_features_definition():
