@@ -358,6 +358,14 @@ int qkrylov_continued_fraction_block_device_f64(qkrylov_hamiltonian_h h, const v
 double qkrylov_evaluate_spectral_function_f64(const double* alphas, const double* betas, size_t n,
                                               double norm_phi0, double omega, double E0, double eta);
 
+/* Host-only: eigen-decomposition of a small Hermitian matrix (row-major
+ * n x n complex), ascending eigenvalues, vecs_out[k*n + j] = component j of
+ * eigenvector k; the solver of the Davidson projected matrix (reference
+ * davidson.cpp:22-82 uses Jacobi rotations; any converged solver gives the
+ * same eigenvalues).  Householder tridiagonalisation and implicit-shift QL. */
+int qkrylov_hermitian_eigh_f64(int n, const double* a_complex, double* evals_out,
+                               double* vecs_complex_out);
+
 /* FTLM for one beta in FP64 (reference src/solvers/ftlm.cpp:122-170):
  * std::mt19937(42) re-seeded per call, normal(0,1), imaginary part drawn
  * before the real part per element. */

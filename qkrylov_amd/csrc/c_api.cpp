@@ -7,6 +7,7 @@
 #include "qkrylov/c_api.h"
 
 #include "qk_hamiltonian.hpp"
+#include "qk_hostla.hpp"
 #include "qk_learned.hpp"
 #include "qk_rng.hpp"
 #include "qk_solvers.hpp"
@@ -764,6 +765,19 @@ double qkrylov_evaluate_spectral_function_f64(const double* alphas, const double
                                               double norm_phi0, double omega, double E0, double eta) {
     if (!alphas || !betas || n == 0) return 0.0;
     return spectral_function(alphas, betas, n, norm_phi0, omega, E0, eta);
+}
+
+int qkrylov_hermitian_eigh_f64(int n, const double* a, double* evals, double* vecs) {
+    if (n <= 0 || !a || !evals) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        std::vector<cplx> A(size_t(n) * n);
+        std::memcpy(A.data(), a, sizeof(cplx) * A.size());
+        std::vector<double> ev;
+        std::vector<cplx> S;
+        hermitian_eigh(A, n, ev, S);
+        std::memcpy(evals, ev.data(), sizeof(double) * n);
+        if (vecs) std::memcpy(vecs, S.data(), sizeof(cplx) * S.size());
+    });
 }
 
 int qkrylov_ftlm(qkrylov_hamiltonian_h h, float beta, int n_random, int n_steps,

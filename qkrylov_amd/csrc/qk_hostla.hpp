@@ -1,10 +1,11 @@
 // Small dense eigenproblems solved on the host in FP64: the Lanczos
 // tridiagonal (implicit-shift QL with a Wilkinson-type shift and 1e-14
-// deflation, at most 1000 sweeps) and the Davidson projected matrix (cyclic
-// largest-element Jacobi, at most 1000 rotations, stop below 1e-15).  The
-// sweep and rotation rules are those of the reference (src/solvers/
-// lanczos.cpp:25-98, ftlm.cpp:61-119, davidson.cpp:22-82) so that the
-// eigenvalues, and hence the Lanczos stopping iteration, are the same.
+// deflation, at most 1000 sweeps; the sweep rules of the reference
+// src/solvers/lanczos.cpp:25-98 and ftlm.cpp:61-119, so that the
+// eigenvalues, and hence the Lanczos stopping iteration, are the same) and
+// Hermitian matrices (the Davidson projected matrix, davidson.cpp:22-82, and
+// the block-Lanczos matrix): cyclic Jacobi run to convergence, or Householder
+// tridiagonalisation followed by the same QL.
 #pragma once
 
 #include "qk_common.hpp"
@@ -27,5 +28,9 @@ double tridiag_ground(const std::vector<double>& a, const std::vector<double>& b
 // eigenvalues; vecs[k*n + j] = component j of eigenvector k.
 void hermitian_jacobi(const std::vector<cplx>& A, int n, std::vector<double>& evals,
                       std::vector<cplx>& vecs);
+// Same contract, by Householder tridiagonalisation and tridiag_ql: O(n^3)
+// with a small constant, for the solvers' per-iteration projected matrices.
+void hermitian_eigh(const std::vector<cplx>& A, int n, std::vector<double>& evals,
+                    std::vector<cplx>& vecs);
 
 }  // namespace qk

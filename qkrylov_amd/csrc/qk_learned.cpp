@@ -250,30 +250,45 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
 // ---------------------------------------------------------------------------
 MlpPreconditioner::MlpPreconditioner(const Mlp& net, const Hamiltonian& H) : net_(net) {
     states_.build(H, H.stream);
-    const u64 n1 = std::max<u64>(states_.n, 1);
-    X_.resize(sizeof(u16) * n1 * kMlpIn);
-    o_.resize(sizeof(float) * n1 * kMlpOut);
-    z_.resize(sizeof(cplx) * n1);
-    w_.resize(sizeof(cplx) * n1);
-    c_.resize(sizeof(cplx));
 }
 
-void MlpPreconditioner::apply(const Hamiltonian& H, const cplx* r, const cplx*, double lambda,
+void MlpPreconditioner::apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda,
                               const cplx* u, cplx* t, hipStream_t st) {
+    cplx* tp[1] = {t};
+    const cplx* up[1] = {u};
+    apply_many(H, 1, &r, diag, &lambda, u ? up : nullptr, tp, st);
+}
+
+void MlpPreconditioner::apply_many(const Hamiltonian& H, int nt, const cplx* const* r, const cplx*,
+                                   const double* lambda, const cplx* const* u, cplx* const* t,
+                                   hipStream_t st) {
     const u64 n = states_.n;
-    if (!n) return;
-    mlp_jacobi(states_, lambda, r, z_.get<cplx>(), st);
-    H.apply(z_.get<cplx>(), t, 1, n, st);   // t holds H z until the combine
-    mlp_neumann(states_, lambda, t, z_.get<cplx>(), w_.get<cplx>(), st);
-    mlp_features(states_, lambda, X_.get<u16>(), st);
-    net_.forward(X_.get<u16>(), n, o_.get<float>(), nullptr, nullptr, st);
-    mlp_combine(states_, o_.get<float>(), z_.get<cplx>(), w_.get<cplx>(), t, st);
-    if (u) {
-        // t approximates (H - lambda)^-1 r, which is u itself for an exact Ritz
-        // pair: remove the u component here, in one pass, before Davidson
-        // orthogonalises t against its basis
-        vec::dot(u, t, n, c_.get<cplx>(), red_, st);
-        mlp_project(u, c_.get<cplx>(), t, n, st);
+    if (!n || nt <= 0) return;
+    const u64 tot = n * u64(nt);
+    X_.resize(sizeof(u16) * tot * kMlpIn);
+    o_.resize(sizeof(float) * tot * kMlpOut);
+    z_.resize(sizeof(cplx) * tot);
+    w_.resize(sizeof(cplx) * tot);
+    c_.resize(sizeof(cplx) * nt);
+    cplx* z = z_.get<cplx>();
+    cplx* w = w_.get<cplx>();
+    for (int j = 0; j < nt; ++j) mlp_jacobi(states_, lambda[j], r[j], z + u64(j) * n, st);
+    H.apply(z, w, nt, n, st);   // w holds H z until the Neumann term overwrites it
+    for (int j = 0; j < nt; ++j) {
+        mlp_neumann(states_, lambda[j], w + u64(j) * n, z + u64(j) * n, w + u64(j) * n, st);
+        mlp_features(states_, lambda[j], X_.get<u16>() + u64(j) * n * kMlpIn, st);
+    }
+    net_.forward(X_.get<u16>(), tot, o_.get<float>(), nullptr, nullptr, st);
+    for (int j = 0; j < nt; ++j) {
+        mlp_combine(states_, o_.get<float>() + u64(j) * n * kMlpOut, z + u64(j) * n, w + u64(j) * n,
+                    t[j], st);
+        if (u && u[j]) {
+            // t approximates (H - lambda)^-1 r, which is u itself for an exact
+            // Ritz pair: remove the u component here, before Davidson
+            // orthogonalises t against its basis
+            vec::dot(u[j], t[j], n, c_.get<cplx>() + j, red_, st);
+            mlp_project(u[j], c_.get<cplx>() + j, t[j], n, st);
+        }
     }
 }
 

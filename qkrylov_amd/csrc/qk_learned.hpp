@@ -41,8 +41,14 @@ public:
     MlpPreconditioner(const Mlp& net, const Hamiltonian& H);
     void apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda, const cplx* u,
                cplx* t, hipStream_t st) override;
+    // all corrections of an iteration with one batched apply of H and one
+    // forward pass over every (correction, state) row
+    void apply_many(const Hamiltonian& H, int nt, const cplx* const* r, const cplx* diag,
+                    const double* lambda, const cplx* const* u, cplx* const* t,
+                    hipStream_t st) override;
     bool wants_ritz() const override { return true; }
-    // the two correction terms of the last apply (device)
+    // the two correction terms of the last apply (device; the first
+    // correction's after apply_many)
     const cplx* z() const { return z_.get<cplx>(); }
     const cplx* w() const { return w_.get<cplx>(); }
 

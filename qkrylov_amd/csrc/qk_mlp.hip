@@ -28,8 +28,13 @@ typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kIn = kMlpIn, kHid = kMlpHid;
+constexpr int kIn1 = kIn + 8, kHid1 = kHid + 8;   // padded LDS row lengths
 constexpr int kRows = 64;          // rows per workgroup tile (4 waves x 16)
 constexpr int kThreads = 256;
+// forward: 8 waves share one LDS copy of the weights (about 80 KB per
+// workgroup, two workgroups per CU, four waves per SIMD)
+constexpr int kFwdRows = 128;
+constexpr int kFwdThreads = 512;
 
 __device__ __forceinline__ u16 d_f2bf(float f) {
     u32 u = __float_as_uint(f);
@@ -50,31 +55,38 @@ __device__ __forceinline__ u16x8 ld8(const u16* p) { return *reinterpret_cast<co
 // Activations are rounded to BF16 (they feed the next MFMA); h2 is rounded
 // too, so backward sees exactly what forward used.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kFwdThreads)
 k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1t,
               const u16* __restrict__ w2t, const u16* __restrict__ w3b,
               const float* __restrict__ prm, float* __restrict__ out, u16* __restrict__ h1s,
               u16* __restrict__ h2s) {
-    __shared__ __align__(16) u16 s_w1t[kHid * kIn];
-    __shared__ __align__(16) u16 s_w2t[kHid * kHid];
-    __shared__ __align__(16) u16 s_h1[kRows * kHid];
+    // LDS rows padded by 16 bytes: the 16 lanes of an operand read hit 16
+    // distinct 4-bank groups instead of one bank (row strides of 64 and 256
+    // bytes map every row to the same banks)
+    __shared__ __align__(16) u16 s_w1t[kHid * kIn1];
+    __shared__ __align__(16) u16 s_w2t[kHid * kHid1];
+    __shared__ __align__(16) u16 s_h1[kFwdRows * kHid1];
     __shared__ float s_b1[kHid], s_b2[kHid], s_w3[kHid * kMlpOut];
-    for (int i = threadIdx.x; i < kHid * kIn / 8; i += kThreads)
-        reinterpret_cast<u16x8*>(s_w1t)[i] = reinterpret_cast<const u16x8*>(w1t)[i];
-    for (int i = threadIdx.x; i < kHid * kHid / 8; i += kThreads)
-        reinterpret_cast<u16x8*>(s_w2t)[i] = reinterpret_cast<const u16x8*>(w2t)[i];
-    for (int i = threadIdx.x; i < kHid; i += kThreads) {
+    for (int i = threadIdx.x; i < kHid * kIn / 8; i += kFwdThreads) {
+        const int r = i / (kIn / 8), c8 = i % (kIn / 8);
+        *reinterpret_cast<u16x8*>(s_w1t + r * kIn1 + c8 * 8) = reinterpret_cast<const u16x8*>(w1t)[i];
+    }
+    for (int i = threadIdx.x; i < kHid * kHid / 8; i += kFwdThreads) {
+        const int r = i / (kHid / 8), c8 = i % (kHid / 8);
+        *reinterpret_cast<u16x8*>(s_w2t + r * kHid1 + c8 * 8) = reinterpret_cast<const u16x8*>(w2t)[i];
+    }
+    for (int i = threadIdx.x; i < kHid; i += kFwdThreads) {
         s_b1[i] = prm[kMlpB1 + i];
         s_b2[i] = prm[kMlpB2 + i];
     }
-    for (int i = threadIdx.x; i < kHid * kMlpOut; i += kThreads) s_w3[i] = d_bf2f(w3b[i]);
+    for (int i = threadIdx.x; i < kHid * kMlpOut; i += kFwdThreads) s_w3[i] = d_bf2f(w3b[i]);
     const float b30 = prm[kMlpB3], b31 = prm[kMlpB3 + 1];
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
-    u16* h1w = s_h1 + wave * 16 * kHid;
-    for (u64 tile = blockIdx.x; tile * kRows < n; tile += gridDim.x) {
-        const u64 m0 = tile * kRows + u64(wave) * 16;
+    u16* h1w = s_h1 + wave * 16 * kHid1;
+    for (u64 tile = blockIdx.x; tile * kFwdRows < n; tile += gridDim.x) {
+        const u64 m0 = tile * kFwdRows + u64(wave) * 16;
         // layer 1: [16 x 32] x [32 x 128]
         const u64 ra = m0 + c;
         u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -83,14 +95,14 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) {
             acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
-            acc[nb] = mfma(a, ld8(s_w1t + (nb * 16 + c) * kIn + 8 * g), acc[nb]);
+            acc[nb] = mfma(a, ld8(s_w1t + (nb * 16 + c) * kIn1 + 8 * g), acc[nb]);
         }
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) {
             const int col = nb * 16 + c;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                h1w[(4 * g + i) * kHid + col] = d_f2bf(fmaxf(acc[nb][i] + s_b1[col], 0.f));
+                h1w[(4 * g + i) * kHid1 + col] = d_f2bf(fmaxf(acc[nb][i] + s_b1[col], 0.f));
         }
         __syncthreads();
         if (h1s) {
@@ -98,7 +110,7 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
                 const int rr = q / (kHid / 8), cc = (q % (kHid / 8)) * 8;
                 if (m0 + rr < n)
                     *reinterpret_cast<u16x8*>(h1s + (m0 + rr) * kHid + cc) =
-                        *reinterpret_cast<const u16x8*>(h1w + rr * kHid + cc);
+                        *reinterpret_cast<const u16x8*>(h1w + rr * kHid1 + cc);
             }
         }
         // layer 2: [16 x 128] x [128 x 128]
@@ -106,10 +118,10 @@ k_mlp_forward(const u16* __restrict__ X, const u64 n, const u16* __restrict__ w1
         for (int nb = 0; nb < 8; ++nb) acc[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kb = 0; kb < kHid / 32; ++kb) {
-            const u16x8 a2 = ld8(h1w + c * kHid + kb * 32 + 8 * g);
+            const u16x8 a2 = ld8(h1w + c * kHid1 + kb * 32 + 8 * g);
 #pragma unroll
             for (int nb = 0; nb < 8; ++nb)
-                acc[nb] = mfma(a2, ld8(s_w2t + (nb * 16 + c) * kHid + kb * 32 + 8 * g), acc[nb]);
+                acc[nb] = mfma(a2, ld8(s_w2t + (nb * 16 + c) * kHid1 + kb * 32 + 8 * g), acc[nb]);
         }
         // layer 3 (two outputs) and the saved h2
         float part0[4] = {0.f, 0.f, 0.f, 0.f}, part1[4] = {0.f, 0.f, 0.f, 0.f};
@@ -157,12 +169,14 @@ k_mlp_backward_act(const u16* __restrict__ h1s, const u16* __restrict__ h2s,
                    const float* __restrict__ go, const u64 n, const u16* __restrict__ w2k,
                    const u16* __restrict__ w3b, u16* __restrict__ gz2s, u16* __restrict__ gz1s,
                    float* __restrict__ part3) {
-    __shared__ __align__(16) u16 s_w2k[kHid * kHid];   // W2[k][n]: B operand of gz2 W2^T
+    __shared__ __align__(16) u16 s_w2k[kHid * kHid1];   // W2[k][n]: B operand of gz2 W2^T
     __shared__ float s_w3[kHid * kMlpOut];
     __shared__ float s_gw3[4][kHid * kMlpOut];
     __shared__ float s_gb3[4][kMlpOut];
-    for (int i = threadIdx.x; i < kHid * kHid / 8; i += kThreads)
-        reinterpret_cast<u16x8*>(s_w2k)[i] = reinterpret_cast<const u16x8*>(w2k)[i];
+    for (int i = threadIdx.x; i < kHid * kHid / 8; i += kThreads) {
+        const int r = i / (kHid / 8), c8 = i % (kHid / 8);
+        *reinterpret_cast<u16x8*>(s_w2k + r * kHid1 + c8 * 8) = reinterpret_cast<const u16x8*>(w2k)[i];
+    }
     for (int i = threadIdx.x; i < kHid * kMlpOut; i += kThreads) s_w3[i] = d_bf2f(w3b[i]);
     __syncthreads();
 
@@ -208,7 +222,7 @@ k_mlp_backward_act(const u16* __restrict__ h1s, const u16* __restrict__ h2s,
             // gh1[m][k] = sum_n gz2[m][n] W2[k][n]
 #pragma unroll
             for (int nb = 0; nb < 8; ++nb)
-                acc[nb] = mfma(a, ld8(s_w2k + (nb * 16 + c) * kHid + k0), acc[nb]);
+                acc[nb] = mfma(a, ld8(s_w2k + (nb * 16 + c) * kHid1 + k0), acc[nb]);
         }
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) {
@@ -410,9 +424,9 @@ __global__ void k_mlp_jacobi(const double* __restrict__ diag, const u64 n, const
     z[i] = cplx{j * r[i].re, j * r[i].im};
 }
 
+// w may alias hz (element i reads hz[i] before writing w[i])
 __global__ void k_mlp_neumann(const double* __restrict__ diag, const u64 n, const double theta,
-                              const cplx* __restrict__ hz, const cplx* __restrict__ z,
-                              cplx* __restrict__ w) {
+                              const cplx* hz, const cplx* __restrict__ z, cplx* w) {
     const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double d = diag[i];
@@ -566,7 +580,7 @@ void Mlp::cast() {
 
 void Mlp::forward(const u16* X, u64 n, float* o, u16* h1, u16* h2, hipStream_t st) const {
     if (!n) return;
-    hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_grid(n, kRows)), dim3(kThreads), 0, st, X, n,
+    hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_grid(n, kFwdRows)), dim3(kFwdThreads), 0, st, X, n,
                        w1t.get<u16>(), w2t.get<u16>(), w3b.get<u16>(), params.get<float>(), o, h1, h2);
     QK_HIP(hipGetLastError());
 }

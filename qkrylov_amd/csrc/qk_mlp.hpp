@@ -117,7 +117,7 @@ struct MlpStates {
 void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st);
 // z = J r (device; J the reference's clamped Jacobi factor for theta)
 void mlp_jacobi(const MlpStates& s, double theta, const cplx* r, cplx* z, hipStream_t st);
-// w = J (hz - diag z), hz = H z  (device)
+// w = J (hz - diag z), hz = H z  (device; w may be hz)
 void mlp_neumann(const MlpStates& s, double theta, const cplx* hz, const cplx* z, cplx* w,
                  hipStream_t st);
 // t = s1(o) z + s2(o) w  (device)

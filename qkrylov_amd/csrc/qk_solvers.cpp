@@ -102,13 +102,16 @@ void parallel_for(int n, F&& f) {
 // Two classical Gram-Schmidt passes of w against k vectors (pointer array
 // on the device).  Replaces the reference's single modified Gram-Schmidt
 // sweep (lanczos.cpp:149-151); both leave w orthogonal to the same span.
-void cgs2(const DevBuf& ptrs, int k, cplx* w, u64 n, DevBuf& h, DevBuf& red, hipStream_t st) {
+void cgs2(const cplx* const* ptrs, int k, cplx* w, u64 n, DevBuf& h, DevBuf& red, hipStream_t st) {
     if (k <= 0) return;
     h.resize(sizeof(cplx) * k);
     for (int pass = 0; pass < 2; ++pass) {
-        vec::mdot(ptrs.get<const cplx*>(), k, w, n, h.get<cplx>(), red, st);
-        vec::msub(ptrs.get<const cplx*>(), k, h.get<cplx>(), w, n, st);
+        vec::mdot(ptrs, k, w, n, h.get<cplx>(), red, st);
+        vec::msub(ptrs, k, h.get<cplx>(), w, n, st);
     }
+}
+void cgs2(const DevBuf& ptrs, int k, cplx* w, u64 n, DevBuf& h, DevBuf& red, hipStream_t st) {
+    cgs2(ptrs.get<const cplx*>(), k, w, n, h, red, st);
 }
 
 }  // namespace
@@ -412,7 +415,7 @@ BlockLanczosResult block_lanczos_lowest(const Hamiltonian& H, int b, int nev, in
                             T[size_t(q * b + k) * m + (q + 1) * b + i] = cconj(x);
                         }
             }
-            hermitian_jacobi(T, m, ev, S);
+            hermitian_eigh(T, m, ev, S);
             std::vector<double> e(ev.begin(), ev.begin() + std::min(nev, m));
             if (!e_old.empty() && e.size() == e_old.size()) {
                 double dmax = 0.0;
@@ -448,7 +451,7 @@ BlockLanczosResult block_lanczos_lowest(const Hamiltonian& H, int b, int nev, in
                         T[size_t(q * b + k) * m + (q + 1) * b + i] = cconj(x);
                     }
         }
-        hermitian_jacobi(T, m, ev, S);
+        hermitian_eigh(T, m, ev, S);
         n = m;
     }
     const int ne = std::min(nev, n);
@@ -478,9 +481,37 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
     hipStream_t st = H.stream;
     n_eig = int(std::min<i64>(n_eig, i64(d)));
     max_subspace = std::max(max_subspace, 2 * n_eig);
-    const int cap = max_subspace + std::max(n_eig, 1) + 2;
+    // the basis never exceeds max_subspace vectors: a step that would pass it
+    // restarts to n_eig first and then adds at most n_eig (:196-223)
+    const int cap = max_subspace + 1;
+    const int nb = std::max(n_eig, 1);
 
-    DevBuf diag(sizeof(cplx) * d), red, sc(64 * sizeof(double)), h;
+    // All device storage is allocated once: the basis V and H V as slabs of
+    // cap vectors with fixed pointer tables, and n_eig-vector slabs for the
+    // residuals, Ritz vectors, corrections and the restart.
+    DevBuf diag(sizeof(cplx) * d), red, sc(64 * sizeof(double)), h(sizeof(cplx) * cap * cap),
+        slabV(sizeof(cplx) * d * cap), slabHV(sizeof(cplx) * d * cap), Rb(sizeof(cplx) * d * nb),
+        Ub(sizeof(cplx) * d * nb), Tb(sizeof(cplx) * d * nb), Nb(sizeof(cplx) * d * nb * 2),
+        nrm(sizeof(double) * nb), coef(sizeof(cplx) * size_t(nb) * 2 * cap);
+    auto vp = [&](int i) { return slabV.get<cplx>() + u64(i) * d; };
+    auto hvp = [&](int i) { return slabHV.get<cplx>() + u64(i) * d; };
+    DevBuf pV(sizeof(void*) * cap), pHV(sizeof(void*) * cap), pI(sizeof(void*) * 2 * cap),
+        pT(sizeof(void*) * nb);
+    {
+        std::vector<const void*> a(cap), b(cap), c(2 * size_t(cap)), t(nb);
+        for (int j = 0; j < nb; ++j) t[j] = Tb.get<cplx>() + u64(j) * d;
+        QK_HIP(hipMemcpyAsync(pT.get(), t.data(), sizeof(void*) * nb, hipMemcpyHostToDevice, st));
+        for (int i = 0; i < cap; ++i) {
+            a[i] = vp(i);
+            b[i] = hvp(i);
+            c[2 * size_t(i)] = hvp(i);   // residual operands interleaved: HV_i, V_i
+            c[2 * size_t(i) + 1] = vp(i);
+        }
+        QK_HIP(hipMemcpyAsync(pV.get(), a.data(), sizeof(void*) * cap, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(pHV.get(), b.data(), sizeof(void*) * cap, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(pI.get(), c.data(), sizeof(void*) * c.size(), hipMemcpyHostToDevice, st));
+    }
+
     H.diagonal(diag.get<cplx>(), st);
     std::vector<cplx> dh(d);
     QK_HIP(hipMemcpyAsync(dh.data(), diag.get(), sizeof(cplx) * d, hipMemcpyDeviceToHost, st));
@@ -489,56 +520,36 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
     std::iota(idx.begin(), idx.end(), u64(0));
     std::sort(idx.begin(), idx.end(), [&](u64 i, u64 j) { return dh[i].re < dh[j].re; });
 
-    std::vector<DevBuf> V, HV;
-    DevBuf pV(sizeof(void*) * cap * 2), pHV(sizeof(void*) * cap), pVHV(sizeof(void*) * cap * 2);
-    auto refresh_ptrs = [&]() {
-        std::vector<const void*> a, b, ab;
-        for (auto& x : V) a.push_back(x.get());
-        for (auto& x : HV) b.push_back(x.get());
-        ab = b;
-        ab.insert(ab.end(), a.begin(), a.end());
-        if (!a.empty())
-            QK_HIP(hipMemcpyAsync(pV.get(), a.data(), sizeof(void*) * a.size(), hipMemcpyHostToDevice, st));
-        if (!b.empty())
-            QK_HIP(hipMemcpyAsync(pHV.get(), b.data(), sizeof(void*) * b.size(), hipMemcpyHostToDevice, st));
-        if (!ab.empty())
-            QK_HIP(hipMemcpyAsync(pVHV.get(), ab.data(), sizeof(void*) * ab.size(), hipMemcpyHostToDevice, st));
-        QK_HIP(hipStreamSynchronize(st));
-    };
-
     // Start vectors: unit vectors on the n_eig smallest Re(diag) (:111-139).
     // Distinct unit vectors are exactly orthonormal, so the reference's
     // orthogonalisation leaves them unchanged and its random fallback never
     // triggers.
-    for (int i = 0; i < n_eig; ++i) {
-        V.emplace_back(sizeof(cplx) * d);
-        QK_HIP(hipMemsetAsync(V.back().get(), 0, sizeof(cplx) * d, st));
-        const cplx one{1.0, 0.0};
-        QK_HIP(hipMemcpyAsync(V.back().get<cplx>() + idx[i], &one, sizeof(cplx),
-                              hipMemcpyHostToDevice, st));
-        HV.emplace_back(sizeof(cplx) * d);
-        H.apply(V.back().get<cplx>(), HV.back().get<cplx>(), 1, d, st);
-        QK_HIP(hipStreamSynchronize(st));
-    }
-    refresh_ptrs();
+    QK_HIP(hipMemsetAsync(vp(0), 0, sizeof(cplx) * d * n_eig, st));
+    const cplx one{1.0, 0.0};
+    for (int i = 0; i < n_eig; ++i)
+        QK_HIP(hipMemcpyAsync(vp(i) + idx[i], &one, sizeof(cplx), hipMemcpyHostToDevice, st));
+    H.apply(vp(0), hvp(0), n_eig, d, st);
+    QK_HIP(hipStreamSynchronize(st));
+    int m = n_eig;
+
+    // the first ne columns of S (Ritz coefficients), uploaded to coef
+    auto upload_ritz = [&](const std::vector<cplx>& S, int mm, int ne) {
+        QK_HIP(hipMemcpyAsync(coef.get(), S.data(), sizeof(cplx) * size_t(ne) * mm, hipMemcpyHostToDevice, st));
+    };
 
     std::vector<cplx> Hs;   // projected matrix, row-major m*m
     int valid_rows = 0;
-    std::vector<DevBuf> R;
-    DevBuf coef;
     max_iter = std::max(max_iter, 1);
+    static const bool use_block_dot = [] {
+        const char* e = std::getenv("QKRYLOV_DAVIDSON_BLOCK_DOT");
+        return !(e && std::string(e) == "0");
+    }();
     for (int iter = 0; iter < max_iter; ++iter) {
-        const int m = int(V.size());
         // H_sub[i][j] = <V_i, HV_j>, j <= i (:143-150); new rows only.
         std::vector<cplx> Hn(size_t(m) * m, cplx{0.0, 0.0});
         for (int i = 0; i < std::min(valid_rows, m); ++i)
             for (int j = 0; j < std::min(valid_rows, m); ++j) Hn[size_t(i) * m + j] = Hs[size_t(i) * valid_rows + j];
-        h.resize(sizeof(cplx) * m * std::max(1, m - valid_rows));
         const int nnew = m - valid_rows;
-        static const bool use_block_dot = [] {
-            const char* e = std::getenv("QKRYLOV_DAVIDSON_BLOCK_DOT");
-            return !(e && std::string(e) == "0");
-        }();
         if (use_block_dot && nnew > 0 && m <= 64 && nnew <= 16) {
             // one FP64-MFMA projection G = HV^H V_new for all new rows
             vec::block_dot(pHV.get<const cplx*>(), m, pV.get<const cplx*>() + valid_rows, nnew, d,
@@ -554,7 +565,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
                 }
         } else {
             for (int i = valid_rows; i < m; ++i) {
-                vec::mdot(pHV.get<const cplx*>(), i + 1, V[i].get<cplx>(), d, h.get<cplx>(), red, st);
+                vec::mdot(pHV.get<const cplx*>(), i + 1, vp(i), d, h.get<cplx>(), red, st);
                 std::vector<cplx> hv(i + 1);
                 QK_HIP(hipMemcpyAsync(hv.data(), h.get(), sizeof(cplx) * (i + 1), hipMemcpyDeviceToHost, st));
                 QK_HIP(hipStreamSynchronize(st));
@@ -565,31 +576,35 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
                 }
             }
         }
-        Hs = Hn;
+        Hs = std::move(Hn);
         valid_rows = m;
 
         std::vector<double> ev;
         std::vector<cplx> S;
-        hermitian_jacobi(Hs, m, ev, S);
+        hermitian_eigh(Hs, m, ev, S);
 
         // residuals r_k = sum_i s_ki (HV_i - lambda_k V_i) (:161-165)
         const int ne = std::min(n_eig, m);
-        std::vector<cplx> C(size_t(ne) * 2 * m);
-        for (int k = 0; k < ne; ++k)
-            for (int i = 0; i < m; ++i) {
-                const cplx s = S[size_t(k) * m + i];
-                C[size_t(k) * 2 * m + i] = s;
-                C[size_t(k) * 2 * m + m + i] = cplx{-ev[k] * s.re, -ev[k] * s.im};
-            }
-        coef.resize(sizeof(cplx) * C.size());
-        QK_HIP(hipMemcpyAsync(coef.get(), C.data(), sizeof(cplx) * C.size(), hipMemcpyHostToDevice, st));
-        DevBuf Rb(sizeof(cplx) * d * std::max(ne, 1));
-        vec::lincomb_many(pVHV.get<const cplx*>(), 2 * m, coef.get<cplx>(), ne, Rb.get<cplx>(), d, d, st);
+        {
+            std::vector<cplx> C(size_t(ne) * 2 * m);
+            for (int k = 0; k < ne; ++k)
+                for (int i = 0; i < m; ++i) {
+                    const cplx s = S[size_t(k) * m + i];
+                    C[size_t(k) * 2 * m + 2 * i] = s;
+                    C[size_t(k) * 2 * m + 2 * i + 1] = cplx{-ev[k] * s.re, -ev[k] * s.im};
+                }
+            QK_HIP(hipMemcpyAsync(coef.get(), C.data(), sizeof(cplx) * C.size(), hipMemcpyHostToDevice, st));
+            vec::lincomb_many(pI.get<const cplx*>(), 2 * m, coef.get<cplx>(), ne, Rb.get<cplx>(), d, d, st);
+        }
+        for (int k = 0; k < ne; ++k) vec::norm2(Rb.get<cplx>() + u64(k) * d, d, nrm.get<double>() + k, red, st);
+        std::vector<double> rn(ne);
+        QK_HIP(hipMemcpyAsync(rn.data(), nrm.get(), sizeof(double) * ne, hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
         res.residuals.assign(ne, 0.0);
         bool all_conv = true;
         std::vector<int> need;
         for (int k = 0; k < ne; ++k) {
-            res.residuals[k] = dev_norm(Rb.get<cplx>() + u64(k) * d, d, red, sc, st);
+            res.residuals[k] = std::sqrt(rn[k]);
             if (res.residuals[k] > tol) {
                 all_conv = false;
                 need.push_back(k);
@@ -600,16 +615,11 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
             res.converged = all_conv;
             for (int k = 0; k < ne; ++k) res.eigenvalues.push_back(ev[k]);
             if (want_vectors && ne > 0) {
-                std::vector<cplx> Sv(size_t(ne) * m);
-                for (int k = 0; k < ne; ++k)
-                    for (int i = 0; i < m; ++i) Sv[size_t(k) * m + i] = S[size_t(k) * m + i];
-                coef.resize(sizeof(cplx) * Sv.size());
-                QK_HIP(hipMemcpyAsync(coef.get(), Sv.data(), sizeof(cplx) * Sv.size(), hipMemcpyHostToDevice, st));
-                DevBuf E(sizeof(cplx) * d * ne);
-                vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, E.get<cplx>(), d, d, st);
+                upload_ritz(S, m, ne);
+                vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, Ub.get<cplx>(), d, d, st);
                 res.eigenvectors.assign(ne, std::vector<cplx>(d));
                 for (int k = 0; k < ne; ++k)
-                    QK_HIP(hipMemcpyAsync(res.eigenvectors[k].data(), E.get<cplx>() + u64(k) * d,
+                    QK_HIP(hipMemcpyAsync(res.eigenvectors[k].data(), Ub.get<cplx>() + u64(k) * d,
                                           sizeof(cplx) * d, hipMemcpyDeviceToHost, st));
                 QK_HIP(hipStreamSynchronize(st));
             }
@@ -617,62 +627,70 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         }
 
         // corrections t = M(r) (:166-177)
-        DevBuf Ub;
-        if (pre && pre->wants_ritz() && ne > 0) {
-            std::vector<cplx> Sv(size_t(ne) * m);
-            for (int k = 0; k < ne; ++k)
-                for (int i = 0; i < m; ++i) Sv[size_t(k) * m + i] = S[size_t(k) * m + i];
-            coef.resize(sizeof(cplx) * Sv.size());
-            QK_HIP(hipMemcpyAsync(coef.get(), Sv.data(), sizeof(cplx) * Sv.size(), hipMemcpyHostToDevice, st));
-            Ub.resize(sizeof(cplx) * d * ne);
+        const bool ritz = pre && pre->wants_ritz() && ne > 0;
+        if (ritz) {
+            upload_ritz(S, m, ne);
             vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, Ub.get<cplx>(), d, d, st);
         }
-        std::vector<DevBuf> T;
-        for (int k : need) {
-            T.emplace_back(sizeof(cplx) * d);
-            const cplx* rk = Rb.get<cplx>() + u64(k) * d;
-            const cplx* uk = Ub.get() ? Ub.get<cplx>() + u64(k) * d : nullptr;
-            if (pre) pre->apply(H, rk, diag.get<cplx>(), ev[k], uk, T.back().get<cplx>(), st);
-            else vec::jacobi_precond(rk, diag.get<cplx>(), ev[k], T.back().get<cplx>(), d, st);
+        const int nt = int(need.size());
+        if (pre) {
+            std::vector<const cplx*> rp(nt), up(nt);
+            std::vector<cplx*> tp(nt);
+            std::vector<double> lam(nt);
+            for (int j = 0; j < nt; ++j) {
+                rp[j] = Rb.get<cplx>() + u64(need[j]) * d;
+                up[j] = ritz ? Ub.get<cplx>() + u64(need[j]) * d : nullptr;
+                tp[j] = Tb.get<cplx>() + u64(j) * d;
+                lam[j] = ev[need[j]];
+            }
+            pre->apply_many(H, nt, rp.data(), diag.get<cplx>(), lam.data(), ritz ? up.data() : nullptr,
+                            tp.data(), st);
+        } else {
+            for (int j = 0; j < nt; ++j)
+                vec::jacobi_precond(Rb.get<cplx>() + u64(need[j]) * d, diag.get<cplx>(), ev[need[j]],
+                                    Tb.get<cplx>() + u64(j) * d, d, st);
         }
 
         // restart to the n_eig Ritz pairs (:196-211)
-        if (V.size() + T.size() > size_t(max_subspace)) {
-            std::vector<cplx> Sv(size_t(ne) * m);
-            for (int k = 0; k < ne; ++k)
-                for (int i = 0; i < m; ++i) Sv[size_t(k) * m + i] = S[size_t(k) * m + i];
-            coef.resize(sizeof(cplx) * Sv.size());
-            QK_HIP(hipMemcpyAsync(coef.get(), Sv.data(), sizeof(cplx) * Sv.size(), hipMemcpyHostToDevice, st));
-            DevBuf nv(sizeof(cplx) * d * ne), nhv(sizeof(cplx) * d * ne);
-            vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, nv.get<cplx>(), d, d, st);
-            vec::lincomb_many(pHV.get<const cplx*>(), m, coef.get<cplx>(), ne, nhv.get<cplx>(), d, d, st);
-            QK_HIP(hipStreamSynchronize(st));
-            std::vector<DevBuf> V2, HV2;
-            for (int k = 0; k < ne; ++k) {
-                V2.emplace_back(sizeof(cplx) * d);
-                HV2.emplace_back(sizeof(cplx) * d);
-                QK_HIP(hipMemcpyAsync(V2.back().get(), nv.get<cplx>() + u64(k) * d, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
-                QK_HIP(hipMemcpyAsync(HV2.back().get(), nhv.get<cplx>() + u64(k) * d, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
-            }
-            QK_HIP(hipStreamSynchronize(st));
-            V = std::move(V2);
-            HV = std::move(HV2);
+        if (m + nt > max_subspace) {
+            upload_ritz(S, m, ne);
+            cplx* nv = Nb.get<cplx>();
+            cplx* nhv = nv + u64(ne) * d;
+            vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, nv, d, d, st);
+            vec::lincomb_many(pHV.get<const cplx*>(), m, coef.get<cplx>(), ne, nhv, d, d, st);
+            QK_HIP(hipMemcpyAsync(vp(0), nv, sizeof(cplx) * d * ne, hipMemcpyDeviceToDevice, st));
+            QK_HIP(hipMemcpyAsync(hvp(0), nhv, sizeof(cplx) * d * ne, hipMemcpyDeviceToDevice, st));
+            m = ne;
             valid_rows = 0;
-            refresh_ptrs();
         }
 
-        // orthogonalise and expand (:213-223)
-        for (auto& t : T) {
-            cgs2(pV, int(V.size()), t.get<cplx>(), d, h, red, st);
-            const double nrm = dev_norm(t.get<cplx>(), d, red, sc, st);
-            if (nrm > 1e-10) {
-                vec::scale(t.get<cplx>(), d, cplx{1.0 / nrm, 0.0}, st);
-                HV.emplace_back(sizeof(cplx) * d);
-                H.apply(t.get<cplx>(), HV.back().get<cplx>(), 1, d, st);
-                V.push_back(std::move(t));
-                refresh_ptrs();
+        // orthogonalise and expand (:213-223).  The reference orthogonalises
+        // each correction against the basis including the corrections accepted
+        // before it; here all corrections are first orthogonalised against the
+        // old basis together (two block passes, each basis vector read once),
+        // then one by one against the accepted ones, which spans the same
+        // space.  H is applied to the accepted vectors together, after the
+        // loop (only V enters the orthogonalisation).
+        const int m0 = m;
+        const bool block = use_block_dot && nt > 1 && m0 <= 64 && nt <= 16;
+        if (block) {
+            for (int pass = 0; pass < 2; ++pass) {
+                vec::block_dot(pV.get<const cplx*>(), m0, pT.get<const cplx*>(), nt, d, h.get<cplx>(), red, st);
+                vec::block_sub(pV.get<const cplx*>(), m0, pT.get<cplx*>(), nt, h.get<cplx>(), d, st);
             }
         }
+        for (int j = 0; j < nt; ++j) {
+            cplx* t = Tb.get<cplx>() + u64(j) * d;
+            if (block) cgs2(pV.get<const cplx*>() + m0, m - m0, t, d, h, red, st);
+            else cgs2(pV, m, t, d, h, red, st);
+            const double tn = dev_norm(t, d, red, sc, st);
+            if (tn > 1e-10) {
+                vec::scale(t, d, cplx{1.0 / tn, 0.0}, st);
+                QK_HIP(hipMemcpyAsync(vp(m), t, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
+                ++m;
+            }
+        }
+        if (m > m0) H.apply(vp(m0), hvp(m0), m - m0, d, st);
     }
     return DavidsonResult{};   // not reached (the last iteration returns), as :226
 }

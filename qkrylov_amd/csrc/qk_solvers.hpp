@@ -54,6 +54,13 @@ struct Preconditioner {
     // u: the Ritz vector of lambda when wants_ritz(), else null.
     virtual void apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda,
                        const cplx* u, cplx* t, hipStream_t st) = 0;
+    // The nt corrections of one Davidson iteration (host arrays of device
+    // pointers / values); the default applies them one by one.
+    virtual void apply_many(const Hamiltonian& H, int nt, const cplx* const* r, const cplx* diag,
+                            const double* lambda, const cplx* const* u, cplx* const* t,
+                            hipStream_t st) {
+        for (int j = 0; j < nt; ++j) apply(H, r[j], diag, lambda[j], u ? u[j] : nullptr, t[j], st);
+    }
     virtual bool wants_ritz() const { return false; }
 };
 

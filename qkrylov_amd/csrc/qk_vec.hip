@@ -140,14 +140,17 @@ __global__ void k_lanczos_sub(cplx* __restrict__ w, const cplx* __restrict__ v,
 }
 
 // h partials: part[j*nb + blk] = sum_{rows of blk} conj(V_j) w
+// PER elements per thread: chunks of PER * kT entries, fewer per block on
+// short vectors so that the grid still covers the chip
+template <int PER>
 __global__ void __launch_bounds__(kT) k_mdot_part(const cplx* const* __restrict__ V, int k,
                                                   const cplx* __restrict__ w, u64 n,
                                                   cplx* __restrict__ part, u32 nb) {
     __shared__ double sh[4];
-    const u64 start = u64(blockIdx.x) * kCH;
-    cplx wl[kPer];
+    const u64 start = u64(blockIdx.x) * (PER * kT);
+    cplx wl[PER];
 #pragma unroll
-    for (int e = 0; e < kPer; ++e) {
+    for (int e = 0; e < PER; ++e) {
         const u64 i = start + u64(e) * kT + threadIdx.x;
         wl[e] = (i < n) ? w[i] : cplx{0.0, 0.0};
     }
@@ -155,7 +158,7 @@ __global__ void __launch_bounds__(kT) k_mdot_part(const cplx* const* __restrict_
         const cplx* vj = V[j];
         double re = 0.0, im = 0.0;
 #pragma unroll
-        for (int e = 0; e < kPer; ++e) {
+        for (int e = 0; e < PER; ++e) {
             const u64 i = start + u64(e) * kT + threadIdx.x;
             if (i < n) {
                 const cplx x = vj[i];
@@ -364,7 +367,7 @@ __global__ void __launch_bounds__(kT) k_seg_norm_fin(const u32* __restrict__ fir
 // a fixed order by k_block_dot_fin.
 // ---------------------------------------------------------------------------
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-constexpr int kBDChunk = 8192;   // complex entries per wave slice
+constexpr int kBDMin = 128;   // least complex entries per wave slice
 
 template <int MB>
 __global__ void __launch_bounds__(kT)
@@ -420,19 +423,49 @@ k_block_dot_part(const cplx* const* __restrict__ V, const int m, const cplx* con
     }
 }
 
-// G[i*nw + j] = sum over workgroups (in order) of the partials, i < m, j < nw
-__global__ void k_block_dot_fin(const cplx* __restrict__ part, const int nparts, const int stride,
-                                const int m, const int nw, cplx* __restrict__ G) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m * nw) return;
+// G[i*nw + j] = sum of the workgroup partials, one workgroup per entry:
+// thread t sums parts t, t + kT, ... in order, then a fixed-order tree
+__global__ void __launch_bounds__(kT)
+k_block_dot_fin(const cplx* __restrict__ part, const int nparts, const int stride, const int m,
+                const int nw, cplx* __restrict__ G) {
+    const int e = blockIdx.x;
     const int i = e / nw, j = e % nw;
     double re = 0.0, iv = 0.0;
-    for (int b = 0; b < nparts; ++b) {
+    for (int b = threadIdx.x; b < nparts; b += kT) {
         const cplx v = part[size_t(b) * stride + i * 16 + j];
         re += v.re;
         iv += v.im;
     }
-    G[e] = cplx{re, iv};
+    __shared__ double sh[4];
+    re = block_sum(re, sh);
+    iv = block_sum(iv, sh);
+    if (threadIdx.x == 0) G[e] = cplx{re, iv};
+}
+
+// W_j -= sum_i V_i G[i*nw + j] for j < nw (each V_i read once for all W_j)
+template <int NO>
+__global__ void __launch_bounds__(kT) k_block_sub(const cplx* const* __restrict__ V, int m,
+                                                  cplx* const* __restrict__ W, int nw,
+                                                  const cplx* __restrict__ G, u64 n) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i >= n) return;
+    cplx acc[NO];
+#pragma unroll
+    for (int o = 0; o < NO; ++o) acc[o] = o < nw ? W[o][i] : cplx{0.0, 0.0};
+    for (int j = 0; j < m; ++j) {
+        const cplx x = V[j][i];
+#pragma unroll
+        for (int o = 0; o < NO; ++o) {
+            if (o < nw) {
+                const cplx c = G[j * nw + o];
+                acc[o].re -= c.re * x.re - c.im * x.im;
+                acc[o].im -= c.re * x.im + c.im * x.re;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < NO; ++o)
+        if (o < nw) W[o][i] = acc[o];
 }
 
 inline u32 nblk(u64 n) { return static_cast<u32>((n + kCH - 1) / kCH); }
@@ -514,10 +547,20 @@ void lanczos_sub(cplx* w, const cplx* v, const cplx* u, u64 n, const double* alp
 void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& scratch,
           hipStream_t st) {
     if (k <= 0) return;
-    const u32 nb = std::max<u32>(1, nblk(n));
+    // the largest chunk that still gives about 1024 blocks (at least 256 entries)
+    int per = kPer;
+    while (per > 1 && (n + u64(per) * kT - 1) / (u64(per) * kT) < 1024) per >>= 1;
+    const u64 chunk = u64(per) * kT;
+    const u32 nb = std::max<u32>(1, u32((n + chunk - 1) / chunk));
     scratch.resize(sizeof(cplx) * nb * size_t(k));
-    hipLaunchKernelGGL(k_mdot_part, dim3(nb), dim3(kT), 0, st, V, k, w, n, scratch.get<cplx>(),
-                       nb);
+    cplx* part = scratch.get<cplx>();
+    switch (per) {
+        case 16: hipLaunchKernelGGL(k_mdot_part<16>, dim3(nb), dim3(kT), 0, st, V, k, w, n, part, nb); break;
+        case 8: hipLaunchKernelGGL(k_mdot_part<8>, dim3(nb), dim3(kT), 0, st, V, k, w, n, part, nb); break;
+        case 4: hipLaunchKernelGGL(k_mdot_part<4>, dim3(nb), dim3(kT), 0, st, V, k, w, n, part, nb); break;
+        case 2: hipLaunchKernelGGL(k_mdot_part<2>, dim3(nb), dim3(kT), 0, st, V, k, w, n, part, nb); break;
+        default: hipLaunchKernelGGL(k_mdot_part<1>, dim3(nb), dim3(kT), 0, st, V, k, w, n, part, nb); break;
+    }
     hipLaunchKernelGGL(k_finish_c, dim3(k), dim3(kT), 0, st, scratch.get<cplx>(), nb, h);
     QK_HIP(hipGetLastError());
 }
@@ -526,7 +569,9 @@ void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n,
                DevBuf& scratch, hipStream_t st) {
     if (m <= 0 || nw <= 0) return;
     if (m > 64 || nw > 16) fail("block_dot: at most 64 x 16 projections per call");
-    const u64 waves = std::max<u64>(1, std::min<u64>((n + kBDChunk - 1) / kBDChunk, 4096));
+    // at most 4096 waves, each over at least kBDMin entries: long vectors
+    // keep long slices, short ones still spread over the chip
+    const u64 waves = std::max<u64>(1, std::min<u64>((n + kBDMin - 1) / kBDMin, 4096));
     u64 per_wave = (n + waves - 1) / waves;
     per_wave = (per_wave + 1) & ~u64(1);                // even: pairs of entries
     const u32 nparts = u32((((n + per_wave - 1) / per_wave) + 3) / 4);
@@ -541,8 +586,20 @@ void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n,
         case 3: hipLaunchKernelGGL((k_block_dot_part<3>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
         default: hipLaunchKernelGGL((k_block_dot_part<4>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
     }
-    hipLaunchKernelGGL(k_block_dot_fin, dim3((m * nw + 255) / 256), dim3(256), 0, st, part, int(g),
-                       stride, m, nw, G);
+    hipLaunchKernelGGL(k_block_dot_fin, dim3(m * nw), dim3(kT), 0, st, part, int(g), stride, m, nw, G);
+    QK_HIP(hipGetLastError());
+}
+
+void block_sub(const cplx* const* V, int m, cplx* const* W, int nw, const cplx* G, u64 n,
+               hipStream_t st) {
+    if (m <= 0 || nw <= 0 || !n) return;
+    if (nw > 16) fail("block_sub: at most 16 vectors per call");
+    if (nw <= 4)
+        hipLaunchKernelGGL(k_block_sub<4>, dim3(ngrid(n)), dim3(kT), 0, st, V, m, W, nw, G, n);
+    else if (nw <= 8)
+        hipLaunchKernelGGL(k_block_sub<8>, dim3(ngrid(n)), dim3(kT), 0, st, V, m, W, nw, G, n);
+    else
+        hipLaunchKernelGGL(k_block_sub<16>, dim3(ngrid(n)), dim3(kT), 0, st, V, m, W, nw, G, n);
     QK_HIP(hipGetLastError());
 }
 

@@ -50,6 +50,10 @@ void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& sc
 // (FP64 MFMA; each vector read once; deterministic two-stage reduction).
 void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n, cplx* G,
                DevBuf& scratch, hipStream_t st);
+// W_j -= sum_i V_i G[i*nw + j] for nw <= 16 vectors (G on device); each V_i
+// is read once for all W_j
+void block_sub(const cplx* const* V, int m, cplx* const* W, int nw, const cplx* G, u64 n,
+               hipStream_t st);
 // w -= sum_j V_j h[j]
 void msub(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, hipStream_t st);
 // out = sum_j c[j] V_j  (c on device)

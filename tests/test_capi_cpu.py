@@ -172,3 +172,32 @@ def test_spectral_function_host(qk, orc):
     s64 = orc.spectral(af.astype(np.float64), bf.astype(np.float64), float(np.float32(1.3)), 0.5,
                        float(np.float32(-0.7)), float(np.float32(0.05)))
     assert abs(s32 - s64) <= 1e-5 * abs(s64)
+
+
+def test_hermitian_eigh_host(qk):
+    """The host eigensolver of the solvers' projected matrices against
+    LAPACK (numpy.linalg.eigh): random, degenerate, diagonal and already
+    tridiagonal Hermitian matrices."""
+    lib = qk.load()
+    rng = np.random.default_rng(21)
+    cases = []
+    for n in (1, 2, 3, 17, 48, 96):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        cases.append(a + a.conj().T)
+    q, _ = np.linalg.qr(rng.standard_normal((30, 30)) + 1j * rng.standard_normal((30, 30)))
+    cases.append(q @ np.diag(np.repeat([-2.0, 0.5, 0.5001, 3.0, 3.0, 3.0], 5)) @ q.conj().T)
+    cases.append(np.diag(rng.standard_normal(12)).astype(complex))
+    t = np.diag(rng.standard_normal(20)) + np.diag(rng.standard_normal(19) * 1j, 1)
+    cases.append(t + np.triu(t, 1).conj().T)
+    for A in cases:
+        n = A.shape[0]
+        A = np.ascontiguousarray(A, dtype=np.complex128)
+        ev = np.empty(n)
+        vecs = np.empty((n, n), dtype=np.complex128)
+        assert lib.qkrylov_hermitian_eigh_f64(n, A.ctypes.data, ev.ctypes.data, vecs.ctypes.data) == 0
+        ref = np.linalg.eigvalsh(A)
+        scale = max(1.0, np.max(np.abs(ref)))
+        assert np.max(np.abs(ev - ref)) < 1e-13 * scale * max(1, n / 10), n
+        res = np.linalg.norm(A @ vecs.T - vecs.T * ev[None, :], axis=0)
+        assert np.max(res) < 1e-12 * scale * max(1, n / 10), n
+        assert np.max(np.abs(vecs.conj() @ vecs.T - np.eye(n))) < 1e-12 * max(1, n / 10), n
