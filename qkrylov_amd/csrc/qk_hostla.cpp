@@ -185,70 +185,4 @@ double tridiag_ground(const std::vector<double>& a, const std::vector<double>& b
     return eig[imin];
 }
 
-void hermitian_jacobi(const std::vector<cplx>& Ain, int n, std::vector<double>& evals,
-                      std::vector<cplx>& vecs) {
-    std::vector<cplx> A = Ain, V(size_t(n) * n, cplx{0.0, 0.0});
-    for (int i = 0; i < n; ++i) V[size_t(i) * n + i] = cplx{1.0, 0.0};
-    auto at = [&](std::vector<cplx>& M, int i, int j) -> cplx& { return M[size_t(i) * n + j]; };
-    auto cabs_ = [](cplx z) { return std::hypot(z.re, z.im); };
-
-    // One Jacobi rotation zeroing A[p][q].  The reference (davidson.cpp:28-82)
-    // applies the same rotation to the largest off-diagonal element, at most
-    // 1000 times, which leaves subspaces much larger than its default 20 only
-    // partly diagonalised; here cyclic sweeps run until every off-diagonal
-    // element is below 1e-15 relative to the largest diagonal magnitude.
-    auto rotate = [&](int p, int q) {
-        const cplx app = at(A, p, p), aqq = at(A, q, q), apq = at(A, p, q);
-        const double r = cabs_(apq);
-        const double phi = 0.5 * std::atan2(2.0 * r, app.re - aqq.re);
-        const double c = std::cos(phi), sphi = std::sin(phi);
-        // J = [[c, -conj(s)], [s, c]] on (p, q); s = sin(phi) * conj(apq)/|apq|
-        // makes (J^H A J)_{pq} vanish for any complex apq (for real apq this
-        // is the reference's rotation).
-        const cplx s{sphi * apq.re / r, -sphi * apq.im / r};
-        const cplx sc{s.re, -s.im};
-        for (int i = 0; i < n; ++i) {
-            const cplx vip = at(V, i, p), viq = at(V, i, q);
-            at(V, i, p) = cadd(cmul({c, 0.0}, vip), cmul(s, viq));
-            at(V, i, q) = cadd(cmul({-sc.re, -sc.im}, vip), cmul({c, 0.0}, viq));
-            if (i != p && i != q) {
-                const cplx aip = at(A, i, p), aiq = at(A, i, q);
-                const cplx nip = cadd(cmul({c, 0.0}, aip), cmul(s, aiq));
-                const cplx niq = cadd(cmul({-sc.re, -sc.im}, aip), cmul({c, 0.0}, aiq));
-                at(A, i, p) = nip;
-                at(A, p, i) = cconj(nip);
-                at(A, i, q) = niq;
-                at(A, q, i) = cconj(niq);
-            }
-        }
-        const double ss = s.re * s.re + s.im * s.im;
-        const double cross = 2.0 * c * cmul(s, apq).re;
-        at(A, p, p) = cplx{c * c * app.re + ss * aqq.re + cross, 0.0};
-        at(A, q, q) = cplx{ss * app.re + c * c * aqq.re - cross, 0.0};
-        at(A, p, q) = cplx{0.0, 0.0};
-        at(A, q, p) = cplx{0.0, 0.0};
-    };
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double dmax = 0.0, off = 0.0;
-        for (int i = 0; i < n; ++i) {
-            dmax = std::max(dmax, std::abs(at(A, i, i).re));
-            for (int j = i + 1; j < n; ++j) off = std::max(off, cabs_(at(A, i, j)));
-        }
-        if (off < 1e-15 * std::max(dmax, 1e-300) || off < 1e-300) break;
-        const double thr = sweep < 3 ? 0.0 : 1e-17 * std::max(dmax, 1e-300);
-        for (int p = 0; p < n; ++p)
-            for (int q = p + 1; q < n; ++q)
-                if (cabs_(at(A, p, q)) > thr) rotate(p, q);
-    }
-    std::vector<int> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
-    std::sort(idx.begin(), idx.end(), [&](int i, int j) { return at(A, i, i).re < at(A, j, j).re; });
-    evals.resize(n);
-    vecs.assign(size_t(n) * n, cplx{0.0, 0.0});
-    for (int k = 0; k < n; ++k) {
-        evals[k] = at(A, idx[k], idx[k]).re;
-        for (int j = 0; j < n; ++j) vecs[size_t(k) * n + j] = at(V, j, idx[k]);
-    }
-}
-
 }  // namespace qk

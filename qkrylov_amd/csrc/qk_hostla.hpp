@@ -3,9 +3,10 @@
 // deflation, at most 1000 sweeps; the sweep rules of the reference
 // src/solvers/lanczos.cpp:25-98 and ftlm.cpp:61-119, so that the
 // eigenvalues, and hence the Lanczos stopping iteration, are the same) and
-// Hermitian matrices (the Davidson projected matrix, davidson.cpp:22-82, and
-// the block-Lanczos matrix): cyclic Jacobi run to convergence, or Householder
-// tridiagonalisation followed by the same QL.
+// Hermitian matrices (the Davidson projected matrix, which the reference
+// diagonalises with Jacobi rotations, davidson.cpp:22-82, and the
+// block-Lanczos matrix): Householder tridiagonalisation followed by the same
+// QL.  Any converged solver gives the same eigenvalues.
 #pragma once
 
 #include "qk_common.hpp"
@@ -25,11 +26,9 @@ double tridiag_ground(const std::vector<double>& a, const std::vector<double>& b
                       std::vector<double>* vec);
 
 // Eigen-decomposition of a small Hermitian matrix (row-major n*n), ascending
-// eigenvalues; vecs[k*n + j] = component j of eigenvector k.
-void hermitian_jacobi(const std::vector<cplx>& A, int n, std::vector<double>& evals,
-                      std::vector<cplx>& vecs);
-// Same contract, by Householder tridiagonalisation and tridiag_ql: O(n^3)
-// with a small constant, for the solvers' per-iteration projected matrices.
+// eigenvalues; vecs[k*n + j] = component j of eigenvector k.  Householder
+// tridiagonalisation and tridiag_ql: O(n^3) with a small constant, for the
+// solvers' per-iteration projected matrices.
 void hermitian_eigh(const std::vector<cplx>& A, int n, std::vector<double>& evals,
                     std::vector<cplx>& vecs);
 
