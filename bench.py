@@ -21,6 +21,7 @@ also carries:
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-ftlm] [--skip-cpu]
                     [--skip-lanczos] [--skip-davidson] [--skip-dynamics]
+                    [--apply-only | --ftlm-only | --davidson-only]
 """
 from __future__ import annotations
 
@@ -308,6 +309,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--apply-only", action="store_true", help="time the b=1 apply only")
     ap.add_argument("--ftlm-only", action="store_true", help="time the FTLM sweep only")
+    ap.add_argument("--davidson-only", action="store_true",
+                    help="run the t-J 4x4 Davidson leg only (training included)")
     ap.add_argument("--skip-lanczos", action="store_true", help="skip the J1-J2 6x4 Lanczos run")
     ap.add_argument("--skip-davidson", action="store_true", help="skip the t-J 4x4 Davidson runs")
     ap.add_argument("--skip-dynamics", action="store_true", help="skip the N=28 dynamics run")
@@ -335,6 +338,9 @@ def main():
     log(f"device: {qk.device_info()}")
     if args.ftlm_only:
         print(json.dumps(bench_ftlm(qk, lat)))
+        return
+    if args.davidson_only:
+        print(json.dumps(bench_davidson_tj(qk, lat)))
         return
     t = time.time()
     basis = qk.SpinHalfBasis(N_SITES, sz=0)

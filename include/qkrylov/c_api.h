@@ -280,7 +280,8 @@ typedef struct {
     int epochs;
     double loss_first;     /* mean loss (qkrylov_mlp_pair_loss_f64) before training */
     double loss_last;      /* after training */
-    double solve_rel_res;  /* worst relative residual of the accurate corrections */
+    double solve_rel_res;  /* worst relative residual of MINRES-solved corrections
+                              (0: the exact image K t* = -r was used) */
 } qkrylov_mlp_train_info_t;
 
 /* Random hidden layers from std::mt19937(seed), output layer zero. */

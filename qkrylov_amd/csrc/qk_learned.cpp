@@ -171,7 +171,9 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
     if (P == 0) return out;
 
     // 2. accurate corrections t*, stored as the unit vector K t* (the loss
-    // target, K = P_u (H - theta) P_u), and the two correction terms
+    // target, K = P_u (H - theta) P_u): K t* = -r for the exact solution, or
+    // K applied to a MINRES solution when cfg.solve_iters > 0; and the two
+    // correction terms
     // z = J r, w = J (H z - diag z) of every pair; everything pair-major
     MlpStates states;
     states.build(Hs, st);
@@ -183,10 +185,18 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
         cplx* tp = T.get<cplx>() + u64(p) * n;
         cplx* zp = Z.get<cplx>() + u64(p) * n;
         const cplx* up = rec.u[p].get<cplx>();
-        const double rr = jd_correction(Hs, rec.r[p].get<cplx>(), up, rec.theta[p], ts.get<cplx>(),
-                                        cfg.solve_iters, cfg.solve_rtol, st);
-        worst = std::max(worst, rr);
-        proj_op(Hs, b, up, rec.theta[p], ts.get<cplx>(), tp);
+        if (cfg.solve_iters > 0) {
+            const double rr = jd_correction(Hs, rec.r[p].get<cplx>(), up, rec.theta[p], ts.get<cplx>(),
+                                            cfg.solve_iters, cfg.solve_rtol, st);
+            worst = std::max(worst, rr);
+            proj_op(Hs, b, up, rec.theta[p], ts.get<cplx>(), tp);
+        } else {
+            // K t* = -r exactly for the solution t* of the correction equation
+            b.copy(tp, rec.r[p].get<cplx>());
+            b.scale(tp, -1.0);
+            const cplx c = b.dot(up, tp);
+            b.axpy(cplx{-c.re, -c.im}, up, tp);
+        }
         const double nt = b.norm(tp);
         if (nt > 0.0) b.scale(tp, 1.0 / nt);
         mlp_jacobi(states, rec.theta[p], rec.r[p].get<cplx>(), zp, st);

@@ -14,7 +14,10 @@ struct MlpTrainConfig {
     double tol = 1e-8;
     int max_iter = 40;
     int max_pairs = 256;
-    int solve_iters = 500;    // MINRES iterations per accurate correction
+    // MINRES iterations per accurate correction; 0: use K t* = -r, exact for
+    // the solution t* of the correction equation (the loss reads t* only
+    // through K t*)
+    int solve_iters = 0;
     double solve_rtol = 1e-8;
     int epochs = 30;
     int batch_pairs = 8;
