@@ -224,6 +224,30 @@ int qkrylov_hamiltonian_apply_f64(qkrylov_hamiltonian_h h, const double* x_compl
 int qkrylov_hamiltonian_apply_device_f64(qkrylov_hamiltonian_h h, const void* x_dev, void* y_dev,
                                          int nvec, uint64_t ld, void* stream);
 
+/* ---- Operator between bases.  No counterpart in the reference, which applies
+ * only Hamiltonians within one basis; the operator-on-state apply its
+ * dynamics tests ask for (tests/test_dynamics.cpp:43-46): y = O x for an
+ * OpSum O taking states of basis_in to basis_out, which must share site kind,
+ * size and sector type (two Sz sectors, two particle-number sectors, two
+ * (nup, ndn) sectors); O need not be Hermitian nor conserve the sector
+ * (S+(q), c_k).  Same factor semantics as the Hamiltonian (factors act on the
+ * ket in list order); y[b] = sum_a <b|O|a> x[a]. ---- */
+typedef struct qkrylov_operator_t* qkrylov_operator_h;
+qkrylov_operator_h qkrylov_operator_create(qkrylov_basis_h basis_in, qkrylov_basis_h basis_out,
+                                          qkrylov_site_h site, qkrylov_opsum_h opsum);
+void qkrylov_operator_destroy(qkrylov_operator_h op);
+uint64_t qkrylov_operator_dimension_in(qkrylov_operator_h op);
+uint64_t qkrylov_operator_dimension_out(qkrylov_operator_h op);
+/* Host FP64 buffers: nvec input vectors of dimension_in complex values one
+ * after another, nvec outputs of dimension_out. */
+int qkrylov_operator_apply_f64(qkrylov_operator_h op, const double* x_complex, double* y_complex,
+                               int nvec);
+/* Device buffers: input v at x_dev + v*ld_in, output v at y_dev + v*ld_out
+ * complex FP64 elements (0: the dimension); asynchronous on stream (NULL: the
+ * handle's stream). */
+int qkrylov_operator_apply_device_f64(qkrylov_operator_h op, const void* x_dev, void* y_dev, int nvec,
+                                      uint64_t ld_in, uint64_t ld_out, void* stream);
+
 /* Full complex diagonal, accumulated per term in OpSum order (reference
  * cpp:81-89); diag_complex: 2*dim doubles. */
 int qkrylov_hamiltonian_diagonal_f64(qkrylov_hamiltonian_h h, double* diag_complex);

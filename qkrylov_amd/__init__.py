@@ -26,7 +26,8 @@ __all__ = [
     "continued_fraction_block", "evaluate_spectral_function", "FTLMResult", "ftlm",
     "ftlm_sweep", "device_info", "gpu_count", "find_gpu", "Bdag", "B",
     "MLPPreconditioner", "davidson_lowest_ex", "jd_correction", "mlp_features",
-    "continued_fraction_block_device", "block_lanczos",
+    "continued_fraction_block_device", "block_lanczos", "Operator", "mlp_correction",
+    "mlp_pair_loss",
 ]
 
 
@@ -431,6 +432,62 @@ class OpSum:
 
 
 # --------------------------------------------------------------------------- Hamiltonian
+class Operator:
+    """y = O x for an OpSum O taking states of ``basis_in`` to ``basis_out``
+    (same site kind, size and sector type; any two sectors), on the GPU in
+    FP64: y[b] = sum_a <b|O|a> x[a], factors acting on the ket in list order.
+    O need not be Hermitian nor conserve the sector (S+(q), c_k).  No
+    reference counterpart: the reference applies Hamiltonians within one
+    basis only (its dynamics tests ask for this, tests/test_dynamics.cpp:43-46)."""
+
+    def __init__(self, basis_in: Basis, basis_out: Basis, site: Optional[Site], ops: OpSum):
+        if site is None:
+            site = site_for(basis_in)
+        self.basis_in, self.basis_out, self.site, self.ops = basis_in, basis_out, site, ops
+        lib = load()
+        oh = ops._handle()
+        try:
+            self._h = lib.qkrylov_operator_create(basis_in._h, basis_out._h, site._h, oh)
+        finally:
+            lib.qkrylov_opsum_destroy(oh)
+        if not self._h:
+            raise RuntimeError(f"Operator create failed: {last_error()}")
+
+    def __del__(self):
+        try:
+            _release(self, "qkrylov_operator_destroy")
+        except Exception:  # noqa: BLE001  (interpreter exit)
+            pass
+
+    @property
+    def shape(self):
+        lib = load()
+        return (int(lib.qkrylov_operator_dimension_out(self._h)),
+                int(lib.qkrylov_operator_dimension_in(self._h)))
+
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        dout, din = self.shape
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.shape[-1] != din or x.ndim > 2:
+            raise ValueError("input length must equal the input basis dimension")
+        nvec = 1 if x.ndim == 1 else x.shape[0]
+        y = np.empty(x.shape[:-1] + (dout,), dtype=np.complex128)
+        if dout == 0 or nvec == 0:
+            return y
+        check(load().qkrylov_operator_apply_f64(self._h, _ptr(x) if din else None, _ptr(y), nvec),
+              "operator apply")
+        return y
+
+    __matmul__ = apply
+
+    def apply_device(self, x_ptr: int, y_ptr: int, nvec: int = 1, ld_in: int = 0, ld_out: int = 0,
+                     stream: int = 0) -> None:
+        """Device buffers (complex FP64), asynchronous on ``stream``."""
+        check(load().qkrylov_operator_apply_device_f64(self._h, x_ptr or None, y_ptr, int(nvec),
+                                                       int(ld_in), int(ld_out), stream or None),
+              "operator apply")
+
+
 class MatrixFreeHamiltonian:
     """Matrix-free H on the GPU (reference hamiltonian.py / matrix_free_hamiltonian.hpp).
 

@@ -152,6 +152,13 @@ SIGNATURES = {
     "qkrylov_mlp_set_params": (c_int, [c_void_p, c_void_p]),
     "qkrylov_mlp_eval": (c_int, [c_void_p, c_void_p, c_u64, c_void_p, c_void_p, c_void_p]),
     "qkrylov_mlp_features": (c_int, [c_void_p, c_double, c_void_p]),
+    "qkrylov_operator_create": (c_void_p, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "qkrylov_operator_destroy": (None, [c_void_p]),
+    "qkrylov_operator_dimension_in": (c_u64, [c_void_p]),
+    "qkrylov_operator_dimension_out": (c_u64, [c_void_p]),
+    "qkrylov_operator_apply_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
+    "qkrylov_operator_apply_device_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_u64, c_u64,
+                                                  c_void_p]),
     "qkrylov_hermitian_eigh_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "qkrylov_mlp_correction_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                                            c_void_p, c_void_p]),

@@ -39,6 +39,9 @@ struct qkrylov_hamiltonian_t {
 struct qkrylov_mlp_t {
     std::unique_ptr<Mlp> ptr;
 };
+struct qkrylov_operator_t {
+    std::unique_ptr<Operator> ptr;
+};
 
 namespace {
 
@@ -428,6 +431,47 @@ int qkrylov_hamiltonian_apply_device_f64(qkrylov_hamiltonian_h h, const void* x,
                       ld ? ld : h->ptr->dim(), static_cast<hipStream_t>(stream));
     });
 }
+qkrylov_operator_h qkrylov_operator_create(qkrylov_basis_h in, qkrylov_basis_h out, qkrylov_site_h site,
+                                          qkrylov_opsum_h opsum) {
+    if (!in || !in->ptr || !out || !out->ptr || !site || !opsum) return nullptr;
+    return guarded_create<qkrylov_operator_t>([&] {
+        auto h = std::make_unique<qkrylov_operator_t>();
+        h->ptr = std::make_unique<Operator>(in->ptr, out->ptr, site->kind, opsum->opsum);
+        return h.release();
+    });
+}
+void qkrylov_operator_destroy(qkrylov_operator_h op) { delete op; }
+uint64_t qkrylov_operator_dimension_in(qkrylov_operator_h op) { return op && op->ptr ? op->ptr->dim_in() : 0; }
+uint64_t qkrylov_operator_dimension_out(qkrylov_operator_h op) { return op && op->ptr ? op->ptr->dim_out() : 0; }
+int qkrylov_operator_apply_f64(qkrylov_operator_h op, const double* x, double* y, int nvec) {
+    if (!op || !op->ptr || !y || nvec <= 0) return QKRYLOV_ERROR_INVALID_ARG;
+    const Operator& O = *op->ptr;
+    if (!x && O.dim_in() > 0) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const u64 di = O.dim_in(), dout = O.dim_out();
+        if (dout == 0) return;
+        hipStream_t st = O.stream();
+        DevBuf xb(sizeof(cplx) * std::max<u64>(di, 1) * nvec), yb(sizeof(cplx) * dout * nvec);
+        if (di) QK_HIP(hipMemcpyAsync(xb.get(), x, sizeof(cplx) * di * nvec, hipMemcpyHostToDevice, st));
+        O.apply(xb.get<cplx>(), yb.get<cplx>(), nvec, std::max<u64>(di, 1), dout, st);
+        QK_HIP(hipMemcpyAsync(y, yb.get(), sizeof(cplx) * dout * nvec, hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+    });
+}
+int qkrylov_operator_apply_device_f64(qkrylov_operator_h op, const void* x, void* y, int nvec, uint64_t ld_in,
+                                      uint64_t ld_out, void* stream) {
+    if (!op || !op->ptr || !y || nvec <= 0) return QKRYLOV_ERROR_INVALID_ARG;
+    const Operator& O = *op->ptr;
+    if (!x && O.dim_in() > 0) return QKRYLOV_ERROR_INVALID_ARG;
+    if (ld_in == 0) ld_in = O.dim_in();
+    if (ld_out == 0) ld_out = O.dim_out();
+    if (nvec > 1 && (ld_in < O.dim_in() || ld_out < O.dim_out())) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        O.apply(static_cast<const cplx*>(x), static_cast<cplx*>(y), nvec, ld_in, ld_out,
+                static_cast<hipStream_t>(stream));
+    });
+}
+
 int qkrylov_hamiltonian_diagonal_f64(qkrylov_hamiltonian_h h, double* out) {
     if (!hok(h) || !out) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {

@@ -109,9 +109,14 @@ __device__ __forceinline__ void store_nt(T* p, const T& v) {
     __builtin_nontemporal_store(v.im, &p->im);
 }
 
-template <typename S, int B>
+// CROSS: rows (y) enumerate one basis and the targets (x) are ranked in
+// another of the same kind, size and counted masks (ColTables: its row
+// offsets and row classes; the low-half ranks are shared).  y has stride
+// c.ld_out, x stride ld.
+template <typename S, int B, bool CROSS>
 __global__ void __launch_bounds__(kGBlock)
-k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, const u64 ld) {
+k_apply(const ApplyParams p, const ColTables c, const cplx* __restrict__ x, cplx* __restrict__ y,
+        const u64 ld) {
     extern __shared__ __align__(16) unsigned char smem[];
     cplx* s_tab = reinterpret_cast<cplx*>(smem);
     u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ntab) * sizeof(cplx));
@@ -143,6 +148,13 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
         const u32 lo = p.lo_list[off + r];
         const S s = (S(hi) << p.L) | S(lo);
         const u64 a = base + r;
+        // row hi of the target basis: same-row targets and the diagonal
+        u64 base_c = base;
+        int cls_c = cls;
+        if constexpr (CROSS) {
+            base_c = c.jhi[hi];
+            cls_c = base_c == ~u64(0) ? -1 : c.hi_cls[hi];
+        }
 
         cplx acc[B];
 #pragma unroll
@@ -167,9 +179,9 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
             const u32 tlo = lo ^ u32(G.flip);
             const u16 rl = s_jlo[tlo];
             if (rl == 0xFFFF) continue;
-            if (__popc(tlo & u32(p.m1lo)) * p.ncls2 + __popc(tlo & u32(p.m2lo)) != cls) continue;
+            if (__popc(tlo & u32(p.m1lo)) * p.ncls2 + __popc(tlo & u32(p.m2lo)) != cls_c) continue;
             if (G.jw && (popc_s(s & S(G.jw)) & 1)) { amp.re = -amp.re; amp.im = -amp.im; }
-            const u64 b = base + rl;
+            const u64 b = base_c + rl;
 #pragma unroll
             for (int v = 0; v < B; ++v) cfma(acc[v], amp, x[v * ld + b]);
         }
@@ -178,8 +190,8 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
         for (; g < g_hi_end; ++g) {
             const DevGroup G = p.groups[g];
             const u32 hi2 = hi ^ u32(G.flip >> p.L);
-            const u64 base2 = p.jhi[hi2];
-            if (base2 == ~u64(0) || p.hi_cls[hi2] != cls) continue;   // uniform
+            const u64 base2 = CROSS ? c.jhi[hi2] : p.jhi[hi2];
+            if (base2 == ~u64(0) || (CROSS ? c.hi_cls[hi2] : p.hi_cls[hi2]) != cls) continue;   // uniform
             cplx amp = s_tab[G.tab + config_of(s, G)];
             if (amp.re == 0.0 && amp.im == 0.0) continue;
             if (G.jw && (popc_s(s & S(G.jw)) & 1)) { amp.re = -amp.re; amp.im = -amp.im; }
@@ -192,9 +204,9 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
         for (; g < g_end; ++g) {
             const DevGroup G = p.groups[g];
             const u32 hi2 = hi ^ u32(G.flip >> p.L);
-            const u64 base2 = p.jhi[hi2];
+            const u64 base2 = CROSS ? c.jhi[hi2] : p.jhi[hi2];
             if (base2 == ~u64(0)) continue;                               // uniform
-            const int cls2 = p.hi_cls[hi2];
+            const int cls2 = CROSS ? c.hi_cls[hi2] : p.hi_cls[hi2];
             cplx amp = s_tab[G.tab + config_of(s, G)];
             if (amp.re == 0.0 && amp.im == 0.0) continue;
             const u32 tlo = lo ^ u32(G.flip & u64(lomask));
@@ -209,8 +221,9 @@ k_apply(const ApplyParams p, const cplx* __restrict__ x, cplx* __restrict__ y, c
 #pragma unroll
         for (int v = 0; v < B; ++v) {
             cplx o = acc[v];
-            if (p.n_diag) cfma(o, dsum, x[v * ld + a]);
-            y[v * ld + a] = o;
+            if (p.n_diag && cls_c == cls) cfma(o, dsum, x[v * ld + base_c + r]);
+            if constexpr (CROSS) y[v * c.ld_out + a] = o;
+            else y[v * ld + a] = o;
         }
     }
 }
@@ -699,8 +712,8 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
             return;
         }
     }
-    hipLaunchKernelGGL((k_apply<S, B>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, x,
-                       y, ld);
+    hipLaunchKernelGGL((k_apply<S, B, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params,
+                       ColTables{}, x, y, ld);
 }
 
 template <typename S>
@@ -844,11 +857,15 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
             QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes));
         };
         if (state32) {
-            set((const void*)k_apply<u32, 1>); set((const void*)k_apply<u32, 2>);
-            set((const void*)k_apply<u32, 4>); set((const void*)k_apply<u32, 8>);
+            set((const void*)k_apply<u32, 1, false>); set((const void*)k_apply<u32, 2, false>);
+            set((const void*)k_apply<u32, 4, false>); set((const void*)k_apply<u32, 8, false>);
+            set((const void*)k_apply<u32, 1, true>); set((const void*)k_apply<u32, 2, true>);
+            set((const void*)k_apply<u32, 4, true>); set((const void*)k_apply<u32, 8, true>);
         } else {
-            set((const void*)k_apply<u64, 1>); set((const void*)k_apply<u64, 2>);
-            set((const void*)k_apply<u64, 4>); set((const void*)k_apply<u64, 8>);
+            set((const void*)k_apply<u64, 1, false>); set((const void*)k_apply<u64, 2, false>);
+            set((const void*)k_apply<u64, 4, false>); set((const void*)k_apply<u64, 8, false>);
+            set((const void*)k_apply<u64, 1, true>); set((const void*)k_apply<u64, 2, true>);
+            set((const void*)k_apply<u64, 4, true>); set((const void*)k_apply<u64, 8, true>);
         }
     }
 
@@ -1201,6 +1218,78 @@ void Hamiltonian::apply(const cplx* x, cplx* y, int nvec, u64 ld, hipStream_t st
     if (!st) st = stream;
     if (state32) launch_apply_s<u32>(*this, x, y, nvec, ld, st);
     else launch_apply_s<u64>(*this, x, y, nvec, ld, st);
+}
+
+// ---------------------------------------------------------------------------
+// Operator between bases
+// ---------------------------------------------------------------------------
+OpSum adjoint(const OpSum& ops) {
+    static const std::pair<const char*, const char*> swap[] = {
+        {"Sp", "Sm"}, {"C", "Cdag"}, {"CUp", "CdagUp"}, {"CDn", "CdagDn"}};
+    OpSum out;
+    for (const OpTerm& t : ops.terms) {
+        OpTerm a;
+        a.coeff = cplx{t.coeff.re, -t.coeff.im};
+        for (auto it = t.factors.rbegin(); it != t.factors.rend(); ++it) {
+            OpFactor f = *it;
+            for (const auto& sw : swap) {
+                if (f.op == sw.first) { f.op = sw.second; break; }
+                if (f.op == sw.second) { f.op = sw.first; break; }
+            }
+            a.factors.push_back(f);
+        }
+        out.terms.push_back(std::move(a));
+    }
+    return out;
+}
+
+Operator::Operator(std::shared_ptr<const Basis> in, std::shared_ptr<const Basis> out, SiteKind site,
+                   const OpSum& ops)
+    : in_(std::move(in)) {
+    const Basis& A = *in_;
+    const Basis& B = *out;
+    if (A.kind() != B.kind() || A.nsites() != B.nsites() || A.L() != B.L() || A.m1() != B.m1() ||
+        A.m2() != B.m2() || A.ncls2() != B.ncls2())
+        fail("operator: input and output bases must share site kind, size and sector type");
+    rows_ = std::make_unique<Hamiltonian>(std::move(out), site, adjoint(ops));
+    upload(jhi_in_, A.jhi().data(), A.jhi().size(), rows_->stream);
+    upload(hicls_in_, A.hi_cls().data(), A.hi_cls().size(), rows_->stream);
+    QK_HIP(hipStreamSynchronize(rows_->stream));
+}
+
+template <typename S>
+static void launch_operator(const Hamiltonian& h, const ColTables& c, const cplx* x, cplx* y, int nvec,
+                            u64 ld_in, u64 ld_out, hipStream_t st) {
+    int v = 0;
+    while (v < nvec) {
+        const int left = nvec - v;
+        const cplx* xv = x ? x + u64(v) * ld_in : nullptr;
+        cplx* yv = y + u64(v) * ld_out;
+        if (left >= 8) {
+            hipLaunchKernelGGL((k_apply<S, 8, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            v += 8;
+        } else if (left >= 4) {
+            hipLaunchKernelGGL((k_apply<S, 4, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            v += 4;
+        } else if (left >= 2) {
+            hipLaunchKernelGGL((k_apply<S, 2, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            v += 2;
+        } else {
+            hipLaunchKernelGGL((k_apply<S, 1, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            v += 1;
+        }
+    }
+    QK_HIP(hipGetLastError());
+}
+
+void Operator::apply(const cplx* x, cplx* y, int nvec, u64 ld_in, u64 ld_out, hipStream_t st) const {
+    if (dim_out() == 0 || nvec <= 0) return;
+    if (!st) st = rows_->stream;
+    // targets outside `in` (every target when it is empty) are skipped by the
+    // row tables, so x is never read then
+    const ColTables c{jhi_in_.get<u64>(), hicls_in_.get<int>(), ld_out};
+    if (rows_->state32) launch_operator<u32>(*rows_, c, x, y, nvec, ld_in, ld_out, st);
+    else launch_operator<u64>(*rows_, c, x, y, nvec, ld_in, ld_out, st);
 }
 
 void Hamiltonian::diagonal(cplx* d, hipStream_t st) const {

@@ -112,6 +112,14 @@ struct ApplyParams {
     u32 tiles_per_xcd;
 };
 
+// Row tables of the basis the targets are ranked in, when it differs from
+// the basis whose rows are enumerated (Operator); unused otherwise.
+struct ColTables {
+    const u64* jhi = nullptr;
+    const int* hi_cls = nullptr;
+    u64 ld_out = 0;
+};
+
 struct Hamiltonian {
     Hamiltonian(std::shared_ptr<const Basis> basis, SiteKind site, const OpSum& ops);
     ~Hamiltonian();
@@ -198,6 +206,34 @@ private:
     DevBuf tiles_, jhis_;
     int grid_ = 0;
     int vb_ = 8;
+};
+
+// y = O x for an OpSum O taking states of basis `in` to basis `out`: both of
+// the same site kind and size and with the same counted masks (e.g. two Sz
+// sectors, two particle-number sectors), so the low-half ranks agree; O need
+// not be Hermitian nor conserve the sector (S+(q), c_k).  The gather kernel
+// computes sum_b conj(amp(a -> b)) x[b] over the rows a of `out`, i.e. the
+// conjugate transpose of what it compiles, so it compiles O's adjoint:
+// coefficients conjugated, factor order reversed, each factor replaced by
+// its adjoint (Sp <-> Sm, C <-> Cdag, CUp <-> CdagUp, CDn <-> CdagDn; Sz, Sx,
+// Sy, N, Nup, Ndn, Nupdn, Id are self-adjoint).  No reference counterpart
+// (the reference applies only Hamiltonians within one basis).
+OpSum adjoint(const OpSum& ops);
+
+class Operator {
+public:
+    Operator(std::shared_ptr<const Basis> in, std::shared_ptr<const Basis> out, SiteKind site,
+             const OpSum& ops);
+    u64 dim_in() const { return in_->size(); }
+    u64 dim_out() const { return rows_->dim(); }
+    // y[v*ld_out + b] = sum_a <b|O|a> x[v*ld_in + a], v < nvec (device pointers)
+    void apply(const cplx* x, cplx* y, int nvec, u64 ld_in, u64 ld_out, hipStream_t st) const;
+    hipStream_t stream() const { return rows_->stream; }
+
+private:
+    std::shared_ptr<const Basis> in_;
+    std::unique_ptr<Hamiltonian> rows_;   // out's rows, O's adjoint compiled
+    DevBuf jhi_in_, hicls_in_;
 };
 
 }  // namespace qk
