@@ -127,30 +127,35 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
     hipStream_t st = H.stream;
     const int nmax = int(std::min<i64>(i64(maxiter), i64(d)));
 
-    std::vector<DevBuf> V;
-    V.reserve(size_t(std::max(nmax, 1)) + 1);
+    // The basis lives in slabs of kSlab vectors, allocated as the run needs
+    // them (one hipMalloc per kSlab steps instead of one per step).
+    constexpr int kSlab = 8;
+    std::vector<DevBuf> slabs;
+    auto vptr = [&](int i) -> cplx* {
+        while (size_t(i / kSlab) >= slabs.size()) slabs.emplace_back(sizeof(cplx) * d * kSlab);
+        return slabs[size_t(i / kSlab)].get<cplx>() + u64(i % kSlab) * d;
+    };
     DevBuf w(sizeof(cplx) * d), red, sc(64 * sizeof(double)), h;
     DevBuf ptrs(sizeof(void*) * (size_t(std::max(nmax, 1)) + 1));
 
-    V.emplace_back(sizeof(cplx) * d);
     if (start) {
-        QK_HIP(hipMemcpyAsync(V[0].get(), start, sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(vptr(0), start, sizeof(cplx) * d, hipMemcpyHostToDevice, st));
     } else {
         const std::vector<cplx> s0 = uniform_start(d, 1234);
-        QK_HIP(hipMemcpyAsync(V[0].get(), s0.data(), sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(vptr(0), s0.data(), sizeof(cplx) * d, hipMemcpyHostToDevice, st));
         QK_HIP(hipStreamSynchronize(st));
     }
     {   // normalize (vector_ops.hpp:91-97: unchanged when the norm is 0)
-        const double nrm = dev_norm(V[0].get<cplx>(), d, red, sc, st);
-        if (nrm != 0.0) vec::scale(V[0].get<cplx>(), d, cplx{1.0 / nrm, 0.0}, st);
+        const double nrm = dev_norm(vptr(0), d, red, sc, st);
+        if (nrm != 0.0) vec::scale(vptr(0), d, cplx{1.0 / nrm, 0.0}, st);
     }
-    put_ptr(ptrs, 0, V[0].get(), st);
+    put_ptr(ptrs, 0, vptr(0), st);
 
     std::vector<double> alphas, betas;
     double e_old = 1e100;
     for (int iter = 0; iter < nmax; ++iter) {
         res.iterations = iter + 1;
-        cplx* v = V[iter].get<cplx>();
+        cplx* v = vptr(iter);
         H.apply(v, w.get<cplx>(), 1, d, st);
         vec::dot(v, w.get<cplx>(), d, sc.get<cplx>(), red, st);
         const double alpha = fetch<cplx>(sc.get<cplx>(), st).re;
@@ -159,20 +164,38 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
         sc.resize(64 * sizeof(double));
         double ab[2] = {alpha, bprev};
         QK_HIP(hipMemcpyAsync(sc.get<double>() + 8, ab, sizeof(ab), hipMemcpyHostToDevice, st));
-        vec::lanczos_sub(w.get<cplx>(), v, iter > 0 ? V[iter - 1].get<cplx>() : nullptr, d,
+        vec::lanczos_sub(w.get<cplx>(), v, iter > 0 ? vptr(iter - 1) : nullptr, d,
                          sc.get<double>() + 8, sc.get<double>() + 9, st);
-        cgs2(ptrs, iter + 1, w.get<cplx>(), d, h, red, st);
-        const double beta = dev_norm(w.get<cplx>(), d, red, sc, st);
+        // Full reorthogonalisation against every stored vector (the
+        // reference's MGS, :141-152) as classical Gram-Schmidt with the
+        // Daniel-Gragg-Kaufman-Stewart test: a second pass only when the
+        // first removed more than half of |w|^2 (the squared norm after each
+        // pass comes from the subtraction pass itself)
+        const int k = iter + 1;
+        h.resize(sizeof(cplx) * k);
+        double* nn = sc.get<double>() + 16;   // |w|^2 before, after pass 1, after pass 2
+        vec::norm2(w.get<cplx>(), d, nn, red, st);
+        vec::mdot(ptrs.get<const cplx*>(), k, w.get<cplx>(), d, h.get<cplx>(), red, st);
+        vec::msub_norm(ptrs.get<const cplx*>(), k, h.get<cplx>(), w.get<cplx>(), d, nn + 1, red, st);
+        double n01[2];
+        QK_HIP(hipMemcpyAsync(n01, nn, sizeof(n01), hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+        double w2 = n01[1];
+        if (!(n01[1] > 0.5 * n01[0])) {
+            vec::mdot(ptrs.get<const cplx*>(), k, w.get<cplx>(), d, h.get<cplx>(), red, st);
+            vec::msub_norm(ptrs.get<const cplx*>(), k, h.get<cplx>(), w.get<cplx>(), d, nn + 2, red, st);
+            w2 = fetch<double>(nn + 2, st);
+        }
+        const double beta = std::sqrt(w2);
         if (beta < 1e-15) {
             res.converged = true;
             break;
         }
         if (iter + 1 == nmax) break;
         betas.push_back(beta);
-        V.emplace_back(sizeof(cplx) * d);
-        QK_HIP(hipMemcpyAsync(V.back().get(), w.get(), sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
-        vec::scale(V.back().get<cplx>(), d, cplx{1.0 / beta, 0.0}, st);
-        put_ptr(ptrs, iter + 1, V.back().get(), st);
+        cplx* vn = vptr(iter + 1);
+        vec::scale_copy(w.get<cplx>(), vn, d, cplx{1.0 / beta, 0.0}, st);
+        put_ptr(ptrs, iter + 1, vn, st);
         if (iter > 0) {
             const double e = tridiag_ground(alphas, betas, int(alphas.size()), nullptr);
             if (std::abs(e - e_old) < tol) {

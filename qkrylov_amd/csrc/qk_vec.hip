@@ -105,6 +105,11 @@ __global__ void k_scale(cplx* __restrict__ x, u64 n, cplx s) {
     if (i < n) x[i] = cmuld(x[i], s);
 }
 
+__global__ void k_scale_copy(const cplx* __restrict__ x, cplx* __restrict__ y, u64 n, cplx s) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i < n) y[i] = cmuld(x[i], s);
+}
+
 __global__ void k_scale_to(const cplx* __restrict__ x, cplx* __restrict__ y, u64 n,
                            const double* __restrict__ s) {
     const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
@@ -172,19 +177,50 @@ __global__ void __launch_bounds__(kT) k_mdot_part(const cplx* const* __restrict_
     }
 }
 
-__global__ void __launch_bounds__(kT) k_msub(const cplx* const* __restrict__ V, int k,
-                                             const cplx* __restrict__ h, cplx* __restrict__ w,
-                                             u64 n) {
-    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
-    if (i >= n) return;
-    cplx r = w[i];
-    for (int j = 0; j < k; ++j) {
-        const cplx x = V[j][i];
-        const cplx c = h[j];
-        r.re -= c.re * x.re - c.im * x.im;
-        r.im -= c.re * x.im + c.im * x.re;
+// w -= sum_j V_j h[j] over one chunk of kCH entries per block, with the
+// block's partial of |w|^2 after the update when part is non-null (fixed
+// order, deterministic).  Vectors outer, the thread's kPer entries inner: the
+// kPer loads of one vector are independent and in flight together.
+template <int PER>
+__global__ void __launch_bounds__(kT) k_msub_chunk(const cplx* const* __restrict__ V, int k,
+                                                   const cplx* __restrict__ h,
+                                                   cplx* __restrict__ w, u64 n,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[4];
+    const u64 start = u64(blockIdx.x) * (PER * kT);
+    const bool full = start + PER * kT <= n;
+    cplx r[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const u64 i = start + u64(e) * kT + threadIdx.x;
+        r[e] = (full || i < n) ? w[i] : cplx{0.0, 0.0};
     }
-    w[i] = r;
+    for (int j = 0; j < k; ++j) {
+        const cplx* vj = V[j];
+        const cplx c = h[j];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const u64 i = start + u64(e) * kT + threadIdx.x;
+            if (full || i < n) {
+                const cplx x = vj[i];
+                r[e].re -= c.re * x.re - c.im * x.im;
+                r[e].im -= c.re * x.im + c.im * x.re;
+            }
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const u64 i = start + u64(e) * kT + threadIdx.x;
+        if (full || i < n) {
+            w[i] = r[e];
+            acc += r[e].re * r[e].re + r[e].im * r[e].im;
+        }
+    }
+    if (part) {
+        acc = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    }
 }
 
 template <int NO>
@@ -473,6 +509,29 @@ inline u32 ngrid(u64 n) { return static_cast<u32>((n + kT - 1) / kT); }
 
 }  // namespace
 
+// entries per thread: the largest power of two up to kPer that still gives
+// about 1024 blocks (short vectors keep the chip covered)
+static int per_thread(u64 n) {
+    int per = kPer;
+    while (per > 1 && (n + u64(per) * kT - 1) / (u64(per) * kT) < 1024) per >>= 1;
+    return per;
+}
+
+static void launch_msub(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, double* part,
+                        u32* nb_out, hipStream_t st) {
+    const int per = per_thread(n);
+    const u64 chunk = u64(per) * kT;
+    const u32 nb = std::max<u32>(1, u32((n + chunk - 1) / chunk));
+    if (nb_out) *nb_out = nb;
+    switch (per) {
+        case 16: hipLaunchKernelGGL(k_msub_chunk<16>, dim3(nb), dim3(kT), 0, st, V, k, h, w, n, part); break;
+        case 8: hipLaunchKernelGGL(k_msub_chunk<8>, dim3(nb), dim3(kT), 0, st, V, k, h, w, n, part); break;
+        case 4: hipLaunchKernelGGL(k_msub_chunk<4>, dim3(nb), dim3(kT), 0, st, V, k, h, w, n, part); break;
+        case 2: hipLaunchKernelGGL(k_msub_chunk<2>, dim3(nb), dim3(kT), 0, st, V, k, h, w, n, part); break;
+        default: hipLaunchKernelGGL(k_msub_chunk<1>, dim3(nb), dim3(kT), 0, st, V, k, h, w, n, part); break;
+    }
+}
+
 void Segments::build(const std::vector<u64>& lens, hipStream_t st) {
     nsamp = static_cast<int>(lens.size());
     len = lens;
@@ -519,6 +578,12 @@ void norm2(const cplx* a, u64 n, double* out, DevBuf& scratch, hipStream_t st) {
     QK_HIP(hipGetLastError());
 }
 
+void scale_copy(const cplx* x, cplx* y, u64 n, cplx s, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_scale_copy, dim3(ngrid(n)), dim3(kT), 0, st, x, y, n, s);
+    QK_HIP(hipGetLastError());
+}
+
 void scale(cplx* x, u64 n, cplx s, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_scale, dim3(ngrid(n)), dim3(kT), 0, st, x, n, s);
@@ -547,9 +612,7 @@ void lanczos_sub(cplx* w, const cplx* v, const cplx* u, u64 n, const double* alp
 void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& scratch,
           hipStream_t st) {
     if (k <= 0) return;
-    // the largest chunk that still gives about 1024 blocks (at least 256 entries)
-    int per = kPer;
-    while (per > 1 && (n + u64(per) * kT - 1) / (u64(per) * kT) < 1024) per >>= 1;
+    const int per = per_thread(n);
     const u64 chunk = u64(per) * kT;
     const u32 nb = std::max<u32>(1, u32((n + chunk - 1) / chunk));
     scratch.resize(sizeof(cplx) * nb * size_t(k));
@@ -605,7 +668,19 @@ void block_sub(const cplx* const* V, int m, cplx* const* W, int nw, const cplx* 
 
 void msub(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, hipStream_t st) {
     if (k <= 0 || !n) return;
-    hipLaunchKernelGGL(k_msub, dim3(ngrid(n)), dim3(kT), 0, st, V, k, h, w, n);
+    launch_msub(V, k, h, w, n, nullptr, nullptr, st);
+    QK_HIP(hipGetLastError());
+}
+
+void msub_norm(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, double* out,
+               DevBuf& scratch, hipStream_t st) {
+    if (!n) return;
+    const int per = per_thread(n);
+    const u32 nbmax = std::max<u32>(1, u32((n + u64(per) * kT - 1) / (u64(per) * kT)));
+    scratch.resize(sizeof(double) * nbmax);
+    u32 nb = 0;
+    launch_msub(V, k, h, w, n, scratch.get<double>(), &nb, st);
+    hipLaunchKernelGGL(k_finish_r, dim3(1), dim3(kT), 0, st, scratch.get<double>(), nb, out);
     QK_HIP(hipGetLastError());
 }
 

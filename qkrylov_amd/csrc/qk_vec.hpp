@@ -36,6 +36,8 @@ void dot(const cplx* a, const cplx* b, u64 n, cplx* out, DevBuf& scratch, hipStr
 void norm2(const cplx* a, u64 n, double* out, DevBuf& scratch, hipStream_t st);
 // x *= s (host scalar)
 void scale(cplx* x, u64 n, cplx s, hipStream_t st);
+// y = s x  (host scalar)
+void scale_copy(const cplx* x, cplx* y, u64 n, cplx s, hipStream_t st);
 // y = s * x (s on device, real)
 void scale_to(const cplx* x, cplx* y, u64 n, const double* s_dev, hipStream_t st);
 // y += a x  (host scalar)
@@ -54,6 +56,10 @@ void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n,
 // is read once for all W_j
 void block_sub(const cplx* const* V, int m, cplx* const* W, int nw, const cplx* G, u64 n,
                hipStream_t st);
+// w -= sum_j V_j h[j] and *out = |w|^2 after the update (device double), in
+// one pass over the vectors
+void msub_norm(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, double* out,
+               DevBuf& scratch, hipStream_t st);
 // w -= sum_j V_j h[j]
 void msub(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, hipStream_t st);
 // out = sum_j c[j] V_j  (c on device)
