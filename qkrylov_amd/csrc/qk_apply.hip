@@ -1157,24 +1157,39 @@ bool MultiApply::build(const std::vector<Group>& groups, hipStream_t st) {
     }
     std::vector<MTile> mt;
     std::vector<const u64*> jh;
-    for (size_t s = 0; s < gs.size(); ++s) {
-        const Group& g = *gs[s];
-        jh.push_back(g.h->xparams.jhi);
-        for (const XTile& T : g.h->xtiles_for(64)) {
-            MTile m{};
-            m.base = T.base;
-            m.xoff = g.off;
-            m.ld = g.h->dim();
-            m.pos0 = T.pos0;
-            m.hi = T.hi;
-            m.r0 = T.r0;
-            m.nvalid = T.nvalid;
-            m.sec = u32(s);
-            m.nvec = u32(g.nvec);
-            m.dh = T.dh;
-            mt.push_back(m);
+    // vblk > 0: one tile per (block of vblk vectors, tile), all tiles of the
+    // first block first, so the grid gathers from one block of vectors at a
+    // time and its rows stay in L2 between neighbouring tiles (N=20 FTLM,
+    // 64 vectors per sector: 8 per block 0.436 s per sweep against 0.551 s
+    // for all 64 per tile; 4: 0.448 s, 16: 0.457 s, 32: 0.467 s); 0: one
+    // tile with all of a group's vectors
+    int vblk = 8;
+    if (const char* e = std::getenv("QKRYLOV_MULTI_VBLOCK")) vblk = std::atoi(e);
+    int maxv = 0;
+    for (const Group* g : gs) maxv = std::max(maxv, g->nvec);
+    const int nvb = vblk > 0 ? (maxv + vblk - 1) / vblk : 1;
+    for (size_t s = 0; s < gs.size(); ++s) jh.push_back(gs[s]->h->xparams.jhi);
+    for (int b = 0; b < nvb; ++b)
+        for (size_t s = 0; s < gs.size(); ++s) {
+            const Group& g = *gs[s];
+            const int v0 = vblk > 0 ? b * vblk : 0;
+            const int nv = vblk > 0 ? std::min(vblk, g.nvec - v0) : g.nvec;
+            if (nv <= 0) continue;
+            for (const XTile& T : g.h->xtiles_for(64)) {
+                MTile m{};
+                m.base = T.base;
+                m.xoff = g.off + u64(v0) * g.h->dim();
+                m.ld = g.h->dim();
+                m.pos0 = T.pos0;
+                m.hi = T.hi;
+                m.r0 = T.r0;
+                m.nvalid = T.nvalid;
+                m.sec = u32(s);
+                m.nvec = u32(nv);
+                m.dh = T.dh;
+                mt.push_back(m);
+            }
         }
-    }
     upload(tiles_, mt.data(), mt.size(), st);
     upload(jhis_, jh.data(), jh.size(), st);
     QK_HIP(hipStreamSynchronize(st));
