@@ -327,8 +327,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every box of this build has one GPU and the H·x path "
-              "runs on one device; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the H·x path runs on one device (sharding across GPUs "
+              "is described, not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch

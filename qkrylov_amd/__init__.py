@@ -506,7 +506,7 @@ class MatrixFreeHamiltonian:
         if site is None:
             site = site_for(basis)
         if str(device).lower() not in ("hip", "gpu", "cuda", "rocm"):
-            raise ValueError(f"device {device!r}: this build runs on the GPU only (use 'hip' or 'gpu')")
+            raise ValueError(f"device {device!r}: this library runs on the GPU only (use 'hip' or 'gpu')")
         if dtype not in (np.float32, np.float64):
             raise ValueError("dtype must be np.float32 or np.float64")
         self.device, self.dtype = str(device), dtype
