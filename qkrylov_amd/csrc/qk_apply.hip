@@ -590,6 +590,7 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
     const u32 lane = threadIdx.x & 63;
     const u32 hi_end = p.n_lo + p.n_hi;
     const u32 mix_end = hi_end + p.n_mix;
+    double dot = 0.0;   // Re<x, y> over this lane's states (used with dot_part)
 
     QK_FOR_TILES_W(p, wave, t, kXWaves) {
         const XTile T = p.xtiles[t];
@@ -598,7 +599,7 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
         if (lane >= T.nvalid) continue;
         u32 lo[SPL], r[SPL];
         bool v[SPL];
-        cplx acc[SPL];
+        cplx acc[SPL], xa[SPL];
 #pragma unroll
         for (int k = 0; k < SPL; ++k) {
             const u32 j = lane + 64u * k;
@@ -613,9 +614,9 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
                     dg += s_tab[G.tab + config_of(s, G)].re;
                 }
             }
-            cplx xa{0.0, 0.0};
-            if (v[k]) xa = x[base + r[k]];
-            acc[k] = cplx{dg * xa.re, dg * xa.im};
+            xa[k] = cplx{0.0, 0.0};
+            if (v[k]) xa[k] = x[base + r[k]];
+            acc[k] = cplx{dg * xa[k].re, dg * xa[k].im};
         }
         u32 b = 0;
         for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB>(p.bonds + b, s_jlo, x, base, lo, v, acc);
@@ -627,7 +628,22 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
         for (; b < mix_end; ++b) xbonds_mix<SPL, 1>(p.bonds + b, p.jhi, s_jlo, hi, p.L, x, lo, v, acc);
 #pragma unroll
         for (int k = 0; k < SPL; ++k)
-            if (v[k]) store_nt(&y[base + r[k]], acc[k]);
+            if (v[k]) {
+                store_nt(&y[base + r[k]], acc[k]);
+                dot += xa[k].re * acc[k].re + xa[k].im * acc[k].im;
+            }
+    }
+    if (p.dot_part) {
+        // this workgroup's Re<x, y>, reduced in a fixed order
+        __shared__ double s_dot[kXWaves];
+        for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off);
+        if (lane == 0) s_dot[wave] = dot;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tsum = 0.0;
+            for (int w = 0; w < kXWaves; ++w) tsum += s_dot[w];
+            p.dot_part[blockIdx.x] = tsum;
+        }
     }
 }
 
@@ -1226,6 +1242,18 @@ Hamiltonian::~Hamiltonian() {
         (void)hipStreamSynchronize(stream);
         (void)hipStreamDestroy(stream);
     }
+}
+
+bool Hamiltonian::apply_dot(const cplx* x, cplx* y, double* part, hipStream_t st) const {
+    if (!xchg || !state32 || !xs_fn || dim() == 0) return false;
+    if (!st) st = stream;
+    XParams q = xparams_s;
+    q.dot_part = part;
+    const cplx* xa = x;
+    cplx* ya = y;
+    void* args[] = {&q, &xa, &ya};
+    QK_HIP(hipLaunchKernel(xs_fn, dim3(xgrid_s), dim3(64 * kXWaves), args, size_t(xlds_bytes), st));
+    return true;
 }
 
 void Hamiltonian::apply(const cplx* x, cplx* y, int nvec, u64 ld, hipStream_t st) const {

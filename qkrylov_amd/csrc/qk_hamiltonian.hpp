@@ -69,6 +69,9 @@ struct XParams {
     u32 nlo;
     int L;
     u32 tiles_per_xcd;
+    // single-vector kernel only: when set, workgroup b also writes
+    // dot_part[b] = sum over its states of Re(conj(x_a) y_a)
+    double* dot_part;
 };
 
 // Tile of the multi-sector exchange kernel: 64 ranks of one row of sector
@@ -130,6 +133,16 @@ struct Hamiltonian {
 
     // y[v*ld + a] = sum_b H^(gather)[a,b] x[v*ld + b], v < nvec, device pointers.
     void apply(const cplx* x, cplx* y, int nvec, u64 ld, hipStream_t st) const;
+    // One vector on the single-vector exchange kernel, also writing
+    // dot_parts() partial sums of Re<x, y> into part; false (nothing launched)
+    // when that kernel is not this Hamiltonian's single-vector path.
+    bool apply_dot(const cplx* x, cplx* y, double* part, hipStream_t st) const;
+    int dot_parts() const { return xgrid_s; }
+    // apply_dot is available and apply() of nvec vectors runs that same
+    // kernel vector by vector
+    bool fused_dot(int nvec) const {
+        return xchg && state32 && xs_fn && dim() > 0 && (vec_group == 1 || nvec == 1);
+    }
     // Full complex diagonal in OpSum order (device output, dim entries).
     void diagonal(cplx* d, hipStream_t st) const;
     // phi[k*ld + a] = N^-1/2 sum_j e^{i 2 pi k j / N} Sz_j psi[a], k < nq.

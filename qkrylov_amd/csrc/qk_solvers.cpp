@@ -735,6 +735,8 @@ struct BlockRec {
     std::vector<Grp> grps;
     Segments seg;
     DevBuf bufU, bufY, bufP, c, cp, bp, alive, alpha, beta, nrm;
+    DevBuf dparts, dcount;   // alpha partials written by the apply kernel
+    u32 dstride = 0;         // 0: alpha from a separate pass over U and Y
     int nsamp = 0;
     hipStream_t st = nullptr;
     MultiApply multi;   // all groups in one apply launch when they allow it
@@ -764,6 +766,26 @@ struct BlockRec {
         }
         const char* e = std::getenv("QKRYLOV_MULTI_APPLY");
         if (!(e && std::string(e) == "0") && ntot > 1 && dmax < (u64(1) << 22)) multi.build(mg, st);
+        // Per-vector single-vector launches can return alpha's partial sums
+        // themselves (the kernel holds x and y when it stores), which saves
+        // reading both vectors again every step.
+        dstride = 0;
+        const char* f = std::getenv("QKRYLOV_FUSED_ALPHA");
+        bool fused = !multi.ok() && !(f && std::string(f) == "0") && !grps.empty();
+        for (const Grp& x : grps)
+            if (!(x.h->fused_dot(x.nvec) || x.h->dim() == 0)) fused = false;
+        if (fused) {
+            std::vector<u32> cnt;
+            for (const Grp& x : grps) {
+                dstride = std::max<u32>(dstride, u32(x.h->dot_parts()));
+                for (int r = 0; r < x.nvec; ++r) cnt.push_back(x.h->dim() ? u32(x.h->dot_parts()) : 0u);
+            }
+            dstride = std::max<u32>(dstride, 1);
+            const size_t pb = sizeof(double) * size_t(dstride) * std::max<size_t>(cnt.size(), 1);
+            dparts.resize(pb);
+            QK_HIP(hipMemsetAsync(dparts.get(), 0, pb, st));
+            upload(dcount, cnt.data(), cnt.size(), st);
+        }
         const size_t bytes = sizeof(cplx) * std::max<u64>(seg.total, 1);
         bufU.resize(bytes);
         bufY.resize(bytes);
@@ -812,12 +834,25 @@ struct BlockRec {
             cplx* Y = bufY.get<cplx>();
             cplx* P = bufP.get<cplx>();
             for (int step = 0; step < M; ++step) {
-                if (multi.ok()) multi.launch(U, Y, st);
-                else
-                    for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
                 double* a = alpha.get<double>() + size_t(step) * nsamp;
                 double* b = beta.get<double>() + size_t(step) * nsamp;
-                vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);
+                if (dstride) {
+                    size_t j = 0;
+                    for (auto& g : grps)
+                        for (int r = 0; r < g.nvec; ++r, ++j) {
+                            const u64 o = g.off + u64(r) * g.h->dim();
+                            double* part = dparts.get<double>() + j * dstride;
+                            if (g.h->dim() && !g.h->apply_dot(U + o, Y + o, part, st))
+                                fail("block recurrence: fused apply not available");
+                        }
+                    vec::alpha_from_parts(dparts.get<double>(), dstride, dcount.get<u32>(), nsamp,
+                                          c.get<double>(), a, alive.get<int>(), st);
+                } else {
+                    if (multi.ok()) multi.launch(U, Y, st);
+                    else
+                        for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
+                    vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);
+                }
                 vec::seg_update(seg, Y, U, P, c.get<double>(), cp.get<double>(), bp.get<double>(), a,
                                 b, alive.get<int>(), step == 0 ? 1 : 0, st);
                 cplx* oldP = P;

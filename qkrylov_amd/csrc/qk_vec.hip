@@ -303,6 +303,21 @@ __global__ void __launch_bounds__(kT) k_seg_alpha_fin(const u32* __restrict__ fi
     if (threadIdx.x == 0 && alive[j]) alpha[j] = cs[j] * cs[j] * s;
 }
 
+// alpha[j] = c_j^2 * sum_{b < count[j]} part[j*stride + b] for live samples
+// (partials written by the apply kernel itself)
+__global__ void __launch_bounds__(kT) k_alpha_from_parts(const double* __restrict__ part, u32 stride,
+                                                         const u32* __restrict__ count,
+                                                         const double* __restrict__ cs,
+                                                         double* __restrict__ alpha,
+                                                         const int* __restrict__ alive) {
+    __shared__ double sh[4];
+    const u32 j = blockIdx.x;
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < count[j]; b += kT) s += part[u64(j) * stride + b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0 && alive[j]) alpha[j] = cs[j] * cs[j] * s;
+}
+
 __global__ void __launch_bounds__(kT) k_seg_update_part(
     const u64* __restrict__ coff, const u32* __restrict__ clen, const u32* __restrict__ csamp,
     cplx* __restrict__ Y, const cplx* __restrict__ U, const cplx* __restrict__ P,
@@ -722,6 +737,14 @@ void seg_alpha(Segments& seg, const cplx* U, const cplx* Y, const double* c, dou
                        seg.d_chunk_sample.get<u32>(), U, Y, alive, seg.partials.get<double>());
     hipLaunchKernelGGL(k_seg_alpha_fin, dim3(seg.nsamp), dim3(kT), 0, st, seg.d_first.get<u32>(),
                        seg.d_count.get<u32>(), seg.partials.get<double>(), c, alpha_out, alive);
+    QK_HIP(hipGetLastError());
+}
+
+void alpha_from_parts(const double* part, u32 stride, const u32* count, int nsamp, const double* c,
+                      double* alpha_out, const int* alive, hipStream_t st) {
+    if (nsamp <= 0) return;
+    hipLaunchKernelGGL(k_alpha_from_parts, dim3(nsamp), dim3(kT), 0, st, part, stride, count, c,
+                       alpha_out, alive);
     QK_HIP(hipGetLastError());
 }
 

@@ -79,6 +79,9 @@ void jacobi_precond(const cplx* r, const cplx* d, double lambda, cplx* t, u64 n,
 // alpha_j = c_j^2 Re<U_j, Y_j>
 void seg_alpha(Segments& seg, const cplx* U, const cplx* Y, const double* c, double* alpha_out,
                const int* alive, hipStream_t st);
+// alpha[j] = c_j^2 * (sum of count[j] partials at part + j*stride), live j only
+void alpha_from_parts(const double* part, u32 stride, const u32* count, int nsamp, const double* c,
+                      double* alpha_out, const int* alive, hipStream_t st);
 // W = c Y - alpha c U - beta_prev cp P (written into Y), beta_j = ||W_j||;
 // then cp <- c, c <- 1/beta (0 and alive = 0 when beta < 1e-15), beta_prev <- beta.
 void seg_update(Segments& seg, cplx* Y, const cplx* U, const cplx* P, double* c, double* cp,
