@@ -438,10 +438,11 @@ k_apply_xchg_multi(const XParams p, const MParams mp, const cplx* __restrict__ x
 }
 
 // Bond batches of the single-vector exchange kernel.  Every load of a batch
-// (bond descriptors, row offsets, LDS ranks, the SPL gathers of all NB bonds)
-// is issued before the first one is used, so a wave keeps NB*SPL gathers in
-// flight instead of waiting out one bond at a time.  Contributions are added
-// in bond order, as in the one-bond loop.
+// (bond records, LDS ranks, the SPL gathers of all NB bonds) is issued before
+// the first one is used, and none of them sits under a lane-dependent branch
+// (a lane without a contribution reads rank 0 of its own row and adds zero),
+// so a wave keeps NB*SPL gathers in flight instead of waiting out one bond at
+// a time.  Contributions are added in bond order, as in the one-bond loop.
 
 // low-half bonds: same row, rank from the LDS table
 template <int SPL, int NB>
@@ -462,7 +463,8 @@ __device__ __forceinline__ void xbonds_lo(const XBond* __restrict__ bonds, const
             const u32 vi = (lo[k] >> X[u].bi) & 1, vj = (lo[k] >> X[u].bj) & 1;
             ok[u][k] = v[k] && vi != vj;
             amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
-            idx[u][k] = ok[u][k] ? u32(s_jlo[lo[k] ^ X[u].mlo]) : 0u;
+            const u32 t = s_jlo[lo[k] ^ X[u].mlo];   // lo ^ mlo < 2^L: in the table
+            idx[u][k] = ok[u][k] ? t : 0u;
         }
     cplx xv[NB][SPL];
 #pragma unroll
@@ -481,80 +483,66 @@ __device__ __forceinline__ void xbonds_lo(const XBond* __restrict__ bonds, const
         }
 }
 
-// high-half bonds: another row (wave-uniform), same ranks.  An exchange keeps
-// the popcount of the high half, so the partner row of a valid row is valid.
+// high-half bonds: the row's active bonds (XRowNb, wave-uniform), same ranks
+// in the partner row.  A lane past the tile's end reads its wave's first rank.
 template <int SPL, int NB>
-__device__ __forceinline__ void xbonds_hi(const XBond* __restrict__ bonds, const u64* __restrict__ jhi,
-                                          const u32 hi, const u32 L, const cplx* __restrict__ x,
+__device__ __forceinline__ void xbonds_hi(const XRowNb* __restrict__ nb, const cplx* __restrict__ x,
                                           const u32 (&r)[SPL], const bool (&v)[SPL],
                                           cplx (&acc)[SPL]) {
-    XBond X[NB];
+    XRowNb E[NB];
 #pragma unroll
-    for (int u = 0; u < NB; ++u) X[u] = bonds[u];
-    u64 b2[NB];
-    bool on[NB];
-    double amp[NB];
-#pragma unroll
-    for (int u = 0; u < NB; ++u) {
-        const u32 vi = (hi >> (X[u].bi - L)) & 1, vj = (hi >> (X[u].bj - L)) & 1;
-        on[u] = vi != vj;
-        amp[u] = vi ? X[u].a10 : X[u].a01;
-        b2[u] = jhi[hi ^ X[u].mhi];
-    }
+    for (int u = 0; u < NB; ++u) E[u] = nb[u];
     cplx xv[NB][SPL];
 #pragma unroll
     for (int u = 0; u < NB; ++u)
 #pragma unroll
-        for (int k = 0; k < SPL; ++k) {
-            xv[u][k] = cplx{0.0, 0.0};
-            if (on[u] && v[k]) xv[u][k] = x[b2[u] + r[k]];
-        }
+        for (int k = 0; k < SPL; ++k) xv[u][k] = x[E[u].base + (v[k] ? r[k] : r[0])];
 #pragma unroll
     for (int u = 0; u < NB; ++u)
-        if (on[u]) {
 #pragma unroll
-            for (int k = 0; k < SPL; ++k) {
-                acc[k].re += amp[u] * xv[u][k].re;
-                acc[k].im += amp[u] * xv[u][k].im;
-            }
+        for (int k = 0; k < SPL; ++k) {
+            acc[k].re += E[u].amp * xv[u][k].re;
+            acc[k].im += E[u].amp * xv[u][k].im;
         }
 }
 
-// bonds straddling the split: another row and another low-half class
+// bonds straddling the split: another row (offset from the per-row record)
+// and another low-half class
 template <int SPL, int NB>
-__device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, const u64* __restrict__ jhi,
+__device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, const u64* __restrict__ mnb,
                                            const u16* s_jlo, const u32 hi, const u32 L,
-                                           const cplx* __restrict__ x, const u32 (&lo)[SPL],
-                                           const bool (&v)[SPL], cplx (&acc)[SPL]) {
+                                           const cplx* __restrict__ x, const u64 base,
+                                           const u32 (&lo)[SPL], const bool (&v)[SPL],
+                                           cplx (&acc)[SPL]) {
     XBond X[NB];
-#pragma unroll
-    for (int u = 0; u < NB; ++u) X[u] = bonds[u];
     u64 b2[NB];
-    u32 vj[NB];
 #pragma unroll
     for (int u = 0; u < NB; ++u) {
-        vj[u] = (hi >> (X[u].bj - L)) & 1;
-        b2[u] = jhi[hi ^ X[u].mhi];
+        X[u] = bonds[u];
+        b2[u] = mnb[u];
     }
     double amp[NB][SPL];
-    u32 idx[NB][SPL];
+    u64 idx[NB][SPL];
     bool ok[NB][SPL];
 #pragma unroll
-    for (int u = 0; u < NB; ++u)
+    for (int u = 0; u < NB; ++u) {
+        const u32 vj = (hi >> (X[u].bj - L)) & 1;
 #pragma unroll
         for (int k = 0; k < SPL; ++k) {
             const u32 vi = (lo[k] >> X[u].bi) & 1;
-            ok[u][k] = v[k] && b2[u] != ~u64(0) && vi != vj[u];
+            ok[u][k] = v[k] && b2[u] != ~u64(0) && vi != vj;
             amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
-            idx[u][k] = ok[u][k] ? u32(s_jlo[lo[k] ^ X[u].mlo]) : 0u;
+            const u32 t = s_jlo[lo[k] ^ X[u].mlo];
+            idx[u][k] = ok[u][k] ? b2[u] + t : base;
         }
+    }
     cplx xv[NB][SPL];
 #pragma unroll
     for (int u = 0; u < NB; ++u)
 #pragma unroll
         for (int k = 0; k < SPL; ++k) {
             xv[u][k] = cplx{0.0, 0.0};
-            if (ok[u][k]) xv[u][k] = x[b2[u] + idx[u][k]];
+            if (ok[u][k]) xv[u][k] = x[idx[u][k]];
         }
 #pragma unroll
     for (int u = 0; u < NB; ++u)
@@ -621,11 +609,19 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
         u32 b = 0;
         for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB>(p.bonds + b, s_jlo, x, base, lo, v, acc);
         for (; b < p.n_lo; ++b) xbonds_lo<SPL, 1>(p.bonds + b, s_jlo, x, base, lo, v, acc);
-        for (; b + NB <= hi_end; b += NB) xbonds_hi<SPL, NB>(p.bonds + b, p.jhi, hi, p.L, x, r, v, acc);
-        for (; b < hi_end; ++b) xbonds_hi<SPL, 1>(p.bonds + b, p.jhi, hi, p.L, x, r, v, acc);
+        {
+            const u32 hc = p.hcnt[hi];
+            const XRowNb* nb = p.hnb + size_t(hi) * p.n_hi;
+            u32 e = 0;
+            for (; e + NB <= hc; e += NB) xbonds_hi<SPL, NB>(nb + e, x, r, v, acc);
+            for (; e < hc; ++e) xbonds_hi<SPL, 1>(nb + e, x, r, v, acc);
+        }
+        b = hi_end;
+        const u64* mnb = p.mnb + size_t(hi) * p.n_mix;
         for (; b + NB <= mix_end; b += NB)
-            xbonds_mix<SPL, NB>(p.bonds + b, p.jhi, s_jlo, hi, p.L, x, lo, v, acc);
-        for (; b < mix_end; ++b) xbonds_mix<SPL, 1>(p.bonds + b, p.jhi, s_jlo, hi, p.L, x, lo, v, acc);
+            xbonds_mix<SPL, NB>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
+        for (; b < mix_end; ++b)
+            xbonds_mix<SPL, 1>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
 #pragma unroll
         for (int k = 0; k < SPL; ++k)
             if (v[k]) {
@@ -978,7 +974,35 @@ void Hamiltonian::setup_exchange() {
     upload(d_dmixtab, mixtab.data(), mixtab.size(), stream);
     QK_HIP(hipStreamSynchronize(stream));
 
+    // per-row bond records of the single-vector kernel
+    {
+        const std::vector<u64>& jh = B.jhi();
+        const size_t nrow = jh.size(), nh = hi_b.size(), nm = mix_b.size();
+        std::vector<XRowNb> hnb(std::max<size_t>(nrow * nh, 1), XRowNb{0, 0.0});
+        std::vector<u32> hcnt(std::max<size_t>(nrow, 1), 0u);
+        std::vector<u64> mnb(std::max<size_t>(nrow * nm, 1), ~u64(0));
+        for (size_t h = 0; h < nrow; ++h) {
+            if (jh[h] == ~u64(0)) continue;
+            u32 c = 0;
+            for (const XBond& X : hi_b) {
+                const u32 vi = u32(h >> (X.bi - L)) & 1, vj = u32(h >> (X.bj - L)) & 1;
+                if (vi == vj) continue;
+                const u64 b2 = jh[h ^ X.mhi];
+                if (b2 == ~u64(0)) fail("exchange path: high-half partner row missing");
+                hnb[h * nh + c++] = XRowNb{b2, vi ? X.a10 : X.a01};
+            }
+            hcnt[h] = c;
+            for (size_t m = 0; m < nm; ++m) mnb[h * nm + m] = jh[h ^ mix_b[m].mhi];
+        }
+        upload(d_hnb, hnb.data(), hnb.size(), stream);
+        upload(d_hcnt, hcnt.data(), hcnt.size(), stream);
+        upload(d_mnb, mnb.data(), mnb.size(), stream);
+    }
+
     XParams& x = xparams;
+    x.hnb = d_hnb.get<XRowNb>();
+    x.hcnt = d_hcnt.get<u32>();
+    x.mnb = d_mnb.get<u64>();
     x.jlo = params.jlo;
     x.jhi = params.jhi;
     x.hi_cls = params.hi_cls;
@@ -1011,7 +1035,7 @@ void Hamiltonian::setup_exchange() {
             QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
     }
     // defaults (measured on the N=28 chain); autotune() may change them
-    int s0 = 2, nb0 = 1;
+    int s0 = 2, nb0 = 4;
     vec_group = 2;
     if (const char* e = std::getenv("QKRYLOV_SPL")) s0 = std::atoi(e);
     if (const char* e = std::getenv("QKRYLOV_BOND_BATCH")) nb0 = std::atoi(e);

@@ -51,6 +51,16 @@ struct XTile {
 };
 static_assert(sizeof(XTile) == 32, "XTile layout");
 
+// Active high-half bond of one row for the single-vector exchange kernel:
+// the partner row's offset jhi[hi ^ mhi] and the amplitude for this row's
+// bit pattern.  Rows keep their active bonds in bond order, so a wave reads
+// them with independent scalar loads rather than a bond -> row-offset chain.
+struct XRowNb {
+    u64 base;
+    double amp;
+};
+static_assert(sizeof(XRowNb) == 16, "XRowNb layout");
+
 struct XParams {
     const XTile* xtiles;
     const u16* jlo;
@@ -72,6 +82,10 @@ struct XParams {
     // single-vector kernel only: when set, workgroup b also writes
     // dot_part[b] = sum over its states of Re(conj(x_a) y_a)
     double* dot_part;
+    // single-vector kernel only: per-row bond records
+    const XRowNb* hnb;      // row hi: hnb[hi * n_hi + e], e < hcnt[hi]
+    const u32* hcnt;        // active high-half bonds per row
+    const u64* mnb;         // mnb[hi * n_mix + m] = jhi[hi ^ mhi] of mixed bond m
 };
 
 // Tile of the multi-sector exchange kernel: 64 ranks of one row of sector
@@ -167,6 +181,7 @@ struct Hamiltonian {
     DevBuf d_jlo, d_jhi, d_hicls, d_clsoff, d_clscnt, d_lolist, d_tiles, d_groups, d_tables;
     DevBuf d_dgroups, d_dtables;   // per-term diagonal program
     DevBuf d_xbonds, d_dlo, d_dhi, d_dmix, d_dmixtab, d_xtiles, d_xtiles_s;   // exchange path
+    DevBuf d_hnb, d_hcnt, d_mnb;   // per-row bond records (single-vector kernel)
     XParams xparams_s{};        // single-vector kernel, spl states per lane
     int xgrid_s = 0;
     int spl = 0;                   // states per lane of the single-vector kernel
