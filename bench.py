@@ -224,6 +224,9 @@ def bench_davidson_tj(qk, lat, msub=48, max_iter=400):
         return qk.MatrixFreeHamiltonian(qk.TJBasis(lx * ly, sector=sec), qk.TJSite(), ops)
 
     Hs, H = tj(3, 4, 5), tj(4, 4, 7)
+    # untimed: kernel choice (row-program or group loop, vectors per launch)
+    tuning = {"3x4": Hs.autotune(8), "4x4": H.autotune(8)}
+    log(f"davidson autotune: {tuning}")
     net = qk.MLPPreconditioner(7)
     t = time.perf_counter()
     hist = net.train(Hs, n_eig=8, max_subspace=48, max_iter=30, max_pairs=192, epochs=20,
@@ -232,7 +235,7 @@ def bench_davidson_tj(qk, lat, msub=48, max_iter=400):
     log(f"mlp training on 3x4: {t_train:.1f} s {net.last_train}")
     out = {"workload": "t-J 4x4 periodic, t=1, J=0.4, 7 up 7 down, d=411,840; lowest 8, "
                        f"residual 1e-8, max_subspace {msub}, at most {max_iter} iterations",
-           "dim": H.dimension, "train_s": t_train, "train": net.last_train,
+           "dim": H.dimension, "train_s": t_train, "train": net.last_train, "autotune": tuning,
            "loss_history": [float(x) for x in hist]}
     res = {}
     for name, pre in (("jacobi", None), ("neumann", qk.MLPPreconditioner(7)), ("mlp", net)):

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <cstdlib>
 #include <sstream>
 #include <string>
@@ -113,15 +114,23 @@ __device__ __forceinline__ void store_nt(T* p, const T& v) {
 // another of the same kind, size and counted masks (ColTables: its row
 // offsets and row classes; the low-half ranks are shared).  y has stride
 // c.ld_out, x stride ld.
-template <typename S, int B, bool CROSS>
+// ROWS: the groups inside one half come from the row-program tables (q) and
+// p holds only the others; contributions are added in group order within
+// each kind.
+template <typename S, int B, bool CROSS, bool ROWS>
 __global__ void __launch_bounds__(kGBlock)
-k_apply(const ApplyParams p, const ColTables c, const cplx* __restrict__ x, cplx* __restrict__ y,
-        const u64 ld) {
+k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* __restrict__ x,
+        cplx* __restrict__ y, const u64 ld) {
+    static_assert(!(CROSS && ROWS), "row programs are built for one basis");
     extern __shared__ __align__(16) unsigned char smem[];
     cplx* s_tab = reinterpret_cast<cplx*>(smem);
     u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ntab) * sizeof(cplx));
+    cplx* s_amp = reinterpret_cast<cplx*>(smem + size_t(p.ntab) * sizeof(cplx) +
+                                          ((size_t(p.nlo) * sizeof(u16) + 15) / 16) * 16);
 
     for (u32 i = threadIdx.x; i < p.ntab; i += kGBlock) s_tab[i] = p.tables[i];
+    if constexpr (ROWS)
+        for (u32 i = threadIdx.x; i < q.namp; i += kGBlock) s_amp[i] = q.amps[i];
     if ((p.nlo & 7) == 0) {
         const uint4* src = reinterpret_cast<const uint4*>(p.jlo);
         uint4* dst = reinterpret_cast<uint4*>(s_jlo);
@@ -160,6 +169,11 @@ k_apply(const ApplyParams p, const ColTables c, const cplx* __restrict__ x, cplx
 #pragma unroll
         for (int v = 0; v < B; ++v) acc[v] = cplx{0.0, 0.0};
         cplx dsum{0.0, 0.0};
+        if constexpr (ROWS) {
+            const cplx d0 = q.dlo[off + r], d1 = q.dhi[hi];
+            dsum.re = d0.re + d1.re;
+            dsum.im = d0.im + d1.im;
+        }
 
         u32 g = 0;
         // diagonal groups
@@ -169,6 +183,28 @@ k_apply(const ApplyParams p, const ColTables c, const cplx* __restrict__ x, cplx
             if (G.jw && (popc_s(s & S(G.jw)) & 1)) { amp.re = -amp.re; amp.im = -amp.im; }
             dsum.re += amp.re;
             dsum.im += amp.im;
+        }
+        if constexpr (ROWS) {
+            // low-half groups of the row program: this rank's entries
+            // entries are read C at a time ahead of their gathers; a lane's
+            // padding entries (after its last one) end its loop
+            constexpr int C = B >= 8 ? 1 : (B >= 4 ? 2 : 4);
+            const u32 w = q.ell_w[cls];
+            const u32* __restrict__ E = q.ell + q.ell_off[cls] + r;
+            for (u32 k = 0; k < w; k += C) {
+                u32 e[C];
+#pragma unroll
+                for (int j = 0; j < C; ++j) e[j] = (k + j < w) ? E[size_t(k + j) * cnt] : ~0u;
+                if (e[0] == ~0u) break;
+#pragma unroll
+                for (int j = 0; j < C; ++j) {
+                    if (e[j] == ~0u) break;
+                    const cplx amp = s_amp[e[j] >> 16];
+                    const u64 b = base + (e[j] & 0xFFFFu);
+#pragma unroll
+                    for (int v = 0; v < B; ++v) cfma(acc[v], amp, x[v * ld + b]);
+                }
+            }
         }
         // low-half groups: same row
         const u32 g_lo_end = p.n_diag + p.n_lo;
@@ -184,6 +220,16 @@ k_apply(const ApplyParams p, const ColTables c, const cplx* __restrict__ x, cplx
             const u64 b = base_c + rl;
 #pragma unroll
             for (int v = 0; v < B; ++v) cfma(acc[v], amp, x[v * ld + b]);
+        }
+        if constexpr (ROWS) {
+            // high-half groups of the row program: the row's entries (uniform)
+            const u32 e1 = q.rptr[hi + 1];
+            for (u32 e = q.rptr[hi]; e < e1; ++e) {
+                const RowEntry R = q.rent[e];
+                const u64 b = R.base + r;
+#pragma unroll
+                for (int v = 0; v < B; ++v) cfma(acc[v], R.amp, x[v * ld + b]);
+            }
         }
         // high-half groups: another row, same column
         const u32 g_hi_end = g_lo_end + p.n_hi;
@@ -221,7 +267,7 @@ k_apply(const ApplyParams p, const ColTables c, const cplx* __restrict__ x, cplx
 #pragma unroll
         for (int v = 0; v < B; ++v) {
             cplx o = acc[v];
-            if (p.n_diag && cls_c == cls) cfma(o, dsum, x[v * ld + base_c + r]);
+            if ((p.n_diag || (ROWS && q.ndiag)) && cls_c == cls) cfma(o, dsum, x[v * ld + base_c + r]);
             if constexpr (CROSS) y[v * c.ld_out + a] = o;
             else y[v * ld + a] = o;
         }
@@ -724,8 +770,15 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
             return;
         }
     }
-    hipLaunchKernelGGL((k_apply<S, B, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params,
-                       ColTables{}, x, y, ld);
+    if constexpr (sizeof(S) == 4) {
+        if (h.rows && (B == 1 ? h.rows_1 : h.rows_b)) {
+            hipLaunchKernelGGL((k_apply<S, B, false, true>), dim3(h.grid), dim3(kGBlock), h.rlds_bytes,
+                               st, h.rest, ColTables{}, h.rparams, x, y, ld);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_apply<S, B, false, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st,
+                       h.params, ColTables{}, RowParams{}, x, y, ld);
 }
 
 template <typename S>
@@ -733,7 +786,7 @@ void launch_apply_s(const Hamiltonian& h, const cplx* x, cplx* y, int nvec, u64 
                     hipStream_t st) {
     // the exchange path may run blocks as groups of fewer vectors (1: one
     // single-vector launch per vector), as chosen at set-up
-    const int gmax = (sizeof(S) == 4 && h.xchg) ? h.vec_group : 8;
+    const int gmax = (sizeof(S) == 4 && h.xchg) ? h.vec_group : h.gen_group;
     int v = 0;
     while (v < nvec) {
         const int left = std::min(nvec - v, gmax);
@@ -869,15 +922,15 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
             QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes));
         };
         if (state32) {
-            set((const void*)k_apply<u32, 1, false>); set((const void*)k_apply<u32, 2, false>);
-            set((const void*)k_apply<u32, 4, false>); set((const void*)k_apply<u32, 8, false>);
-            set((const void*)k_apply<u32, 1, true>); set((const void*)k_apply<u32, 2, true>);
-            set((const void*)k_apply<u32, 4, true>); set((const void*)k_apply<u32, 8, true>);
+            set((const void*)k_apply<u32, 1, false, false>); set((const void*)k_apply<u32, 2, false, false>);
+            set((const void*)k_apply<u32, 4, false, false>); set((const void*)k_apply<u32, 8, false, false>);
+            set((const void*)k_apply<u32, 1, true, false>); set((const void*)k_apply<u32, 2, true, false>);
+            set((const void*)k_apply<u32, 4, true, false>); set((const void*)k_apply<u32, 8, true, false>);
         } else {
-            set((const void*)k_apply<u64, 1, false>); set((const void*)k_apply<u64, 2, false>);
-            set((const void*)k_apply<u64, 4, false>); set((const void*)k_apply<u64, 8, false>);
-            set((const void*)k_apply<u64, 1, true>); set((const void*)k_apply<u64, 2, true>);
-            set((const void*)k_apply<u64, 4, true>); set((const void*)k_apply<u64, 8, true>);
+            set((const void*)k_apply<u64, 1, false, false>); set((const void*)k_apply<u64, 2, false, false>);
+            set((const void*)k_apply<u64, 4, false, false>); set((const void*)k_apply<u64, 8, false, false>);
+            set((const void*)k_apply<u64, 1, true, false>); set((const void*)k_apply<u64, 2, true, false>);
+            set((const void*)k_apply<u64, 4, true, false>); set((const void*)k_apply<u64, 8, true, false>);
         }
     }
 
@@ -889,7 +942,198 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
 
     const char* mode = std::getenv("QKRYLOV_APPLY");
     force_generic = mode && std::string(mode) == "generic";
-    if (state32 && !force_generic) setup_exchange();
+    const bool force_rows = mode && std::string(mode) == "rows";
+    if (state32 && !force_generic && !force_rows) setup_exchange();
+    if (state32 && !force_generic && !xchg) setup_rows();
+}
+
+// Row-program path for the groups that lie inside one half (RowParams).
+// Declined (generic group loop for everything) when no group qualifies, when
+// the amplitudes take more than 65,535 distinct values or when the tables
+// would be unreasonably large.
+void Hamiltonian::setup_rows() {
+    const Basis& B = *basis;
+    const int L = B.L();
+    const u64 lomask = B.lomask();
+    const u32 m1 = u32(B.m1() & lomask), m2 = u32(B.m2() & lomask);
+    const int ncls2 = B.ncls2();
+    auto inside_lo = [&](const Group& g) {
+        for (int q : g.pos)
+            if (q >= L) return false;
+        return (g.jw & ~lomask) == 0;
+    };
+    auto inside_hi = [&](const Group& g) {
+        for (int q : g.pos)
+            if (q < L) return false;
+        return (g.jw & lomask) == 0;
+    };
+    // the kernels' amplitude: table entry of the support configuration,
+    // negated for odd Jordan-Wigner parity
+    auto amp_of = [](const Group& g, u64 st) {
+        u32 cf = 0;
+        for (size_t k = 0; k < g.pos.size(); ++k) cf |= u32((st >> g.pos[k]) & 1) << k;
+        cplx a = g.table[cf];
+        if (g.jw && (__builtin_popcountll(st & g.jw) & 1)) a = cplx{-a.re, -a.im};
+        return a;
+    };
+    auto nonzero = [](const cplx& a) { return a.re != 0.0 || a.im != 0.0; };
+
+    std::vector<const Group*> dlo_g, dhi_g, lo_g, hi_g;
+    std::vector<Group> rest_diag, rest_lo, rest_hi, rest_mix;
+    for (const Group& g : prog.groups) {
+        if (g.kind == kDiag && inside_lo(g)) dlo_g.push_back(&g);
+        else if (g.kind == kDiag && inside_hi(g)) dhi_g.push_back(&g);
+        else if (g.kind == kLo && inside_lo(g)) lo_g.push_back(&g);
+        else if (g.kind == kHi && inside_hi(g)) hi_g.push_back(&g);
+        else if (g.kind == kDiag) rest_diag.push_back(g);
+        else if (g.kind == kLo) rest_lo.push_back(g);
+        else if (g.kind == kHi) rest_hi.push_back(g);
+        else rest_mix.push_back(g);
+    }
+    if (dlo_g.empty() && dhi_g.empty() && lo_g.empty() && hi_g.empty()) return;
+
+    const std::vector<u32>& cls_off = B.cls_off();
+    const std::vector<u32>& cls_cnt = B.cls_cnt();
+    const std::vector<u32>& lo_list = B.lo_list();
+    const std::vector<u16>& jlo = B.jlo();
+    const std::vector<u64>& jh = B.jhi();
+    const std::vector<int>& hcls = B.hi_cls();
+    const size_t ncls = cls_cnt.size(), nrow = jh.size();
+
+    // diagonal sums
+    std::vector<cplx> dlo(std::max<size_t>(lo_list.size(), 1), cplx{0.0, 0.0});
+    std::vector<cplx> dhi(std::max<size_t>(nrow, 1), cplx{0.0, 0.0});
+    for (size_t i = 0; i < lo_list.size(); ++i)
+        for (const Group* g : dlo_g) {
+            const cplx a = amp_of(*g, lo_list[i]);
+            dlo[i] = cplx{dlo[i].re + a.re, dlo[i].im + a.im};
+        }
+    for (size_t h = 0; h < nrow; ++h) {
+        if (jh[h] == ~u64(0)) continue;
+        for (const Group* g : dhi_g) {
+            const cplx a = amp_of(*g, u64(h) << L);
+            dhi[h] = cplx{dhi[h].re + a.re, dhi[h].im + a.im};
+        }
+    }
+
+    // amplitude ids
+    std::map<std::pair<u64, u64>, u32> ids;
+    std::vector<cplx> amps;
+    bool too_many = false;
+    auto id_of = [&](const cplx& a) -> u32 {
+        u64 kr, ki;
+        std::memcpy(&kr, &a.re, 8);
+        std::memcpy(&ki, &a.im, 8);
+        auto it = ids.find({kr, ki});
+        if (it != ids.end()) return it->second;
+        if (amps.size() >= 0xFFFF) {
+            too_many = true;
+            return 0;
+        }
+        const u32 id = u32(amps.size());
+        ids[{kr, ki}] = id;
+        amps.push_back(a);
+        return id;
+    };
+
+    // low-half groups: per class, padded column-major entries per rank
+    const size_t kMaxEntries = size_t(1) << 26;
+    std::vector<u32> ell, ell_off(std::max<size_t>(ncls, 1), 0u), ell_w(std::max<size_t>(ncls, 1), 0u);
+    std::vector<u32> row_e;
+    for (size_t c = 0; c < ncls && !lo_g.empty(); ++c) {
+        const u32 cnt = cls_cnt[c], off = cls_off[c];
+        std::vector<std::vector<u32>> per(cnt);
+        u32 w = 0;
+        for (u32 r = 0; r < cnt; ++r) {
+            const u32 lo = lo_list[off + r];
+            for (const Group* g : lo_g) {
+                const cplx a = amp_of(*g, lo);
+                if (!nonzero(a)) continue;
+                const u32 tlo = lo ^ u32(g->flip);
+                const u16 rl = jlo[tlo];
+                if (rl == 0xFFFF) continue;
+                if (u32(__builtin_popcount(tlo & m1) * ncls2 + __builtin_popcount(tlo & m2)) != c) continue;
+                per[r].push_back(u32(rl) | (id_of(a) << 16));
+            }
+            w = std::max<u32>(w, u32(per[r].size()));
+        }
+        if (too_many) return;
+        ell_off[c] = u32(ell.size());
+        ell_w[c] = w;
+        if (ell.size() + size_t(w) * cnt > kMaxEntries) return;
+        ell.resize(ell.size() + size_t(w) * cnt, ~0u);
+        for (u32 r = 0; r < cnt; ++r)
+            for (size_t k = 0; k < per[r].size(); ++k) ell[ell_off[c] + k * cnt + r] = per[r][k];
+    }
+
+    // high-half groups: per row, the partner row offsets and amplitudes
+    std::vector<u32> rptr(nrow + 1, 0u);
+    std::vector<RowEntry> rent;
+    for (size_t h = 0; h < nrow; ++h) {
+        rptr[h] = u32(rent.size());
+        if (jh[h] == ~u64(0)) continue;
+        for (const Group* g : hi_g) {
+            const size_t h2 = h ^ size_t(g->flip >> L);
+            if (jh[h2] == ~u64(0) || hcls[h2] != hcls[h]) continue;
+            const cplx a = amp_of(*g, u64(h) << L);
+            if (!nonzero(a)) continue;
+            rent.push_back(RowEntry{jh[h2], a});
+        }
+        if (rent.size() > kMaxEntries / 4) return;
+    }
+    rptr[nrow] = u32(rent.size());
+
+    // the remaining groups keep the group loop
+    std::vector<Group> restg;
+    for (auto* v : {&rest_diag, &rest_lo, &rest_hi, &rest_mix}) restg.insert(restg.end(), v->begin(), v->end());
+    std::vector<cplx> rtab;
+    std::vector<DevGroup> rdev = pack_groups(restg, rtab);
+    const int lds = int(rtab.size() * sizeof(cplx) + ((B.jlo().size() * sizeof(u16) + 15) / 16) * 16 +
+                        amps.size() * sizeof(cplx));
+    if (lds > 160 * 1024) return;
+
+    upload(d_rgroups, rdev.data(), rdev.size(), stream);
+    upload(d_rtables, rtab.data(), rtab.size(), stream);
+    upload(d_rdlo, dlo.data(), dlo.size(), stream);
+    upload(d_rdhi, dhi.data(), dhi.size(), stream);
+    if (ell.empty()) ell.push_back(~0u);
+    upload(d_ell, ell.data(), ell.size(), stream);
+    upload(d_elloff, ell_off.data(), ell_off.size(), stream);
+    upload(d_ellw, ell_w.data(), ell_w.size(), stream);
+    if (amps.empty()) amps.push_back(cplx{0.0, 0.0});
+    upload(d_amps, amps.data(), amps.size(), stream);
+    upload(d_rptr, rptr.data(), rptr.size(), stream);
+    if (rent.empty()) rent.push_back(RowEntry{0, cplx{0.0, 0.0}});
+    upload(d_rent, rent.data(), rent.size(), stream);
+    QK_HIP(hipStreamSynchronize(stream));
+
+    rest = params;
+    rest.groups = d_rgroups.get<DevGroup>();
+    rest.tables = d_rtables.get<cplx>();
+    rest.ngroups = u32(rdev.size());
+    rest.n_diag = u32(rest_diag.size());
+    rest.n_lo = u32(rest_lo.size());
+    rest.n_hi = u32(rest_hi.size());
+    rest.n_mix = u32(rest_mix.size());
+    rest.ntab = u32(rtab.size());
+    RowParams& q = rparams;
+    q.dlo = d_rdlo.get<cplx>();
+    q.dhi = d_rdhi.get<cplx>();
+    q.ell = d_ell.get<u32>();
+    q.ell_off = d_elloff.get<u32>();
+    q.ell_w = d_ellw.get<u32>();
+    q.amps = d_amps.get<cplx>();
+    q.rptr = d_rptr.get<u32>();
+    q.rent = d_rent.get<RowEntry>();
+    q.namp = u32(amps.size());
+    q.ndiag = u32(dlo_g.size() + dhi_g.size());
+    rlds_bytes = lds;
+    if (rlds_bytes > 64 * 1024) {
+        for (const void* f : {(const void*)k_apply<u32, 1, false, true>, (const void*)k_apply<u32, 2, false, true>,
+                              (const void*)k_apply<u32, 4, false, true>, (const void*)k_apply<u32, 8, false, true>})
+            QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, rlds_bytes));
+    }
+    rows = true;
 }
 
 // Select the exchange fast path when every off-diagonal group is a real
@@ -1106,7 +1350,7 @@ void Hamiltonian::set_single_kernel(int s, int nb) {
 std::string Hamiltonian::autotune(int nvec) {
     std::ostringstream os;
     os << "{";
-    if (!xchg || dim() == 0) {
+    if (dim() == 0 || (!xchg && !rows)) {
         os << "\"tuned\": 0}";
         return os.str();
     }
@@ -1128,6 +1372,50 @@ std::string Hamiltonian::autotune(int nvec) {
         return double(ms) / reps;
     };
     const int reps = d > (u64(1) << 24) ? 3 : 10;
+    if (!xchg) {
+        // row-program against group-loop kernel, then vectors per launch
+        double best1 = 1e300, bestb = 1e300;
+        bool r1 = rows_1, rb = rows_b;
+        int gg = gen_group;
+        os << "\"single\": {";
+        for (int k = 0; k < 2; ++k) {
+            rows_1 = k == 0;
+            const double ms = time_ms(1, reps);
+            os << (k ? ", " : "") << (k == 0 ? "\"rows\"" : "\"loop\"") << ": " << ms;
+            if (ms < best1) {
+                best1 = ms;
+                r1 = rows_1;
+            }
+        }
+        rows_1 = r1;
+        os << "}, \"row_single\": " << (r1 ? 1 : 0);
+        if (nvec > 1) {
+            os << ", \"groups\": {";
+            bool first = true;
+            for (int k = 0; k < 2; ++k)
+                for (int g : {1, 2, 4, 8}) {
+                    if (g > nvec && g != 1) continue;
+                    if (g == 1 && k == 1) continue;   // single launches follow row_single
+                    rows_b = k == 0;
+                    gen_group = g;
+                    const double ms = time_ms(nvec, reps);
+                    os << (first ? "" : ", ") << "\"" << (k == 0 ? "rows" : "loop") << "/" << g << "\": " << ms;
+                    first = false;
+                    if (ms < bestb) {
+                        bestb = ms;
+                        rb = rows_b;
+                        gg = g;
+                    }
+                }
+            rows_b = rb;
+            gen_group = gg;
+            os << "}, \"row_block\": " << (rb ? 1 : 0) << ", \"gen_group\": " << gg;
+        }
+        os << ", \"tuned\": 1}";
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return os.str();
+    }
     double best = 1e300;
     int bs = spl, bn = bond_batch;
     os << "\"single\": {";
@@ -1333,16 +1621,16 @@ static void launch_operator(const Hamiltonian& h, const ColTables& c, const cplx
         const cplx* xv = x ? x + u64(v) * ld_in : nullptr;
         cplx* yv = y + u64(v) * ld_out;
         if (left >= 8) {
-            hipLaunchKernelGGL((k_apply<S, 8, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            hipLaunchKernelGGL((k_apply<S, 8, true, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, RowParams{}, xv, yv, ld_in);
             v += 8;
         } else if (left >= 4) {
-            hipLaunchKernelGGL((k_apply<S, 4, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            hipLaunchKernelGGL((k_apply<S, 4, true, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, RowParams{}, xv, yv, ld_in);
             v += 4;
         } else if (left >= 2) {
-            hipLaunchKernelGGL((k_apply<S, 2, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            hipLaunchKernelGGL((k_apply<S, 2, true, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, RowParams{}, xv, yv, ld_in);
             v += 2;
         } else {
-            hipLaunchKernelGGL((k_apply<S, 1, true>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, xv, yv, ld_in);
+            hipLaunchKernelGGL((k_apply<S, 1, true, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st, h.params, c, RowParams{}, xv, yv, ld_in);
             v += 1;
         }
     }
@@ -1404,7 +1692,11 @@ std::string Hamiltonian::info() const {
        << ", \"max_support\": " << prog.max_support << ", \"exchange_path\": " << (xchg ? 1 : 0)
        << ", \"xbonds\": [" << xparams.n_lo << ", " << xparams.n_hi << ", " << xparams.n_mix
        << "], \"diag_mixed\": " << xparams.n_dmix << ", \"states_per_lane\": " << spl
-       << ", \"bond_batch\": " << bond_batch << ", \"vec_group\": " << vec_group << ", \"xgrid\": " << xgrid_s << "}";
+       << ", \"bond_batch\": " << bond_batch << ", \"vec_group\": " << vec_group << ", \"xgrid\": " << xgrid_s
+       << ", \"row_path\": " << (rows ? 1 : 0) << ", \"row_single\": " << (rows && rows_1 ? 1 : 0)
+       << ", \"row_block\": " << (rows && rows_b ? 1 : 0) << ", \"gen_group\": " << gen_group
+       << ", \"loop_groups\": " << (rows ? rest.ngroups : params.ngroups)
+       << ", \"row_lds\": " << rlds_bytes << "}";
     return os.str();
 }
 

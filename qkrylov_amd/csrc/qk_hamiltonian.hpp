@@ -129,6 +129,34 @@ struct ApplyParams {
     u32 tiles_per_xcd;
 };
 
+// Row-program form of the groups that lie inside one half of the state
+// (32-bit states, one basis): low-half groups whose support and
+// Jordan-Wigner bits are in the low half act inside a row the same way for
+// every row of a class, so per class they become a padded column-major table
+// of (target rank, amplitude id) per rank; high-half groups inside the high
+// half act the same way on every state of a row, so per row they become a
+// list of (partner row offset, amplitude); diagonal groups inside one half
+// become per-state (low) and per-row (high) sums.  Every other group stays
+// in the kernel's group loop.
+struct RowEntry {
+    u64 base;
+    cplx amp;
+};
+static_assert(sizeof(RowEntry) == 24, "RowEntry layout");
+
+struct RowParams {
+    const cplx* dlo;        // lo_list order: diagonal groups inside the low half
+    const cplx* dhi;        // per row: diagonal groups inside the high half
+    const u32* ell;         // class c, rank r, entry k: ell[ell_off[c] + k * cnt_c + r]
+    const u32* ell_off;     //   entry = target rank | amplitude id << 16, ~0u: none
+    const u32* ell_w;       // entries per rank of class c
+    const cplx* amps;       // amplitude ids -> values (copied to LDS)
+    const u32* rptr;        // row hi: rent[rptr[hi] .. rptr[hi + 1])
+    const RowEntry* rent;
+    u32 namp;
+    u32 ndiag;              // diagonal groups folded into dlo and dhi
+};
+
 // Row tables of the basis the targets are ranked in, when it differs from
 // the basis whose rows are enumerated (Operator); unused otherwise.
 struct ColTables {
@@ -163,10 +191,13 @@ struct Hamiltonian {
     void sz_q(const cplx* psi, cplx* phi, int nq, u64 ld, hipStream_t st) const;
 
     std::string info() const;
-    // Time the exchange-path kernel variants for blocks of nvec vectors and
-    // keep the fastest; returns the timings (JSON).
+    // Time the kernel variants (exchange path: states per lane, bond batch,
+    // vectors per launch; otherwise: row-program or group-loop kernel for
+    // single vectors and blocks, vectors per launch) for blocks of nvec
+    // vectors and keep the fastest; returns the timings (JSON).
     std::string autotune(int nvec);
     void setup_exchange();
+    void setup_rows();
     void set_single_kernel(int states_per_lane, int bond_batch);
     std::vector<XTile> xtiles_for(u32 span) const;
 
@@ -201,6 +232,16 @@ struct Hamiltonian {
     int xlds_bytes = 0;
     int xgrid = 0;
     bool force_generic = false; // QKRYLOV_APPLY=generic
+    // row-program path (setup_rows): the remaining groups, their tables and
+    // the row-program tables
+    bool rows = false;          // row-program tables built
+    bool rows_1 = true;         // single-vector launches use them
+    bool rows_b = true;         // block launches use them
+    int gen_group = 8;          // vectors per block launch off the exchange path
+    ApplyParams rest{};
+    RowParams rparams{};
+    int rlds_bytes = 0;
+    DevBuf d_rgroups, d_rtables, d_rdlo, d_rdhi, d_ell, d_elloff, d_ellw, d_amps, d_rptr, d_rent;
     bool state32 = true;
     int lds_bytes = 0;
     int dlds_bytes = 0;

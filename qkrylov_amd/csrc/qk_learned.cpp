@@ -275,8 +275,10 @@ void MlpPreconditioner::apply_many(const Hamiltonian& H, int nt, const cplx* con
     const u64 n = states_.n;
     if (!n || nt <= 0) return;
     const u64 tot = n * u64(nt);
-    X_.resize(sizeof(u16) * tot * kMlpIn);
-    o_.resize(sizeof(float) * tot * kMlpOut);
+    // network rows per correction: one per class of equal inputs, or one per state
+    const u64 rows = states_.ncls ? states_.ncls : n;
+    X_.resize(sizeof(u16) * rows * nt * kMlpIn);
+    o_.resize(sizeof(float) * rows * nt * kMlpOut);
     z_.resize(sizeof(cplx) * tot);
     w_.resize(sizeof(cplx) * tot);
     c_.resize(sizeof(cplx) * nt);
@@ -286,12 +288,15 @@ void MlpPreconditioner::apply_many(const Hamiltonian& H, int nt, const cplx* con
     H.apply(z, w, nt, n, st);   // w holds H z until the Neumann term overwrites it
     for (int j = 0; j < nt; ++j) {
         mlp_neumann(states_, lambda[j], w + u64(j) * n, z + u64(j) * n, w + u64(j) * n, st);
-        mlp_features(states_, lambda[j], X_.get<u16>() + u64(j) * n * kMlpIn, st);
+        u16* Xj = X_.get<u16>() + u64(j) * rows * kMlpIn;
+        if (states_.ncls) mlp_features_cls(states_, lambda[j], Xj, st);
+        else mlp_features(states_, lambda[j], Xj, st);
     }
-    net_.forward(X_.get<u16>(), tot, o_.get<float>(), nullptr, nullptr, st);
+    net_.forward(X_.get<u16>(), rows * u64(nt), o_.get<float>(), nullptr, nullptr, st);
     for (int j = 0; j < nt; ++j) {
-        mlp_combine(states_, o_.get<float>() + u64(j) * n * kMlpOut, z + u64(j) * n, w + u64(j) * n,
-                    t[j], st);
+        const float* oj = o_.get<float>() + u64(j) * rows * kMlpOut;
+        if (states_.ncls) mlp_combine_cls(states_, oj, z + u64(j) * n, w + u64(j) * n, t[j], st);
+        else mlp_combine(states_, oj, z + u64(j) * n, w + u64(j) * n, t[j], st);
         if (u && u[j]) {
             // t approximates (H - lambda)^-1 r, which is u itself for an exact
             // Ritz pair: remove the u component here, before Davidson

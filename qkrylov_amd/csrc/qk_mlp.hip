@@ -18,6 +18,7 @@
 #include <cstring>
 #include <random>
 #include <thread>
+#include <unordered_map>
 
 namespace qk {
 
@@ -452,6 +453,16 @@ __global__ void k_mlp_combine(const u64 n, const float* __restrict__ o, const cp
     t[i] = cplx{s1 * z[i].re + s2 * w[i].re, s1 * z[i].im + s2 * w[i].im};
 }
 
+__global__ void k_mlp_combine_cls(const u64 n, const float* __restrict__ o, const u32* __restrict__ cls,
+                                  const cplx* __restrict__ z, const cplx* __restrict__ w,
+                                  cplx* __restrict__ t) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s1, s2, ta, tb;
+    mlp_coef(o, cls[i], s1, s2, ta, tb);
+    t[i] = cplx{s1 * z[i].re + s2 * w[i].re, s1 * z[i].im + s2 * w[i].im};
+}
+
 __global__ void k_mlp_project(const cplx* __restrict__ u, const cplx* __restrict__ c,
                               cplx* __restrict__ t, const u64 n) {
     const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -698,7 +709,65 @@ void MlpStates::build(const Hamiltonian& H, hipStream_t st) {
     for (auto& t : th) t.join();
     QK_HIP(hipMemcpyAsync(diag.get(), dr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
     QK_HIP(hipMemcpyAsync(stat.get(), sh.data(), sizeof(float) * 5 * n, hipMemcpyHostToDevice, st));
+
+    // classes of states with bit-identical network inputs
+    struct Key {
+        u64 d;
+        u32 f[5];
+        bool operator==(const Key& o) const { return d == o.d && std::memcmp(f, o.f, sizeof f) == 0; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const {
+            u64 h = k.d * 0x9E3779B97F4A7C15ull;
+            for (u32 v : k.f) h = (h ^ v) * 0x100000001B3ull;
+            return size_t(h ^ (h >> 29));
+        }
+    };
+    std::unordered_map<Key, u32, KeyHash> ids;
+    std::vector<u32> cl(n);
+    std::vector<double> rd;
+    std::vector<float> rs;
+    ncls = 0;
+    const u64 limit = n / 4;
+    bool ok = true;
+    for (u64 i = 0; i < n && ok; ++i) {
+        Key k;
+        std::memcpy(&k.d, &dr[i], 8);
+        std::memcpy(k.f, &sh[5 * i], sizeof k.f);
+        auto it = ids.find(k);
+        if (it == ids.end()) {
+            if (ids.size() >= std::max<u64>(limit, 1)) {
+                ok = false;
+                break;
+            }
+            it = ids.emplace(k, u32(ids.size())).first;
+            rd.push_back(dr[i]);
+            rs.insert(rs.end(), &sh[5 * i], &sh[5 * i] + 5);
+        }
+        cl[i] = it->second;
+    }
+    if (ok && !ids.empty()) {
+        ncls = ids.size();
+        upload(cls, cl.data(), cl.size(), st);
+        upload(rdiag, rd.data(), rd.size(), st);
+        upload(rstat, rs.data(), rs.size(), st);
+    }
     QK_HIP(hipStreamSynchronize(st));
+}
+
+void mlp_features_cls(const MlpStates& s, double theta, u16* X, hipStream_t st) {
+    if (!s.ncls) return;
+    hipLaunchKernelGGL(k_mlp_features, dim3(unsigned((s.ncls + 255) / 256)), dim3(256), 0, st,
+                       s.rdiag.get<double>(), s.rstat.get<float>(), s.ncls, theta, 1.0 / s.nsites, X);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_combine_cls(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
+                     hipStream_t st) {
+    if (!s.n) return;
+    hipLaunchKernelGGL(k_mlp_combine_cls, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st, s.n, o,
+                       s.cls.get<u32>(), z, w, t);
+    QK_HIP(hipGetLastError());
 }
 
 void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st) {

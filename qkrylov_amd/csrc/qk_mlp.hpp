@@ -110,8 +110,20 @@ struct MlpStates {
     DevBuf stat;   // float[4 + 1]: occupation fractions (4), connectivity
     u64 n = 0;
     int nsites = 1;
+    // States with equal (diag, stat) have equal features for every theta, so
+    // inference evaluates the network once per class of such states:
+    // cls[i] is state i's class, rdiag/rstat the class's inputs.  ncls = 0
+    // when classes would not save at least three quarters of the rows.
+    DevBuf cls, rdiag, rstat;
+    u64 ncls = 0;
     void build(const Hamiltonian& H, hipStream_t st);
 };
+
+// Features of the ncls class representatives for theta (device).
+void mlp_features_cls(const MlpStates& s, double theta, u16* X, hipStream_t st);
+// t = s1 z + s2 w with (s1, s2) from the output row of the state's class
+void mlp_combine_cls(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
+                     hipStream_t st);
 
 // X[i] = features of state i for theta (device).
 void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st);

@@ -39,6 +39,36 @@ struct Timer {
     }
 };
 
+// Per-phase totals of a solver loop (QKRYLOV_TIMING): each mark waits for
+// the stream and charges the time since the previous mark to `what`.
+struct PhaseClock {
+    const char* name;
+    bool on;
+    std::chrono::steady_clock::time_point t0;
+    std::vector<std::pair<std::string, double>> acc;
+    int extra = 0;   // solver-specific count reported with the totals
+    explicit PhaseClock(const char* n)
+        : name(n), on(std::getenv("QKRYLOV_TIMING") != nullptr), t0(std::chrono::steady_clock::now()) {}
+    void mark(hipStream_t st, const char* what) {
+        if (!on) return;
+        QK_HIP(hipStreamSynchronize(st));
+        const auto t = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(t - t0).count();
+        t0 = t;
+        for (auto& a : acc)
+            if (a.first == what) {
+                a.second += ms;
+                return;
+            }
+        acc.emplace_back(what, ms);
+    }
+    ~PhaseClock() {
+        if (!on) return;
+        for (auto& a : acc) std::fprintf(stderr, "[qkrylov %s] %-24s %9.3f ms\n", name, a.first.c_str(), a.second);
+        std::fprintf(stderr, "[qkrylov %s] count %d\n", name, extra);
+    }
+};
+
 // The reference normal stream for a seed (std::mt19937 + normal(0,1),
 // ftlm.cpp:133-143) in pinned host memory, kept for the process: a longer
 // request extends it (the stream for n draws is a prefix of the stream for
@@ -515,7 +545,8 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
     DevBuf diag(sizeof(cplx) * d), red, sc(64 * sizeof(double)), h(sizeof(cplx) * cap * cap),
         slabV(sizeof(cplx) * d * cap), slabHV(sizeof(cplx) * d * cap), Rb(sizeof(cplx) * d * nb),
         Ub(sizeof(cplx) * d * nb), Tb(sizeof(cplx) * d * nb), Nb(sizeof(cplx) * d * nb * 2),
-        nrm(sizeof(double) * nb), coef(sizeof(cplx) * size_t(nb) * 2 * cap);
+        nrm(sizeof(double) * nb), coef(sizeof(cplx) * size_t(nb) * 2 * cap),
+        tnorm(sizeof(double) * 3 * nb);
     auto vp = [&](int i) { return slabV.get<cplx>() + u64(i) * d; };
     auto hvp = [&](int i) { return slabHV.get<cplx>() + u64(i) * d; };
     DevBuf pV(sizeof(void*) * cap), pHV(sizeof(void*) * cap), pI(sizeof(void*) * 2 * cap),
@@ -567,7 +598,9 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         const char* e = std::getenv("QKRYLOV_DAVIDSON_BLOCK_DOT");
         return !(e && std::string(e) == "0");
     }();
+    PhaseClock clk("davidson");
     for (int iter = 0; iter < max_iter; ++iter) {
+        clk.mark(st, "apply (expansion)");
         // H_sub[i][j] = <V_i, HV_j>, j <= i (:143-150); new rows only.
         std::vector<cplx> Hn(size_t(m) * m, cplx{0.0, 0.0});
         for (int i = 0; i < std::min(valid_rows, m); ++i)
@@ -602,9 +635,11 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         Hs = std::move(Hn);
         valid_rows = m;
 
+        clk.mark(st, "projection");
         std::vector<double> ev;
         std::vector<cplx> S;
         hermitian_eigh(Hs, m, ev, S);
+        clk.mark(st, "small eigensolve");
 
         // residuals r_k = sum_i s_ki (HV_i - lambda_k V_i) (:161-165)
         const int ne = std::min(n_eig, m);
@@ -623,6 +658,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         std::vector<double> rn(ne);
         QK_HIP(hipMemcpyAsync(rn.data(), nrm.get(), sizeof(double) * ne, hipMemcpyDeviceToHost, st));
         QK_HIP(hipStreamSynchronize(st));
+        clk.mark(st, "residuals");
         res.residuals.assign(ne, 0.0);
         bool all_conv = true;
         std::vector<int> need;
@@ -636,13 +672,38 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         if (all_conv || iter == max_iter - 1) {
             res.iterations = iter + 1;
             res.converged = all_conv;
-            for (int k = 0; k < ne; ++k) res.eigenvalues.push_back(ev[k]);
-            if (want_vectors && ne > 0) {
+            // Eigenvalues as Rayleigh quotients <u, Hu>/<u, u> of the Ritz
+            // vectors (Hu = HV s): the Ritz values themselves while V is
+            // orthonormal, and still second order in the residual where
+            // rounding has cost V some orthogonality.
+            std::vector<int> order(ne);
+            std::iota(order.begin(), order.end(), 0);
+            if (ne > 0) {
                 upload_ritz(S, m, ne);
+                cplx* hu = Nb.get<cplx>();
                 vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, Ub.get<cplx>(), d, d, st);
+                vec::lincomb_many(pHV.get<const cplx*>(), m, coef.get<cplx>(), ne, hu, d, d, st);
+                for (int k = 0; k < ne; ++k) {
+                    vec::dot(Ub.get<cplx>() + u64(k) * d, hu + u64(k) * d, d, h.get<cplx>() + k, red, st);
+                    vec::norm2(Ub.get<cplx>() + u64(k) * d, d, nrm.get<double>() + k, red, st);
+                }
+                std::vector<cplx> uhu(ne);
+                std::vector<double> uu(ne);
+                QK_HIP(hipMemcpyAsync(uhu.data(), h.get(), sizeof(cplx) * ne, hipMemcpyDeviceToHost, st));
+                QK_HIP(hipMemcpyAsync(uu.data(), nrm.get(), sizeof(double) * ne, hipMemcpyDeviceToHost, st));
+                QK_HIP(hipStreamSynchronize(st));
+                for (int k = 0; k < ne; ++k)
+                    if (uu[k] > 0.0) ev[k] = uhu[k].re / uu[k];
+                std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ev[a] < ev[b]; });
+            }
+            for (int k = 0; k < ne; ++k) res.eigenvalues.push_back(ev[order[k]]);
+            std::vector<double> rs(ne);
+            for (int k = 0; k < ne; ++k) rs[k] = res.residuals[order[k]];
+            res.residuals = rs;
+            if (want_vectors && ne > 0) {
                 res.eigenvectors.assign(ne, std::vector<cplx>(d));
                 for (int k = 0; k < ne; ++k)
-                    QK_HIP(hipMemcpyAsync(res.eigenvectors[k].data(), Ub.get<cplx>() + u64(k) * d,
+                    QK_HIP(hipMemcpyAsync(res.eigenvectors[k].data(), Ub.get<cplx>() + u64(order[k]) * d,
                                           sizeof(cplx) * d, hipMemcpyDeviceToHost, st));
                 QK_HIP(hipStreamSynchronize(st));
             }
@@ -674,6 +735,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
                                     Tb.get<cplx>() + u64(j) * d, d, st);
         }
 
+        clk.mark(st, "preconditioner");
         // restart to the n_eig Ritz pairs (:196-211)
         if (m + nt > max_subspace) {
             upload_ritz(S, m, ne);
@@ -696,23 +758,53 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         // loop (only V enters the orthogonalisation).
         const int m0 = m;
         const bool block = use_block_dot && nt > 1 && m0 <= 64 && nt <= 16;
+        double* tn2 = tnorm.get<double>();   // [0, nt): before, [nt, 2nt): after the basis
+        cplx* T = Tb.get<cplx>();
+        for (int j = 0; j < nt; ++j) vec::norm2(T + u64(j) * d, d, tn2 + j, red, st);
         if (block) {
             for (int pass = 0; pass < 2; ++pass) {
                 vec::block_dot(pV.get<const cplx*>(), m0, pT.get<const cplx*>(), nt, d, h.get<cplx>(), red, st);
                 vec::block_sub(pV.get<const cplx*>(), m0, pT.get<cplx*>(), nt, h.get<cplx>(), d, st);
             }
+        } else {
+            for (int j = 0; j < nt; ++j) cgs2(pV, m0, T + u64(j) * d, d, h, red, st);
         }
+        for (int j = 0; j < nt; ++j) vec::norm2(T + u64(j) * d, d, tn2 + nt + j, red, st);
+        std::vector<double> tn(2 * size_t(nt));
+        if (nt > 0) {
+            QK_HIP(hipMemcpyAsync(tn.data(), tn2, sizeof(double) * 2 * nt, hipMemcpyDeviceToHost, st));
+            QK_HIP(hipStreamSynchronize(st));
+        }
+        // a correction that lost most of its norm to the basis is left
+        // orthogonal to it only to rounding relative to its original norm:
+        // one more pass
+        for (int j = 0; j < nt; ++j)
+            if (std::sqrt(tn[nt + j]) > 1e-10 && std::sqrt(tn[nt + j]) < 1e-3 * std::sqrt(tn[j])) {
+                cgs2(pV, m0, T + u64(j) * d, d, h, red, st);
+                ++clk.extra;
+            }
+        // Among the corrections, in order and without host round trips: each
+        // is orthogonalised against the earlier ones and normalised, or zeroed
+        // when its norm is 1e-10 or less (a zero vector adds nothing to the
+        // later projections, as a dropped one would not).
+        double* inv = tn2 + 2 * nt;
         for (int j = 0; j < nt; ++j) {
-            cplx* t = Tb.get<cplx>() + u64(j) * d;
-            if (block) cgs2(pV.get<const cplx*>() + m0, m - m0, t, d, h, red, st);
-            else cgs2(pV, m, t, d, h, red, st);
-            const double tn = dev_norm(t, d, red, sc, st);
-            if (tn > 1e-10) {
-                vec::scale(t, d, cplx{1.0 / tn, 0.0}, st);
-                QK_HIP(hipMemcpyAsync(vp(m), t, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
+            cplx* t = T + u64(j) * d;
+            if (j > 0) cgs2(pT.get<const cplx*>(), j, t, d, h, red, st);
+            vec::norm2(t, d, tn2 + j, red, st);
+            vec::inv_norm_or_zero(tn2 + j, inv + j, 1, 1e-10, st);
+            vec::scale_to(t, t, d, inv + j, st);
+        }
+        if (nt > 0) {
+            QK_HIP(hipMemcpyAsync(tn.data(), tn2, sizeof(double) * nt, hipMemcpyDeviceToHost, st));
+            QK_HIP(hipStreamSynchronize(st));
+        }
+        for (int j = 0; j < nt; ++j)
+            if (std::sqrt(tn[j]) > 1e-10) {
+                QK_HIP(hipMemcpyAsync(vp(m), T + u64(j) * d, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
                 ++m;
             }
-        }
+        clk.mark(st, "restart + orthogonalise");
         if (m > m0) H.apply(vp(m0), hvp(m0), m - m0, d, st);
     }
     return DavidsonResult{};   // not reached (the last iteration returns), as :226

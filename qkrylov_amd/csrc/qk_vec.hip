@@ -117,6 +117,14 @@ __global__ void k_scale_to(const cplx* __restrict__ x, cplx* __restrict__ y, u64
     if (i < n) y[i] = cplx{x[i].re * c, x[i].im * c};
 }
 
+__global__ void k_inv_norm(const double* __restrict__ nrm2, double* __restrict__ s, int k,
+                           double thresh) {
+    const int j = int(blockIdx.x) * kT + int(threadIdx.x);
+    if (j >= k) return;
+    const double t = sqrt(nrm2[j]);
+    s[j] = t > thresh ? 1.0 / t : 0.0;
+}
+
 __global__ void k_axpy(cplx a, const cplx* __restrict__ x, cplx* __restrict__ y, u64 n) {
     const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
     if (i < n) {
@@ -608,6 +616,12 @@ void scale(cplx* x, u64 n, cplx s, hipStream_t st) {
 void scale_to(const cplx* x, cplx* y, u64 n, const double* s_dev, hipStream_t st) {
     if (!n) return;
     hipLaunchKernelGGL(k_scale_to, dim3(ngrid(n)), dim3(kT), 0, st, x, y, n, s_dev);
+    QK_HIP(hipGetLastError());
+}
+
+void inv_norm_or_zero(const double* nrm2, double* s, int k, double thresh, hipStream_t st) {
+    if (k <= 0) return;
+    hipLaunchKernelGGL(k_inv_norm, dim3((k + kT - 1) / kT), dim3(kT), 0, st, nrm2, s, k, thresh);
     QK_HIP(hipGetLastError());
 }
 

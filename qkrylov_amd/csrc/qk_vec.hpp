@@ -40,6 +40,8 @@ void scale(cplx* x, u64 n, cplx s, hipStream_t st);
 void scale_copy(const cplx* x, cplx* y, u64 n, cplx s, hipStream_t st);
 // y = s * x (s on device, real)
 void scale_to(const cplx* x, cplx* y, u64 n, const double* s_dev, hipStream_t st);
+// s[j] = 1/sqrt(nrm2[j]) where sqrt(nrm2[j]) > thresh, else 0 (j < k; device)
+void inv_norm_or_zero(const double* nrm2, double* s, int k, double thresh, hipStream_t st);
 // y += a x  (host scalar)
 void axpy(cplx a, const cplx* x, cplx* y, u64 n, hipStream_t st);
 // w -= alpha v + beta u   (alpha, beta real device scalars; u may be null)

@@ -22,17 +22,24 @@ def _rand(d, seed, nvec=None):
 EXCHANGE_MODELS = {"heis_chain10_sz0", "heis_obc4_sz0", "j1j2_4x3_sz0", "heis_chain20_sz0"}
 
 
-@pytest.mark.parametrize("path", ["auto", "generic"])
+@pytest.mark.parametrize("path", ["auto", "generic", "rows"])
 @pytest.mark.parametrize("name", sorted(MODELS))
 def test_apply_matches_oracle(qk, orc, name, path, monkeypatch):
+    """auto: exchange kernel where it applies, else the row-program kernel;
+    generic: every group in the group loop; rows: the row-program kernel
+    also for exchange models."""
     print(f"[apply] {name} path={path}", flush=True)
-    if path == "generic":
-        monkeypatch.setenv("QKRYLOV_APPLY", "generic")
+    if path != "auto":
+        monkeypatch.setenv("QKRYLOV_APPLY", path)
     H, oh, _ = build_pair(orc, name)
+    info = H.info()
     if path == "auto" and name in EXCHANGE_MODELS:
-        assert H.info()["exchange_path"] == 1
+        assert info["exchange_path"] == 1
     if path == "generic":
-        assert H.info()["exchange_path"] == 0
+        assert info["exchange_path"] == 0 and info["row_path"] == 0
+    if path == "rows":
+        assert info["exchange_path"] == 0
+        assert info["row_path"] == 1 or info["state32"] == 0
     assert H.dimension == oh.dim
     x = _rand(H.dimension, 11)
     y = H.apply(x)
@@ -73,9 +80,13 @@ def test_matrix_columns(qk, orc, name):
         assert np.abs(M - M_ref).max() <= 1e-15 * np.abs(M_ref).max()
 
 
+@pytest.mark.parametrize("name", ["j1j2_4x3_sz0", "tj_4x3", "hubbard_chain8_half", "fermion_complex_11_n4"])
 @pytest.mark.parametrize("nvec", [2, 3, 5, 8, 11])
-def test_block_apply_equals_single(qk, orc, nvec):
-    H, oh, _ = build_pair(orc, "j1j2_4x3_sz0")
+def test_block_apply_equals_single(qk, orc, nvec, name):
+    """Blocks of 8, 4, 2 and 1 vectors per launch add each state's terms in
+    the same order as the single-vector kernel (exchange kernel for j1j2,
+    row-program kernel for the others): identical bits."""
+    H, oh, _ = build_pair(orc, name)
     X = _rand(H.dimension, 5, nvec)
     Y = H.apply(X)
     for v in range(nvec):
@@ -155,3 +166,20 @@ def test_hermiticity_and_linearity_n24(qk, orc, lat):
         ref = part.apply(x)
         got = Hx[r0:r0 + 4096]
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("name", ["tj_4x3", "hubbard_chain8_half", "heis_chain20_sz0"])
+def test_autotune_keeps_results(qk, orc, name):
+    """The autotuner only picks among kernels that compute the same y: after
+    tuning for blocks of 4, single and block applies still match the oracle."""
+    H, oh, _ = build_pair(orc, name)
+    tun = H.autotune(4)
+    print(f"[autotune] {name}: {tun}", flush=True)
+    assert tun["tuned"] == 1
+    X = _rand(H.dimension, 9, 5)
+    Y = H.apply(X)
+    for j in range(5):
+        y_ref = oh.apply(X[j])
+        assert np.abs(Y[j] - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
+    y1 = H.apply(X[2])
+    assert np.abs(y1 - oh.apply(X[2])).max() <= 1e-12 * np.abs(y1).max()
