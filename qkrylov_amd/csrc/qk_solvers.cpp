@@ -654,7 +654,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
             QK_HIP(hipMemcpyAsync(coef.get(), C.data(), sizeof(cplx) * C.size(), hipMemcpyHostToDevice, st));
             vec::lincomb_many(pI.get<const cplx*>(), 2 * m, coef.get<cplx>(), ne, Rb.get<cplx>(), d, d, st);
         }
-        for (int k = 0; k < ne; ++k) vec::norm2(Rb.get<cplx>() + u64(k) * d, d, nrm.get<double>() + k, red, st);
+        vec::norms(Rb.get<cplx>(), ne, d, d, nrm.get<double>(), red, st);
         std::vector<double> rn(ne);
         QK_HIP(hipMemcpyAsync(rn.data(), nrm.get(), sizeof(double) * ne, hipMemcpyDeviceToHost, st));
         QK_HIP(hipStreamSynchronize(st));
@@ -683,10 +683,9 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
                 cplx* hu = Nb.get<cplx>();
                 vec::lincomb_many(pV.get<const cplx*>(), m, coef.get<cplx>(), ne, Ub.get<cplx>(), d, d, st);
                 vec::lincomb_many(pHV.get<const cplx*>(), m, coef.get<cplx>(), ne, hu, d, d, st);
-                for (int k = 0; k < ne; ++k) {
+                for (int k = 0; k < ne; ++k)
                     vec::dot(Ub.get<cplx>() + u64(k) * d, hu + u64(k) * d, d, h.get<cplx>() + k, red, st);
-                    vec::norm2(Ub.get<cplx>() + u64(k) * d, d, nrm.get<double>() + k, red, st);
-                }
+                vec::norms(Ub.get<cplx>(), ne, d, d, nrm.get<double>(), red, st);
                 std::vector<cplx> uhu(ne);
                 std::vector<double> uu(ne);
                 QK_HIP(hipMemcpyAsync(uhu.data(), h.get(), sizeof(cplx) * ne, hipMemcpyDeviceToHost, st));
@@ -760,7 +759,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         const bool block = use_block_dot && nt > 1 && m0 <= 64 && nt <= 16;
         double* tn2 = tnorm.get<double>();   // [0, nt): before, [nt, 2nt): after the basis
         cplx* T = Tb.get<cplx>();
-        for (int j = 0; j < nt; ++j) vec::norm2(T + u64(j) * d, d, tn2 + j, red, st);
+        vec::norms(T, nt, d, d, tn2, red, st);
         if (block) {
             for (int pass = 0; pass < 2; ++pass) {
                 vec::block_dot(pV.get<const cplx*>(), m0, pT.get<const cplx*>(), nt, d, h.get<cplx>(), red, st);
@@ -769,7 +768,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         } else {
             for (int j = 0; j < nt; ++j) cgs2(pV, m0, T + u64(j) * d, d, h, red, st);
         }
-        for (int j = 0; j < nt; ++j) vec::norm2(T + u64(j) * d, d, tn2 + nt + j, red, st);
+        vec::norms(T, nt, d, d, tn2 + nt, red, st);
         std::vector<double> tn(2 * size_t(nt));
         if (nt > 0) {
             QK_HIP(hipMemcpyAsync(tn.data(), tn2, sizeof(double) * 2 * nt, hipMemcpyDeviceToHost, st));

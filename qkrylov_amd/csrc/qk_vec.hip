@@ -74,6 +74,25 @@ __global__ void __launch_bounds__(kT) k_norm_part(const cplx* __restrict__ a, u6
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// k_norm_part for k vectors at stride ld (vector j = blockIdx.y): the same
+// chunks and sums per vector, so each norm equals norm2's bit for bit
+__global__ void __launch_bounds__(kT) k_norms_part(const cplx* __restrict__ a, u64 n, u64 ld,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[4];
+    const cplx* x = a + u64(blockIdx.y) * ld;
+    const u64 start = u64(blockIdx.x) * kCH;
+    double s = 0.0;
+    for (int k = 0; k < kPer; ++k) {
+        const u64 i = start + u64(k) * kT + threadIdx.x;
+        if (i < n) {
+            const cplx v = x[i];
+            s += v.re * v.re + v.im * v.im;
+        }
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[u64(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
 // out[j] = sum_{b < nb} part[j*nb + b]   (one workgroup per j)
 __global__ void __launch_bounds__(kT) k_finish_c(const cplx* __restrict__ part, u32 nb,
                                                  cplx* __restrict__ out) {
@@ -598,6 +617,15 @@ void norm2(const cplx* a, u64 n, double* out, DevBuf& scratch, hipStream_t st) {
     scratch.resize(sizeof(double) * nb);
     hipLaunchKernelGGL(k_norm_part, dim3(nb), dim3(kT), 0, st, a, n, scratch.get<double>());
     hipLaunchKernelGGL(k_finish_r, dim3(1), dim3(kT), 0, st, scratch.get<double>(), nb, out);
+    QK_HIP(hipGetLastError());
+}
+
+void norms(const cplx* a, int k, u64 n, u64 ld, double* out, DevBuf& scratch, hipStream_t st) {
+    if (k <= 0) return;
+    const u32 nb = std::max<u32>(1, nblk(n));
+    scratch.resize(sizeof(double) * nb * size_t(k));
+    hipLaunchKernelGGL(k_norms_part, dim3(nb, k), dim3(kT), 0, st, a, n, ld, scratch.get<double>());
+    hipLaunchKernelGGL(k_finish_r, dim3(k), dim3(kT), 0, st, scratch.get<double>(), nb, out);
     QK_HIP(hipGetLastError());
 }
 

@@ -34,6 +34,9 @@ namespace vec {
 void dot(const cplx* a, const cplx* b, u64 n, cplx* out, DevBuf& scratch, hipStream_t st);
 // out[0] = sum |a|^2.
 void norm2(const cplx* a, u64 n, double* out, DevBuf& scratch, hipStream_t st);
+// out[j] = sum |a_j|^2 for the k vectors a_j = a + j*ld (device; one launch
+// pair, each value equal to norm2's)
+void norms(const cplx* a, int k, u64 n, u64 ld, double* out, DevBuf& scratch, hipStream_t st);
 // x *= s (host scalar)
 void scale(cplx* x, u64 n, cplx s, hipStream_t st);
 // y = s x  (host scalar)

@@ -192,10 +192,13 @@ def test_training_lowers_loss_and_davidson_agrees(qk, lat):
     # start (unit vectors on the lowest diagonal entries) spans no component
     # of eigenvectors in some symmetry sectors, H and every diagonal
     # preconditioner keep that so, and such an eigenvector enters only through
-    # accumulated rounding (the reference's Jacobi method misses the fourth
-    # eigenvalue of 2x5 that way).  Where the trained network and Jacobi
-    # return the same eigenvalues, the network must take fewer iterations;
-    # the 4x4 target lattice is measured by bench.py.
+    # accumulated rounding (asked for 4, the reference's Jacobi method misses
+    # the fourth eigenvalue of 2x5 that way, and which variant misses the
+    # fourth of 2x4 depends on rounding).  The lowest three are reached by
+    # every variant on both lattices, so that is what is asked for; where the
+    # trained network and Jacobi return the same eigenvalues, the network
+    # must take fewer iterations.  The 4x4 target lattice is measured by
+    # bench.py.
     from oracle import oracle as orc
     compared = 0
     for (lx, ly, k), tag in (((2, 4, 3), "2x4"), ((2, 5, 4), "2x5")):
@@ -206,12 +209,12 @@ def test_training_lowers_loss_and_davidson_agrees(qk, lat):
         res = {}
         for name, pre in (("jacobi", None), ("neumann", qk.MLPPreconditioner(3)), ("mlp", net)):
             t = time.time()
-            r = qk.davidson_lowest_ex(H, 4, 24, 1e-8, 400, pre)
+            r = qk.davidson_lowest_ex(H, 3, 24, 1e-8, 400, pre)
             res[name] = r
             ev = np.asarray(r.eigenvalues)
             dist = np.max(np.min(np.abs(ev[:, None] - spec[None, :]), axis=1))
             print(f"[davidson {tag}] {name}: iterations={r.iterations} converged={r.converged} "
-                  f"time={time.time() - t:.2f} s ev={ev} lowest={spec[:4]} "
+                  f"time={time.time() - t:.2f} s ev={ev} lowest={spec[:3]} "
                   f"max distance to the spectrum={dist:.1e}", flush=True)
             assert r.converged, name
             # a unit Ritz vector with residual r has an eigenvalue of the
