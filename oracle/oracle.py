@@ -43,6 +43,7 @@ def lib():
             "orc_threads": (i, []),
             "orc_basis_create": (vp, [i] * 10),
             "orc_basis_destroy": (None, [vp]),
+            "orc_basis_from_states": (vp, [i, i, vp, u64]),
             "orc_basis_dim": (u64, [vp]),
             "orc_basis_states": (None, [vp, vp]),
             "orc_basis_index": (i64, [vp, u64]),
@@ -123,6 +124,18 @@ class Basis:
         self.h = getattr(self.L, f"{self.prefix}_basis_create")(self.kind, n, *_sector_args(sector))
         if not self.h:
             raise ValueError("basis create failed")
+
+    @classmethod
+    def from_states(cls, kind: str, n: int, states) -> "Basis":
+        """Basis of an explicit ascending state list (states wider than 32
+        bits, where enumerating all 2^nbits states is out of reach)."""
+        b = cls.__new__(cls)
+        b.L, b.prefix, b.kind = lib(), "orc", KIND[kind]
+        arr = np.ascontiguousarray(np.asarray(states, dtype=np.uint64))
+        b.h = b.L.orc_basis_from_states(b.kind, n, _p(arr), arr.size)
+        if not b.h:
+            raise ValueError(b.L.orc_last_error().decode())
+        return b
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -741,6 +741,25 @@ void* orc_basis_create(int kind, int N, int use_sz, int sz2, int use_nup, int nu
         return nullptr;
     }
 }
+// A basis from an explicit ascending list of states: for states wider than
+// 32 bits, where the reference's loop over all 2^nbits states (make_basis)
+// is out of reach, the caller lists the sector's states itself.  Only the
+// state list and the site kind matter to the Hamiltonian built on it.
+void* orc_basis_from_states(int kind, int N, const uint64_t* states, uint64_t n) {
+    try {
+        auto* b = new Basis{kind, N, Sector{}, {}};
+        b->states.assign(states, states + n);
+        for (uint64_t i = 1; i < n; ++i)
+            if (b->states[i] <= b->states[i - 1]) {
+                delete b;
+                throw std::runtime_error("states must be strictly ascending");
+            }
+        return b;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
 void orc_basis_destroy(void* b) { delete static_cast<Basis*>(b); }
 uint64_t orc_basis_dim(void* b) { return static_cast<Basis*>(b)->size(); }
 void orc_basis_states(void* b, uint64_t* out) {

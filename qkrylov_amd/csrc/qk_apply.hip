@@ -770,12 +770,10 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
             return;
         }
     }
-    if constexpr (sizeof(S) == 4) {
-        if (h.rows && (B == 1 ? h.rows_1 : h.rows_b)) {
-            hipLaunchKernelGGL((k_apply<S, B, false, true>), dim3(h.grid), dim3(kGBlock), h.rlds_bytes,
-                               st, h.rest, ColTables{}, h.rparams, x, y, ld);
-            return;
-        }
+    if (h.rows && (B == 1 ? h.rows_1 : h.rows_b)) {
+        hipLaunchKernelGGL((k_apply<S, B, false, true>), dim3(h.grid), dim3(kGBlock), h.rlds_bytes, st,
+                           h.rest, ColTables{}, h.rparams, x, y, ld);
+        return;
     }
     hipLaunchKernelGGL((k_apply<S, B, false, false>), dim3(h.grid), dim3(kGBlock), h.lds_bytes, st,
                        h.params, ColTables{}, RowParams{}, x, y, ld);
@@ -944,7 +942,7 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
     force_generic = mode && std::string(mode) == "generic";
     const bool force_rows = mode && std::string(mode) == "rows";
     if (state32 && !force_generic && !force_rows) setup_exchange();
-    if (state32 && !force_generic && !xchg) setup_rows();
+    if (!force_generic && !xchg) setup_rows();
 }
 
 // Row-program path for the groups that lie inside one half (RowParams).
@@ -1129,8 +1127,11 @@ void Hamiltonian::setup_rows() {
     q.ndiag = u32(dlo_g.size() + dhi_g.size());
     rlds_bytes = lds;
     if (rlds_bytes > 64 * 1024) {
-        for (const void* f : {(const void*)k_apply<u32, 1, false, true>, (const void*)k_apply<u32, 2, false, true>,
-                              (const void*)k_apply<u32, 4, false, true>, (const void*)k_apply<u32, 8, false, true>})
+        const void* f32[] = {(const void*)k_apply<u32, 1, false, true>, (const void*)k_apply<u32, 2, false, true>,
+                             (const void*)k_apply<u32, 4, false, true>, (const void*)k_apply<u32, 8, false, true>};
+        const void* f64[] = {(const void*)k_apply<u64, 1, false, true>, (const void*)k_apply<u64, 2, false, true>,
+                             (const void*)k_apply<u64, 4, false, true>, (const void*)k_apply<u64, 8, false, true>};
+        for (const void* f : (state32 ? f32 : f64))
             QK_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, rlds_bytes));
     }
     rows = true;

@@ -130,7 +130,7 @@ struct ApplyParams {
 };
 
 // Row-program form of the groups that lie inside one half of the state
-// (32-bit states, one basis): low-half groups whose support and
+// (one basis): low-half groups whose support and
 // Jordan-Wigner bits are in the low half act inside a row the same way for
 // every row of a class, so per class they become a padded column-major table
 // of (target rank, amplitude id) per rank; high-half groups inside the high

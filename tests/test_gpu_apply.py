@@ -39,7 +39,7 @@ def test_apply_matches_oracle(qk, orc, name, path, monkeypatch):
         assert info["exchange_path"] == 0 and info["row_path"] == 0
     if path == "rows":
         assert info["exchange_path"] == 0
-        assert info["row_path"] == 1 or info["state32"] == 0
+        assert info["row_path"] == 1
     assert H.dimension == oh.dim
     x = _rand(H.dimension, 11)
     y = H.apply(x)

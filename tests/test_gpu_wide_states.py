@@ -1,0 +1,121 @@
+"""States wider than 32 bits (64-bit state words): bases, H·x on every
+kernel path and a Lanczos ground state against the oracle.  The oracle
+enumerates bases by looping over all 2^nbits states, as the reference does,
+which is out of reach here, so the sector's states are listed by
+combinations (ascending) and the oracle builds its CSR on that list
+(orc.Basis.from_states, pinned against the enumerating constructor on the
+CPU).  Bar: basis states bit-exact, y within 1e-12 relative, energies
+within 1e-10."""
+import itertools
+
+import numpy as np
+import pytest
+
+import qkrylov_amd as qkm
+from models import make_sector
+
+pytestmark = pytest.mark.gpu
+
+
+def _spin_states(n, ndown):
+    full = (1 << n) - 1
+    return sorted(full ^ sum(1 << i for i in c) for c in itertools.combinations(range(n), ndown))
+
+
+def _fermion_states(n, k):
+    return sorted(sum(1 << i for i in c) for c in itertools.combinations(range(n), k))
+
+
+def _two_species_states(n, nup, ndn, tj):
+    out = []
+    for up in itertools.combinations(range(n), nup):
+        for dn in itertools.combinations(range(n), ndn):
+            if tj and set(up) & set(dn):
+                continue
+            out.append(sum(1 << (2 * i) for i in up) | sum(1 << (2 * i + 1) for i in dn))
+    return sorted(out)
+
+
+def _models(lat):
+    def fermion_ops(n):
+        os = lat.spinless_fermion_chain(n, 1.0, 0.5, periodic=True)
+        for i in range(n):
+            os.add_term(0.3 * i - 0.4, "N", i)
+        os.add_term(0.25j, "C", 0, "Cdag", n - 1)
+        os.add_term(-0.25j, "C", n - 1, "Cdag", 0)
+        return os
+
+    def hubbard_ops(n):
+        os = lat.hubbard_model(lat.chain_bonds(n), n, 1.0, 4.0)
+        for i in range(n):
+            os.add_term(0.1 * i, "Nup", i)
+        return os
+
+    return {
+        # name: (basis kind, nsites, sector, states, opsum)
+        "heis_chain36_2down": ("spin", 36, {"use_sz": True, "sz2": 32}, _spin_states(36, 2),
+                               lat.heisenberg_chain(36)),
+        "fermion_ring40_n2": ("fermion", 40, {"use_n": True, "n": 2}, _fermion_states(40, 2),
+                              fermion_ops(40)),
+        "hubbard_chain17_1_2": ("hubbard", 17, {"use_nup": True, "nup": 1, "use_ndn": True, "ndn": 2},
+                                _two_species_states(17, 1, 2, False), hubbard_ops(17)),
+        "tj_chain18_2_1": ("tj", 18, {"use_nup": True, "nup": 2, "use_ndn": True, "ndn": 1},
+                           _two_species_states(18, 2, 1, True),
+                           lat.tj_model(lat.chain_bonds(18), 1.0, 0.4)),
+    }
+
+
+BASIS = {"spin": "SpinHalfBasis", "fermion": "FermionBasis", "hubbard": "HubbardBasis", "tj": "TJBasis"}
+SITE = {"spin": "SpinHalfSite", "fermion": "FermionSite", "hubbard": "HubbardSite", "tj": "TJSite"}
+
+
+def _build(qk, orc, lat, name):
+    kind, n, sec, states, ops = _models(lat)[name]
+    b = getattr(qk, BASIS[kind])(n, sector=make_sector(sec))
+    H = qk.MatrixFreeHamiltonian(b, getattr(qk, SITE[kind])(), ops)
+    ob = orc.Basis.from_states(kind, n, states)
+    oh = orc.Ham(ob, kind, ops)
+    return b, H, ob, oh, np.asarray(states, dtype=np.uint64)
+
+
+NAMES = ["heis_chain36_2down", "fermion_ring40_n2", "hubbard_chain17_1_2", "tj_chain18_2_1"]
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_wide_basis_states(qk, orc, lat, name):
+    b, H, ob, oh, states = _build(qk, orc, lat, name)
+    assert b.size == states.size
+    got = np.array([b.state(i) for i in range(b.size)], dtype=np.uint64)
+    assert np.array_equal(got, states)
+    assert H.info()["state32"] == 0
+
+
+@pytest.mark.parametrize("path", ["auto", "generic", "rows"])
+@pytest.mark.parametrize("name", NAMES)
+def test_wide_apply_matches_oracle(qk, orc, lat, name, path, monkeypatch):
+    print(f"[wide apply] {name} path={path}", flush=True)
+    if path != "auto":
+        monkeypatch.setenv("QKRYLOV_APPLY", path)
+    b, H, ob, oh, states = _build(qk, orc, lat, name)
+    info = H.info()
+    print(f"[wide apply] {name}: {info}", flush=True)
+    assert info["row_path"] == (0 if path == "generic" else 1)
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal((3, b.size)) + 1j * rng.standard_normal((3, b.size))
+    Y = H.apply(X)
+    for j in range(3):
+        y_ref = oh.apply(X[j])
+        scale = np.abs(y_ref).max()
+        assert np.abs(Y[j] - y_ref).max() <= 1e-12 * scale
+        np.testing.assert_array_equal(H.apply(X[j]), Y[j])
+    # the diagonal, term by term in OpSum order, bit-exact
+    assert np.array_equal(H.diagonal().view(np.float64), oh.diag().view(np.float64))
+
+
+@pytest.mark.parametrize("name", ["heis_chain36_2down", "tj_chain18_2_1"])
+def test_wide_lanczos_matches_dense(qk, orc, lat, name):
+    b, H, ob, oh, states = _build(qk, orc, lat, name)
+    e_ref = np.linalg.eigvalsh(oh.to_scipy().toarray())[0]
+    r = qk.lanczos_ground_state(H, 300, 1e-12)
+    print(f"[wide lanczos] {name}: E0={r.energy:.12f} dense={e_ref:.12f} it={r.iterations}", flush=True)
+    assert abs(r.energy - e_ref) <= 1e-10 * max(1.0, abs(e_ref))

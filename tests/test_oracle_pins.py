@@ -268,3 +268,22 @@ def test_q_zero_structure_factor_vanishes(orc, lat):
     psi = np.ones(ob.size, complex)
     a, b, nrm = oh.cf(sz_tot * psi, 10)
     assert len(a) == 0 and nrm == 0.0
+
+
+def test_basis_from_states_equals_enumeration(orc, lat):
+    """The explicit-list constructor (for states wider than 32 bits) gives the
+    same basis and the same CSR as the enumerating one on sectors both reach."""
+    import itertools
+    n = 8
+    b_enum = orc.Basis("hubbard", n, {"use_nup": True, "nup": 3, "use_ndn": True, "ndn": 2})
+    states = sorted(
+        sum(1 << (2 * i) for i in up) | sum(1 << (2 * i + 1) for i in dn)
+        for up in itertools.combinations(range(n), 3) for dn in itertools.combinations(range(n), 2))
+    b_list = orc.Basis.from_states("hubbard", n, states)
+    assert np.array_equal(b_enum.states(), b_list.states())
+    ops = lat.hubbard_model(lat.chain_bonds(n), n, 1.0, 4.0)
+    a = orc.Ham(b_enum, "hubbard", ops).to_scipy()
+    c = orc.Ham(b_list, "hubbard", ops).to_scipy()
+    assert (a != c).nnz == 0
+    with pytest.raises(ValueError):
+        orc.Basis.from_states("hubbard", n, [5, 3])
