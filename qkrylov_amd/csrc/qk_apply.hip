@@ -316,7 +316,7 @@ k_apply_xchg(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y, 
 
         double dg = p.dlo[T.pos0 + lane] + T.dh;
         if (p.n_dmix) {
-            const u32 s = (hi << p.L) | lo;
+            const u64 s = (u64(hi) << p.L) | lo;
             for (u32 g = 0; g < p.n_dmix; ++g) {
                 const DevGroup G = p.dmix[g];
                 dg += s_tab[G.tab + config_of(s, G)].re;
@@ -412,7 +412,7 @@ k_apply_xchg_multi(const XParams p, const MParams mp, const cplx* __restrict__ x
         const u64 a = T.base + r;
         double dg = p.dlo[T.pos0 + lane] + T.dh;
         if (p.n_dmix) {
-            const u32 s = (hi << p.L) | lo;
+            const u64 s = (u64(hi) << p.L) | lo;
             for (u32 g = 0; g < p.n_dmix; ++g) {
                 const DevGroup G = p.dmix[g];
                 dg += s_tab[G.tab + config_of(s, G)].re;
@@ -642,7 +642,7 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
             lo[k] = v[k] ? p.lo_list[T.pos0 + j] : 0u;
             double dg = v[k] ? p.dlo[T.pos0 + j] + T.dh : 0.0;
             if (p.n_dmix) {
-                const u32 s = (hi << p.L) | lo[k];
+                const u64 s = (u64(hi) << p.L) | lo[k];
                 for (u32 g = 0; g < p.n_dmix; ++g) {
                     const DevGroup G = p.dmix[g];
                     dg += s_tab[G.tab + config_of(s, G)].re;
@@ -755,20 +755,20 @@ k_sz_q(const ApplyParams p, const cplx* __restrict__ phase, int nsites, int nq,
 
 template <typename S, int B>
 void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStream_t st) {
-    if constexpr (sizeof(S) == 4) {
-        if (h.xchg) {
-            if constexpr (B == 1) {
-                const cplx* xa = x;
-                cplx* ya = y;
-                void* args[] = {const_cast<XParams*>(&h.xparams_s), &xa, &ya};
-                QK_HIP(hipLaunchKernel(h.xs_fn, dim3(h.xgrid_s), dim3(64 * kXWaves), args,
-                                       size_t(h.xlds_bytes), st));
-                return;
-            }
-            hipLaunchKernelGGL((k_apply_xchg<B>), dim3(h.xgrid), dim3(64 * kXWaves), h.xlds_bytes,
-                               st, h.xparams, x, y, ld);
+    // the exchange kernels work on (row, rank) pairs and never form a state
+    // word except as 64 bits, so they serve both state widths
+    if (h.xchg) {
+        if constexpr (B == 1) {
+            const cplx* xa = x;
+            cplx* ya = y;
+            void* args[] = {const_cast<XParams*>(&h.xparams_s), &xa, &ya};
+            QK_HIP(hipLaunchKernel(h.xs_fn, dim3(h.xgrid_s), dim3(64 * kXWaves), args,
+                                   size_t(h.xlds_bytes), st));
             return;
         }
+        hipLaunchKernelGGL((k_apply_xchg<B>), dim3(h.xgrid), dim3(64 * kXWaves), h.xlds_bytes, st,
+                           h.xparams, x, y, ld);
+        return;
     }
     if (h.rows && (B == 1 ? h.rows_1 : h.rows_b)) {
         hipLaunchKernelGGL((k_apply<S, B, false, true>), dim3(h.grid), dim3(kGBlock), h.rlds_bytes, st,
@@ -784,7 +784,7 @@ void launch_apply_s(const Hamiltonian& h, const cplx* x, cplx* y, int nvec, u64 
                     hipStream_t st) {
     // the exchange path may run blocks as groups of fewer vectors (1: one
     // single-vector launch per vector), as chosen at set-up
-    const int gmax = (sizeof(S) == 4 && h.xchg) ? h.vec_group : h.gen_group;
+    const int gmax = h.xchg ? h.vec_group : h.gen_group;
     int v = 0;
     while (v < nvec) {
         const int left = std::min(nvec - v, gmax);
@@ -941,7 +941,7 @@ Hamiltonian::Hamiltonian(std::shared_ptr<const Basis> b, SiteKind sk, const OpSu
     const char* mode = std::getenv("QKRYLOV_APPLY");
     force_generic = mode && std::string(mode) == "generic";
     const bool force_rows = mode && std::string(mode) == "rows";
-    if (state32 && !force_generic && !force_rows) setup_exchange();
+    if (!force_generic && !force_rows) setup_exchange();
     if (!force_generic && !xchg) setup_rows();
 }
 
@@ -1558,7 +1558,7 @@ Hamiltonian::~Hamiltonian() {
 }
 
 bool Hamiltonian::apply_dot(const cplx* x, cplx* y, double* part, hipStream_t st) const {
-    if (!xchg || !state32 || !xs_fn || dim() == 0) return false;
+    if (!xchg || !xs_fn || dim() == 0) return false;
     if (!st) st = stream;
     XParams q = xparams_s;
     q.dot_part = part;

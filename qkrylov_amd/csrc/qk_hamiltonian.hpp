@@ -183,7 +183,7 @@ struct Hamiltonian {
     // apply_dot is available and apply() of nvec vectors runs that same
     // kernel vector by vector
     bool fused_dot(int nvec) const {
-        return xchg && state32 && xs_fn && dim() > 0 && (vec_group == 1 || nvec == 1);
+        return xchg && xs_fn && dim() > 0 && (vec_group == 1 || nvec == 1);
     }
     // Full complex diagonal in OpSum order (device output, dim entries).
     void diagonal(cplx* d, hipStream_t st) const;

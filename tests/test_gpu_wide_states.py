@@ -99,7 +99,12 @@ def test_wide_apply_matches_oracle(qk, orc, lat, name, path, monkeypatch):
     b, H, ob, oh, states = _build(qk, orc, lat, name)
     info = H.info()
     print(f"[wide apply] {name}: {info}", flush=True)
-    assert info["row_path"] == (0 if path == "generic" else 1)
+    if path == "generic":
+        assert info["exchange_path"] == 0 and info["row_path"] == 0
+    elif path == "rows" or not name.startswith("heis"):
+        assert info["exchange_path"] == 0 and info["row_path"] == 1
+    else:   # a real two-bit exchange model: the exchange kernels, 64-bit states
+        assert info["exchange_path"] == 1
     rng = np.random.default_rng(17)
     X = rng.standard_normal((3, b.size)) + 1j * rng.standard_normal((3, b.size))
     Y = H.apply(X)
@@ -114,8 +119,17 @@ def test_wide_apply_matches_oracle(qk, orc, lat, name, path, monkeypatch):
 
 @pytest.mark.parametrize("name", ["heis_chain36_2down", "tj_chain18_2_1"])
 def test_wide_lanczos_matches_dense(qk, orc, lat, name):
+    """Also the block recurrence on 64-bit states (CF with the apply's fused
+    alpha on the exchange path), against the oracle's CF."""
     b, H, ob, oh, states = _build(qk, orc, lat, name)
     e_ref = np.linalg.eigvalsh(oh.to_scipy().toarray())[0]
     r = qk.lanczos_ground_state(H, 300, 1e-12)
     print(f"[wide lanczos] {name}: E0={r.energy:.12f} dense={e_ref:.12f} it={r.iterations}", flush=True)
     assert abs(r.energy - e_ref) <= 1e-10 * max(1.0, abs(e_ref))
+    rng = np.random.default_rng(8)
+    phi = rng.standard_normal((2, b.size)) + 1j * rng.standard_normal((2, b.size))
+    cf = qk.continued_fraction_block(H, phi, 20)
+    for j in range(2):
+        a_ref, b_ref, n_ref = oh.cf(phi[j], 20)
+        assert np.abs(np.asarray(cf[j].alphas[:8]) - a_ref[:8]).max() < 1e-10
+        assert abs(cf[j].norm_phi0 - n_ref) <= 1e-13 * n_ref
