@@ -465,20 +465,48 @@ k_block_dot_part(const cplx* const* __restrict__ V, const int m, const cplx* con
     }
     const u64 w0 = (u64(blockIdx.x) * 4 + wave) * per_wave;
     const u64 w1 = min(n, w0 + per_wave);
-    for (u64 c = w0; c < w1; c += 2) {
-        const u64 cc = c + half;
-        const bool in = cc < w1;
-        cplx wv{0.0, 0.0};
-        if (in && wp) wv = wp[cc];
-        const double bre = im ? wv.im : wv.re;
-        const double bsw = im ? -wv.re : wv.im;
+    // 8 entries per round: lane half h takes entries c + 4h .. c + 4h + 3 in
+    // the four MFMA steps, so each lane reads 64 contiguous bytes of each of
+    // its vectors (L1 line reuse within the round instead of across rounds).
+    // Loads past the slice or for absent vectors read a valid entry and are
+    // replaced by zero, so no load sits under a branch.
+    const bool wok = wp != nullptr;
+    const cplx* wq = wok ? wp : W[0];
+    bool vok[MB];
+    const cplx* vq[MB];
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        vok[b] = vp[b] != nullptr;
+        vq[b] = vok[b] ? vp[b] : V[0];
+    }
+    for (u64 c = w0; c < w1; c += 8) {
+        const u64 e0 = c + 4 * u64(half);
+        double wr[4], wi[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cplx t = wq[min(e0 + s, w1 - 1)];
+            const bool ok = wok && e0 + s < w1;
+            wr[s] = ok ? t.re : 0.0;
+            wi[s] = ok ? t.im : 0.0;
+        }
 #pragma unroll
         for (int b = 0; b < MB; ++b) {
-            cplx vv{0.0, 0.0};
-            if (in && vp[b]) vv = vp[b][cc];
-            const double a = im ? vv.im : vv.re;
-            acc_re[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bre, acc_re[b], 0, 0, 0);
-            acc_im[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bsw, acc_im[b], 0, 0, 0);
+            double vr[4], vi[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cplx t = vq[b][min(e0 + s, w1 - 1)];
+                const bool ok = vok[b] && e0 + s < w1;
+                vr[s] = ok ? t.re : 0.0;
+                vi[s] = ok ? t.im : 0.0;
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const double a = im ? vi[s] : vr[s];
+                const double bre = im ? wi[s] : wr[s];
+                const double bsw = im ? -wr[s] : wi[s];
+                acc_re[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bre, acc_re[b], 0, 0, 0);
+                acc_im[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bsw, acc_im[b], 0, 0, 0);
+            }
         }
     }
     // reduce the 4 waves in LDS (fixed order), write this workgroup's partial
@@ -693,7 +721,7 @@ void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n,
     // keep long slices, short ones still spread over the chip
     const u64 waves = std::max<u64>(1, std::min<u64>((n + kBDMin - 1) / kBDMin, 4096));
     u64 per_wave = (n + waves - 1) / waves;
-    per_wave = (per_wave + 1) & ~u64(1);                // even: pairs of entries
+    per_wave = (per_wave + 7) & ~u64(7);                // whole rounds of 8 entries
     const u32 nparts = u32((((n + per_wave - 1) / per_wave) + 3) / 4);
     const int MB = (m + 15) / 16;
     const int stride = MB * 16 * 16;
