@@ -760,19 +760,32 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
         double* tn2 = tnorm.get<double>();   // [0, nt): before, [nt, 2nt): after the basis
         cplx* T = Tb.get<cplx>();
         vec::norms(T, nt, d, d, tn2, red, st);
+        std::vector<double> tn(2 * size_t(nt));
+        auto fetch_norms = [&](int first, int count) {
+            vec::norms(T, nt, d, d, tn2 + nt, red, st);
+            QK_HIP(hipMemcpyAsync(tn.data() + first, tn2 + first, sizeof(double) * count,
+                                  hipMemcpyDeviceToHost, st));
+            QK_HIP(hipStreamSynchronize(st));
+        };
+        auto block_pass = [&]() {
+            vec::block_dot(pV.get<const cplx*>(), m0, pT.get<const cplx*>(), nt, d, h.get<cplx>(), red, st);
+            vec::block_sub(pV.get<const cplx*>(), m0, pT.get<cplx*>(), nt, h.get<cplx>(), d, st);
+        };
         if (block) {
-            for (int pass = 0; pass < 2; ++pass) {
-                vec::block_dot(pV.get<const cplx*>(), m0, pT.get<const cplx*>(), nt, d, h.get<cplx>(), red, st);
-                vec::block_sub(pV.get<const cplx*>(), m0, pT.get<cplx*>(), nt, h.get<cplx>(), d, st);
+            // one block projection, and a second only when a correction lost
+            // more than half its squared norm in the first (the Daniel-Gragg-
+            // Kaufman-Stewart test, as in Lanczos)
+            block_pass();
+            if (nt > 0) fetch_norms(0, 2 * nt);
+            bool again = false;
+            for (int j = 0; j < nt; ++j) again = again || tn[nt + j] < 0.5 * tn[j];
+            if (again) {
+                block_pass();
+                fetch_norms(nt, nt);
             }
         } else {
             for (int j = 0; j < nt; ++j) cgs2(pV, m0, T + u64(j) * d, d, h, red, st);
-        }
-        vec::norms(T, nt, d, d, tn2 + nt, red, st);
-        std::vector<double> tn(2 * size_t(nt));
-        if (nt > 0) {
-            QK_HIP(hipMemcpyAsync(tn.data(), tn2, sizeof(double) * 2 * nt, hipMemcpyDeviceToHost, st));
-            QK_HIP(hipStreamSynchronize(st));
+            if (nt > 0) fetch_norms(0, 2 * nt);
         }
         // a correction that lost most of its norm to the basis is left
         // orthogonal to it only to rounding relative to its original norm:
