@@ -281,29 +281,32 @@ void MlpPreconditioner::apply_many(const Hamiltonian& H, int nt, const cplx* con
     o_.resize(sizeof(float) * rows * nt * kMlpOut);
     z_.resize(sizeof(cplx) * tot);
     w_.resize(sizeof(cplx) * tot);
-    c_.resize(sizeof(cplx) * nt);
     cplx* z = z_.get<cplx>();
     cplx* w = w_.get<cplx>();
-    for (int j = 0; j < nt; ++j) mlp_jacobi(states_, lambda[j], r[j], z + u64(j) * n, st);
-    H.apply(z, w, nt, n, st);   // w holds H z until the Neumann term overwrites it
-    for (int j = 0; j < nt; ++j) {
-        mlp_neumann(states_, lambda[j], w + u64(j) * n, z + u64(j) * n, w + u64(j) * n, st);
-        u16* Xj = X_.get<u16>() + u64(j) * rows * kMlpIn;
-        if (states_.ncls) mlp_features_cls(states_, lambda[j], Xj, st);
-        else mlp_features(states_, lambda[j], Xj, st);
-    }
-    net_.forward(X_.get<u16>(), rows * u64(nt), o_.get<float>(), nullptr, nullptr, st);
-    for (int j = 0; j < nt; ++j) {
-        const float* oj = o_.get<float>() + u64(j) * rows * kMlpOut;
-        if (states_.ncls) mlp_combine_cls(states_, oj, z + u64(j) * n, w + u64(j) * n, t[j], st);
-        else mlp_combine(states_, oj, z + u64(j) * n, w + u64(j) * n, t[j], st);
-        if (u && u[j]) {
-            // t approximates (H - lambda)^-1 r, which is u itself for an exact
-            // Ritz pair: remove the u component here, before Davidson
-            // orthogonalises t against its basis
-            vec::dot(u[j], t[j], n, c_.get<cplx>() + j, red_, st);
-            mlp_project(u[j], c_.get<cplx>() + j, t[j], n, st);
+    // the corrections in batches of up to kMlpBatch, one launch per stage
+    for (int j0 = 0; j0 < nt; j0 += kMlpBatch) {
+        const int nb = std::min(kMlpBatch, nt - j0);
+        MlpBatch b{};
+        b.nt = nb;
+        for (int j = 0; j < nb; ++j) {
+            b.r[j] = r[j0 + j];
+            b.t[j] = t[j0 + j];
+            b.u[j] = u ? u[j0 + j] : nullptr;
+            b.theta[j] = lambda[j0 + j];
         }
+        cplx* zb = z + u64(j0) * n;
+        cplx* wb = w + u64(j0) * n;
+        mlp_jacobi_many(states_, b, zb, st);
+        H.apply(zb, wb, nb, n, st);   // w holds H z until the Neumann term overwrites it
+        mlp_neumann_many(states_, b, wb, zb, wb, st);
+        u16* Xb = X_.get<u16>() + u64(j0) * rows * kMlpIn;
+        float* ob = o_.get<float>() + u64(j0) * rows * kMlpOut;
+        mlp_features_many(states_, b, Xb, st);
+        net_.forward(Xb, rows * u64(nb), ob, nullptr, nullptr, st);
+        // t approximates (H - lambda)^-1 r, which is u itself for an exact
+        // Ritz pair: the u component is removed here, before Davidson
+        // orthogonalises t against its basis
+        mlp_combine_project_many(states_, b, ob, zb, wb, red_, st);
     }
 }
 

@@ -58,7 +58,7 @@ public:
 private:
     const Mlp& net_;
     MlpStates states_;
-    DevBuf X_, o_, z_, w_, c_, red_;
+    DevBuf X_, o_, z_, w_, red_;
 };
 
 }  // namespace qk

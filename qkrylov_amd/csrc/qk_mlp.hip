@@ -390,12 +390,10 @@ __device__ __forceinline__ double jac_den(double d, double theta) {
     return den;
 }
 
-__global__ void k_mlp_features(const double* __restrict__ diag, const float* __restrict__ stat,
-                               const u64 n, const double theta, const double inv_sites,
-                               u16* __restrict__ X) {
-    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double d = diag[i];
+// one network input row from H_ii (d), the state's five statistics and theta
+__device__ __forceinline__ void mlp_feature_row(const double d, const float* __restrict__ st5,
+                                                const double theta, const double inv_sites,
+                                                u16* __restrict__ Xrow) {
     const double den = jac_den(d, theta);
     float f[kIn];
 #pragma unroll
@@ -404,9 +402,9 @@ __global__ void k_mlp_features(const double* __restrict__ diag, const float* __r
     f[1] = float(fmin(fmax(1.0 / den, -4.0), 4.0) * 0.25);
     f[2] = float(theta * inv_sites);
     f[3] = float(d * inv_sites);
-    f[4] = stat[i * 5 + 4];
+    f[4] = st5[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) f[5 + k] = stat[i * 5 + k];
+    for (int k = 0; k < 4; ++k) f[5 + k] = st5[k];
     f[9] = float(tanh(d - theta));
     u16x8 v[kIn / 8];
 #pragma unroll
@@ -414,7 +412,15 @@ __global__ void k_mlp_features(const double* __restrict__ diag, const float* __r
 #pragma unroll
         for (int j = 0; j < 8; ++j) v[q][j] = d_f2bf(f[q * 8 + j]);
 #pragma unroll
-    for (int q = 0; q < kIn / 8; ++q) reinterpret_cast<u16x8*>(X + i * kIn)[q] = v[q];
+    for (int q = 0; q < kIn / 8; ++q) reinterpret_cast<u16x8*>(Xrow)[q] = v[q];
+}
+
+__global__ void k_mlp_features(const double* __restrict__ diag, const float* __restrict__ stat,
+                               const u64 n, const double theta, const double inv_sites,
+                               u16* __restrict__ X) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    mlp_feature_row(diag[i], stat + i * 5, theta, inv_sites, X + i * kIn);
 }
 
 __global__ void k_mlp_jacobi(const double* __restrict__ diag, const u64 n, const double theta,
@@ -461,6 +467,121 @@ __global__ void k_mlp_combine_cls(const u64 n, const float* __restrict__ o, cons
     double s1, s2, ta, tb;
     mlp_coef(o, cls[i], s1, s2, ta, tb);
     t[i] = cplx{s1 * z[i].re + s2 * w[i].re, s1 * z[i].im + s2 * w[i].im};
+}
+
+// ---- batched over the corrections of one iteration (grid.y = j) ----------
+__global__ void k_mlp_jacobi_many(const double* __restrict__ diag, const u64 n, const MlpBatch b,
+                                  cplx* __restrict__ z) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= n) return;
+    const double f = 1.0 / jac_den(diag[i], b.theta[j]);
+    const cplx r = b.r[j][i];
+    z[u64(j) * n + i] = cplx{f * r.re, f * r.im};
+}
+
+__global__ void k_mlp_neumann_many(const double* __restrict__ diag, const u64 n, const MlpBatch b,
+                                   const cplx* hz, const cplx* __restrict__ z, cplx* w) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= n) return;
+    const double d = diag[i];
+    const double f = 1.0 / jac_den(d, b.theta[j]);
+    const u64 k = u64(j) * n + i;
+    const cplx h = hz[k], zz = z[k];
+    w[k] = cplx{f * (h.re - d * zz.re), f * (h.im - d * zz.im)};
+}
+
+// the features of k_mlp_features for row i of correction j
+__global__ void k_mlp_features_many(const double* __restrict__ diag, const float* __restrict__ stat,
+                                    const u64 rows, const MlpBatch b, const double inv_sites,
+                                    u16* __restrict__ X) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= rows) return;
+    mlp_feature_row(diag[i], stat + i * 5, b.theta[j], inv_sites, X + (u64(j) * rows + i) * kIn);
+}
+
+// t_j = s1 z_j + s2 w_j; partial sums of <u_j, t_j> per workgroup
+constexpr int kCombT = 256;
+__global__ void __launch_bounds__(kCombT)
+k_mlp_combine_many(const u64 n, const float* __restrict__ o, const u64 rows, const u32* __restrict__ cls,
+                   const cplx* __restrict__ z, const cplx* __restrict__ w, const MlpBatch b,
+                   double* __restrict__ part) {
+    const u64 i = u64(blockIdx.x) * kCombT + threadIdx.x;
+    const int j = blockIdx.y;
+    double dr = 0.0, di = 0.0;
+    if (i < n) {
+        double s1, s2, ta, tb;
+        mlp_coef(o + u64(j) * rows * kMlpOut, cls ? cls[i] : i, s1, s2, ta, tb);
+        const u64 k = u64(j) * n + i;
+        const cplx t = cplx{s1 * z[k].re + s2 * w[k].re, s1 * z[k].im + s2 * w[k].im};
+        b.t[j][i] = t;
+        if (b.u[j]) {
+            const cplx uu = b.u[j][i];
+            dr = uu.re * t.re + uu.im * t.im;
+            di = uu.re * t.im - uu.im * t.re;
+        }
+    }
+    __shared__ double sh[2][kCombT / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        dr += __shfl_xor(dr, off);
+        di += __shfl_xor(di, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = dr;
+        sh[1][threadIdx.x >> 6] = di;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int q = 0; q < kCombT / 64; ++q) {
+            a += sh[0][q];
+            c += sh[1][q];
+        }
+        part[(u64(j) * gridDim.x + blockIdx.x) * 2] = a;
+        part[(u64(j) * gridDim.x + blockIdx.x) * 2 + 1] = c;
+    }
+}
+
+// c_j = sum of correction j's partials (fixed order), one workgroup per j
+__global__ void __launch_bounds__(kCombT) k_mlp_dot_fin(const double* __restrict__ part, const int nblk,
+                                                       cplx* __restrict__ c) {
+    const int j = blockIdx.x;
+    double a = 0.0, v = 0.0;
+    for (int q = threadIdx.x; q < nblk; q += kCombT) {
+        a += part[(size_t(j) * nblk + q) * 2];
+        v += part[(size_t(j) * nblk + q) * 2 + 1];
+    }
+    __shared__ double sh[2][kCombT / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        v += __shfl_xor(v, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = a;
+        sh[1][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+        for (int q = 0; q < kCombT / 64; ++q) {
+            x += sh[0][q];
+            y += sh[1][q];
+        }
+        c[j] = cplx{x, y};
+    }
+}
+
+__global__ void k_mlp_project_many(const u64 n, const MlpBatch b, const cplx* __restrict__ c) {
+    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= n || !b.u[j]) return;
+    const cplx cc = c[j], uu = b.u[j][i];
+    cplx t = b.t[j][i];
+    t.re -= cc.re * uu.re - cc.im * uu.im;
+    t.im -= cc.re * uu.im + cc.im * uu.re;
+    b.t[j][i] = t;
 }
 
 __global__ void k_mlp_project(const cplx* __restrict__ u, const cplx* __restrict__ c,
@@ -797,6 +918,51 @@ void mlp_combine(const MlpStates& s, const float* o, const cplx* z, const cplx* 
     if (!s.n) return;
     hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st, s.n, o,
                        z, w, t);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_jacobi_many(const MlpStates& s, const MlpBatch& b, cplx* z, hipStream_t st) {
+    if (!s.n || b.nt <= 0) return;
+    hipLaunchKernelGGL(k_mlp_jacobi_many, dim3(unsigned((s.n + 255) / 256), b.nt), dim3(256), 0, st,
+                       s.diag.get<double>(), s.n, b, z);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_neumann_many(const MlpStates& s, const MlpBatch& b, const cplx* hz, const cplx* z, cplx* w,
+                      hipStream_t st) {
+    if (!s.n || b.nt <= 0) return;
+    hipLaunchKernelGGL(k_mlp_neumann_many, dim3(unsigned((s.n + 255) / 256), b.nt), dim3(256), 0, st,
+                       s.diag.get<double>(), s.n, b, hz, z, w);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_features_many(const MlpStates& s, const MlpBatch& b, u16* X, hipStream_t st) {
+    if (!s.n || b.nt <= 0) return;
+    const u64 rows = s.ncls ? s.ncls : s.n;
+    const double* d = s.ncls ? s.rdiag.get<double>() : s.diag.get<double>();
+    const float* f = s.ncls ? s.rstat.get<float>() : s.stat.get<float>();
+    hipLaunchKernelGGL(k_mlp_features_many, dim3(unsigned((rows + 255) / 256), b.nt), dim3(256), 0, st,
+                       d, f, rows, b, 1.0 / s.nsites, X);
+    QK_HIP(hipGetLastError());
+}
+
+void mlp_combine_project_many(const MlpStates& s, const MlpBatch& b, const float* o, const cplx* z,
+                              const cplx* w, DevBuf& scratch, hipStream_t st) {
+    if (!s.n || b.nt <= 0) return;
+    const u64 rows = s.ncls ? s.ncls : s.n;
+    const int nblk = int((s.n + kCombT - 1) / kCombT);
+    scratch.resize(sizeof(double) * 2 * size_t(nblk) * b.nt + sizeof(cplx) * kMlpBatch);
+    double* part = scratch.get<double>();
+    cplx* c = reinterpret_cast<cplx*>(part + 2 * size_t(nblk) * b.nt);
+    hipLaunchKernelGGL(k_mlp_combine_many, dim3(nblk, b.nt), dim3(kCombT), 0, st, s.n, o, rows,
+                       s.ncls ? s.cls.get<u32>() : nullptr, z, w, b, part);
+    bool any_u = false;
+    for (int j = 0; j < b.nt; ++j) any_u = any_u || b.u[j];
+    if (any_u) {
+        hipLaunchKernelGGL(k_mlp_dot_fin, dim3(b.nt), dim3(kCombT), 0, st, part, nblk, c);
+        hipLaunchKernelGGL(k_mlp_project_many, dim3(unsigned((s.n + 255) / 256), b.nt), dim3(256), 0, st,
+                           s.n, b, c);
+    }
     QK_HIP(hipGetLastError());
 }
 

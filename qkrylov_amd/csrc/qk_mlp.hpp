@@ -119,6 +119,30 @@ struct MlpStates {
     void build(const Hamiltonian& H, hipStream_t st);
 };
 
+// The corrections of one Davidson iteration, passed by value to the batched
+// kernels below (grid.y = correction): residuals r, outputs t, Ritz vectors
+// u (null: none), shifts theta.
+constexpr int kMlpBatch = 16;
+struct MlpBatch {
+    const cplx* r[kMlpBatch];
+    cplx* t[kMlpBatch];
+    const cplx* u[kMlpBatch];
+    double theta[kMlpBatch];
+    int nt;
+};
+// z_j = J_j r_j and, after hz_j = H z_j, w_j = J_j (hz_j - diag z_j) (w may
+// be hz); z, w hold the nt vectors contiguously (n each)
+void mlp_jacobi_many(const MlpStates& s, const MlpBatch& b, cplx* z, hipStream_t st);
+void mlp_neumann_many(const MlpStates& s, const MlpBatch& b, const cplx* hz, const cplx* z, cplx* w,
+                      hipStream_t st);
+// features of every network row of every correction: rows = ncls (classes)
+// or n (states) per correction, correction j at X + j * rows * kIn
+void mlp_features_many(const MlpStates& s, const MlpBatch& b, u16* X, hipStream_t st);
+// t_j = s1 z_j + s2 w_j from output rows j * rows + (class of the state or
+// the state), then t_j -= <u_j, t_j> u_j where u_j is given
+void mlp_combine_project_many(const MlpStates& s, const MlpBatch& b, const float* o, const cplx* z,
+                              const cplx* w, DevBuf& scratch, hipStream_t st);
+
 // Features of the ncls class representatives for theta (device).
 void mlp_features_cls(const MlpStates& s, double theta, u16* X, hipStream_t st);
 // t = s1 z + s2 w with (s1, s2) from the output row of the state's class
