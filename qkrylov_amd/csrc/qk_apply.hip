@@ -655,15 +655,16 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
         u32 b = 0;
         for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB>(p.bonds + b, s_jlo, x, base, lo, v, acc);
         for (; b < p.n_lo; ++b) xbonds_lo<SPL, 1>(p.bonds + b, s_jlo, x, base, lo, v, acc);
+        const u32 vr = p.vrow ? p.vrow[hi] : hi;   // record slot of this row
         {
-            const u32 hc = p.hcnt[hi];
-            const XRowNb* nb = p.hnb + size_t(hi) * p.n_hi;
+            const u32 hc = p.hcnt[vr];
+            const XRowNb* nb = p.hnb + size_t(vr) * p.n_hi;
             u32 e = 0;
             for (; e + NB <= hc; e += NB) xbonds_hi<SPL, NB>(nb + e, x, r, v, acc);
             for (; e < hc; ++e) xbonds_hi<SPL, 1>(nb + e, x, r, v, acc);
         }
         b = hi_end;
-        const u64* mnb = p.mnb + size_t(hi) * p.n_mix;
+        const u64* mnb = p.mnb + size_t(vr) * p.n_mix;
         for (; b + NB <= mix_end; b += NB)
             xbonds_mix<SPL, NB>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
         for (; b < mix_end; ++b)
@@ -1219,29 +1220,46 @@ void Hamiltonian::setup_exchange() {
     upload(d_dmixtab, mixtab.data(), mixtab.size(), stream);
     QK_HIP(hipStreamSynchronize(stream));
 
-    // per-row bond records of the single-vector kernel
+    // Per-row bond records of the single-vector kernel, indexed by the row
+    // itself while the table for every possible high half stays small, else
+    // by the row's number among the rows that hold states (vrow): a sparse
+    // sector with a wide high half (few states, up to 2^26 high halves)
+    // keeps records only for its own rows.
     {
         const std::vector<u64>& jh = B.jhi();
         const size_t nrow = jh.size(), nh = hi_b.size(), nm = mix_b.size();
-        std::vector<XRowNb> hnb(std::max<size_t>(nrow * nh, 1), XRowNb{0, 0.0});
-        std::vector<u32> hcnt(std::max<size_t>(nrow, 1), 0u);
-        std::vector<u64> mnb(std::max<size_t>(nrow * nm, 1), ~u64(0));
+        const size_t per_row = nh * sizeof(XRowNb) + nm * sizeof(u64) + sizeof(u32);
+        const bool dense = nrow * per_row <= (size_t(256) << 20);
+        std::vector<u32> vrow;
+        size_t nv = nrow;
+        if (!dense) {
+            vrow.assign(nrow, ~0u);
+            nv = 0;
+            for (size_t h = 0; h < nrow; ++h)
+                if (jh[h] != ~u64(0)) vrow[h] = u32(nv++);
+        }
+        std::vector<XRowNb> hnb(std::max<size_t>(nv * nh, 1), XRowNb{0, 0.0});
+        std::vector<u32> hcnt(std::max<size_t>(nv, 1), 0u);
+        std::vector<u64> mnb(std::max<size_t>(nv * nm, 1), ~u64(0));
         for (size_t h = 0; h < nrow; ++h) {
             if (jh[h] == ~u64(0)) continue;
+            const size_t v = dense ? h : vrow[h];
             u32 c = 0;
             for (const XBond& X : hi_b) {
                 const u32 vi = u32(h >> (X.bi - L)) & 1, vj = u32(h >> (X.bj - L)) & 1;
                 if (vi == vj) continue;
                 const u64 b2 = jh[h ^ X.mhi];
                 if (b2 == ~u64(0)) fail("exchange path: high-half partner row missing");
-                hnb[h * nh + c++] = XRowNb{b2, vi ? X.a10 : X.a01};
+                hnb[v * nh + c++] = XRowNb{b2, vi ? X.a10 : X.a01};
             }
-            hcnt[h] = c;
-            for (size_t m = 0; m < nm; ++m) mnb[h * nm + m] = jh[h ^ mix_b[m].mhi];
+            hcnt[v] = c;
+            for (size_t m = 0; m < nm; ++m) mnb[v * nm + m] = jh[h ^ mix_b[m].mhi];
         }
         upload(d_hnb, hnb.data(), hnb.size(), stream);
         upload(d_hcnt, hcnt.data(), hcnt.size(), stream);
         upload(d_mnb, mnb.data(), mnb.size(), stream);
+        if (!dense) upload(d_vrow, vrow.data(), vrow.size(), stream);
+        xparams.vrow = dense ? nullptr : d_vrow.get<u32>();
     }
 
     XParams& x = xparams;

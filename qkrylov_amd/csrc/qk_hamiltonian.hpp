@@ -86,6 +86,7 @@ struct XParams {
     const XRowNb* hnb;      // row hi: hnb[hi * n_hi + e], e < hcnt[hi]
     const u32* hcnt;        // active high-half bonds per row
     const u64* mnb;         // mnb[hi * n_mix + m] = jhi[hi ^ mhi] of mixed bond m
+    const u32* vrow;        // null: records indexed by hi; else by vrow[hi]
 };
 
 // Tile of the multi-sector exchange kernel: 64 ranks of one row of sector
@@ -212,7 +213,7 @@ struct Hamiltonian {
     DevBuf d_jlo, d_jhi, d_hicls, d_clsoff, d_clscnt, d_lolist, d_tiles, d_groups, d_tables;
     DevBuf d_dgroups, d_dtables;   // per-term diagonal program
     DevBuf d_xbonds, d_dlo, d_dhi, d_dmix, d_dmixtab, d_xtiles, d_xtiles_s;   // exchange path
-    DevBuf d_hnb, d_hcnt, d_mnb;   // per-row bond records (single-vector kernel)
+    DevBuf d_hnb, d_hcnt, d_mnb, d_vrow;   // per-row bond records (single-vector kernel)
     XParams xparams_s{};        // single-vector kernel, spl states per lane
     int xgrid_s = 0;
     int spl = 0;                   // states per lane of the single-vector kernel
