@@ -170,7 +170,7 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
         for (int v = 0; v < B; ++v) acc[v] = cplx{0.0, 0.0};
         cplx dsum{0.0, 0.0};
         if constexpr (ROWS) {
-            const cplx d0 = q.dlo[off + r], d1 = q.dhi[hi];
+            const cplx d0 = q.dlo[off + r], d1 = q.dhi[q.vrow ? q.vrow[hi] : hi];
             dsum.re = d0.re + d1.re;
             dsum.im = d0.im + d1.im;
         }
@@ -223,8 +223,9 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
         }
         if constexpr (ROWS) {
             // high-half groups of the row program: the row's entries (uniform)
-            const u32 e1 = q.rptr[hi + 1];
-            for (u32 e = q.rptr[hi]; e < e1; ++e) {
+            const u32 vr = q.vrow ? q.vrow[hi] : hi;
+            const u32 e1 = q.rptr[vr + 1];
+            for (u32 e = q.rptr[vr]; e < e1; ++e) {
                 const RowEntry R = q.rent[e];
                 const u64 b = R.base + r;
 #pragma unroll
@@ -1001,7 +1002,19 @@ void Hamiltonian::setup_rows() {
 
     // diagonal sums
     std::vector<cplx> dlo(std::max<size_t>(lo_list.size(), 1), cplx{0.0, 0.0});
-    std::vector<cplx> dhi(std::max<size_t>(nrow, 1), cplx{0.0, 0.0});
+    // per-row values indexed by the high half while that stays small, else
+    // by the row's number among the rows that hold states (vrow)
+    const bool dense = nrow * (sizeof(cplx) + sizeof(u32)) <= (size_t(64) << 20);
+    std::vector<u32> vrow;
+    size_t nv = nrow;
+    if (!dense) {
+        vrow.assign(nrow, ~0u);
+        nv = 0;
+        for (size_t h = 0; h < nrow; ++h)
+            if (jh[h] != ~u64(0)) vrow[h] = u32(nv++);
+    }
+    auto slot = [&](size_t h) { return dense ? h : size_t(vrow[h]); };
+    std::vector<cplx> dhi(std::max<size_t>(nv, 1), cplx{0.0, 0.0});
     for (size_t i = 0; i < lo_list.size(); ++i)
         for (const Group* g : dlo_g) {
             const cplx a = amp_of(*g, lo_list[i]);
@@ -1009,9 +1022,10 @@ void Hamiltonian::setup_rows() {
         }
     for (size_t h = 0; h < nrow; ++h) {
         if (jh[h] == ~u64(0)) continue;
+        cplx& dv = dhi[slot(h)];
         for (const Group* g : dhi_g) {
             const cplx a = amp_of(*g, u64(h) << L);
-            dhi[h] = cplx{dhi[h].re + a.re, dhi[h].im + a.im};
+            dv = cplx{dv.re + a.re, dv.im + a.im};
         }
     }
 
@@ -1066,11 +1080,12 @@ void Hamiltonian::setup_rows() {
     }
 
     // high-half groups: per row, the partner row offsets and amplitudes
-    std::vector<u32> rptr(nrow + 1, 0u);
+    std::vector<u32> rptr(nv + 1, 0u);
     std::vector<RowEntry> rent;
     for (size_t h = 0; h < nrow; ++h) {
-        rptr[h] = u32(rent.size());
+        if (dense) rptr[h] = u32(rent.size());
         if (jh[h] == ~u64(0)) continue;
+        if (!dense) rptr[vrow[h]] = u32(rent.size());
         for (const Group* g : hi_g) {
             const size_t h2 = h ^ size_t(g->flip >> L);
             if (jh[h2] == ~u64(0) || hcls[h2] != hcls[h]) continue;
@@ -1080,7 +1095,7 @@ void Hamiltonian::setup_rows() {
         }
         if (rent.size() > kMaxEntries / 4) return;
     }
-    rptr[nrow] = u32(rent.size());
+    rptr[nv] = u32(rent.size());
 
     // the remaining groups keep the group loop
     std::vector<Group> restg;
@@ -1102,6 +1117,7 @@ void Hamiltonian::setup_rows() {
     if (amps.empty()) amps.push_back(cplx{0.0, 0.0});
     upload(d_amps, amps.data(), amps.size(), stream);
     upload(d_rptr, rptr.data(), rptr.size(), stream);
+    if (!dense) upload(d_rvrow, vrow.data(), vrow.size(), stream);
     if (rent.empty()) rent.push_back(RowEntry{0, cplx{0.0, 0.0}});
     upload(d_rent, rent.data(), rent.size(), stream);
     QK_HIP(hipStreamSynchronize(stream));
@@ -1123,6 +1139,7 @@ void Hamiltonian::setup_rows() {
     q.ell_w = d_ellw.get<u32>();
     q.amps = d_amps.get<cplx>();
     q.rptr = d_rptr.get<u32>();
+    q.vrow = dense ? nullptr : d_rvrow.get<u32>();
     q.rent = d_rent.get<RowEntry>();
     q.namp = u32(amps.size());
     q.ndiag = u32(dlo_g.size() + dhi_g.size());

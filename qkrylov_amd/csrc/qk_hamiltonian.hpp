@@ -147,13 +147,14 @@ static_assert(sizeof(RowEntry) == 24, "RowEntry layout");
 
 struct RowParams {
     const cplx* dlo;        // lo_list order: diagonal groups inside the low half
-    const cplx* dhi;        // per row: diagonal groups inside the high half
+    const cplx* dhi;        // per row (slot as rptr): diagonal groups inside the high half
     const u32* ell;         // class c, rank r, entry k: ell[ell_off[c] + k * cnt_c + r]
     const u32* ell_off;     //   entry = target rank | amplitude id << 16, ~0u: none
     const u32* ell_w;       // entries per rank of class c
     const cplx* amps;       // amplitude ids -> values (copied to LDS)
-    const u32* rptr;        // row hi: rent[rptr[hi] .. rptr[hi + 1])
+    const u32* rptr;        // row hi: rent[rptr[v] .. rptr[v + 1]), v = hi or vrow[hi]
     const RowEntry* rent;
+    const u32* vrow;        // null: dhi and rptr indexed by hi; else by vrow[hi]
     u32 namp;
     u32 ndiag;              // diagonal groups folded into dlo and dhi
 };
@@ -242,7 +243,7 @@ struct Hamiltonian {
     ApplyParams rest{};
     RowParams rparams{};
     int rlds_bytes = 0;
-    DevBuf d_rgroups, d_rtables, d_rdlo, d_rdhi, d_ell, d_elloff, d_ellw, d_amps, d_rptr, d_rent;
+    DevBuf d_rgroups, d_rtables, d_rdlo, d_rdhi, d_ell, d_elloff, d_ellw, d_amps, d_rptr, d_rent, d_rvrow;
     bool state32 = true;
     int lds_bytes = 0;
     int dlds_bytes = 0;
