@@ -133,3 +133,32 @@ def test_wide_lanczos_matches_dense(qk, orc, lat, name):
         a_ref, b_ref, n_ref = oh.cf(phi[j], 20)
         assert np.abs(np.asarray(cf[j].alphas[:8]) - a_ref[:8]).max() < 1e-10
         assert abs(cf[j].norm_phi0 - n_ref) <= 1e-13 * n_ref
+
+
+def test_wide_operator_between_sectors(qk, orc, lat):
+    """S+(q) from two down spins to one on the N=36 chain (64-bit states,
+    the operator kernel's cross-basis ranking), against the oracle's gather
+    matrix on the union of the two sectors (the oracle's Hamiltonian is
+    square, so both sectors share one listed basis)."""
+    n = 36
+    q = 2 * np.pi * 5 / n
+    sp = qk.OpSum()
+    for j in range(n):
+        sp.add_term(np.exp(1j * q * j) / np.sqrt(n), "Sp", j)
+    s_in, s_out = _spin_states(n, 2), _spin_states(n, 1)
+    b_in = qk.SpinHalfBasis(n, sector=make_sector({"use_sz": True, "sz2": 32}))
+    b_out = qk.SpinHalfBasis(n, sector=make_sector({"use_sz": True, "sz2": 34}))
+    op = qk.Operator(b_in, b_out, qk.SpinHalfSite(), sp)
+    union = sorted(set(s_in) | set(s_out))
+    ob = orc.Basis.from_states("spin", n, union)
+    G = orc.Ham(ob, "spin", sp).to_scipy().tocsr()
+    O = G.conj().T.tocsr()
+    pos = {s: i for i, s in enumerate(union)}
+    rows = np.array([pos[s] for s in s_out])
+    cols = np.array([pos[s] for s in s_in])
+    ref = O[rows][:, cols].toarray()
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal((2, len(s_in))) + 1j * rng.standard_normal((2, len(s_in)))
+    y = op.apply(x)
+    yr = x @ ref.T
+    assert np.abs(y - yr).max() <= 1e-12 * np.abs(yr).max()
