@@ -526,6 +526,104 @@ BlockLanczosResult block_lanczos_lowest(const Hamiltonian& H, int b, int nev, in
 // ---------------------------------------------------------------------------
 // Davidson (reference src/solvers/davidson.cpp:85-227)
 // ---------------------------------------------------------------------------
+namespace {
+
+// Cholesky factor of a k x k Hermitian Gram matrix G (row-major) with
+// dependent columns dropped: R upper triangular over the kept columns,
+// kept[j] = column j kept.  A column whose remaining squared norm is 1e-20
+// or less is dropped; false when a kept pivot holds less than rel_min of its
+// column's squared norm (the factorisation would be inaccurate).
+bool gram_cholesky(const std::vector<cplx>& G, int k, std::vector<cplx>& R, std::vector<char>& kept,
+                   double rel_min) {
+    R.assign(size_t(k) * k, cplx{0.0, 0.0});
+    kept.assign(k, 0);
+    for (int j = 0; j < k; ++j) {
+        double d2 = G[size_t(j) * k + j].re;
+        for (int i = 0; i < j; ++i) {
+            if (!kept[i]) continue;
+            cplx sum = G[size_t(i) * k + j];
+            for (int q = 0; q < i; ++q) {
+                if (!kept[q]) continue;
+                const cplx a = R[size_t(q) * k + i], b = R[size_t(q) * k + j];
+                sum.re -= a.re * b.re + a.im * b.im;   // conj(a) b
+                sum.im -= a.re * b.im - a.im * b.re;
+            }
+            const double rii = R[size_t(i) * k + i].re;
+            R[size_t(i) * k + j] = cplx{sum.re / rii, sum.im / rii};
+            d2 -= (sum.re * sum.re + sum.im * sum.im) / (rii * rii);
+        }
+        if (d2 <= 1e-20) continue;
+        if (d2 < rel_min * G[size_t(j) * k + j].re) return false;
+        R[size_t(j) * k + j] = cplx{std::sqrt(d2), 0.0};
+        kept[j] = 1;
+    }
+    return true;
+}
+
+// Coefficients of Q = T R^-1 for the kept columns: c[o * k + i] for output o
+// (the o-th kept column) and input i; returns the number of outputs.
+int inverse_coefficients(const std::vector<cplx>& R, const std::vector<char>& kept, int k,
+                         std::vector<cplx>& c) {
+    std::vector<int> ids;
+    for (int j = 0; j < k; ++j)
+        if (kept[j]) ids.push_back(j);
+    const int nk = int(ids.size());
+    c.assign(size_t(std::max(nk, 1)) * k, cplx{0.0, 0.0});
+    // column o of R^-1 over the kept indices, by back substitution
+    for (int o = 0; o < nk; ++o) {
+        const int jo = ids[o];
+        std::vector<cplx> x(k, cplx{0.0, 0.0});
+        const double rjj = R[size_t(jo) * k + jo].re;
+        x[jo] = cplx{1.0 / rjj, 0.0};
+        for (int p = o - 1; p >= 0; --p) {
+            const int i = ids[p];
+            cplx sum{0.0, 0.0};
+            for (int q = p + 1; q <= o; ++q) {
+                const int l = ids[q];
+                const cplx a = R[size_t(i) * k + l], b = x[l];
+                sum.re += a.re * b.re - a.im * b.im;
+                sum.im += a.re * b.im + a.im * b.re;
+            }
+            const double rii = R[size_t(i) * k + i].re;
+            x[i] = cplx{-sum.re / rii, -sum.im / rii};
+        }
+        for (int i = 0; i < k; ++i) c[size_t(o) * k + i] = x[i];
+    }
+    return nk;
+}
+
+}  // namespace
+
+// Two Cholesky-QR passes over the k vectors of T (pointer table pT): the
+// first writes Q1 to `tmp` (table pTmp), the second writes the orthonormal
+// result to `out` (consecutive vectors of n).  Returns the number of vectors
+// written, or -1 (nothing written to `out`) when a pivot is too small for
+// the Gram route.
+static int cholqr2_into(const cplx* const* pT, const cplx* const* pTmp, int k, cplx* tmp, cplx* out,
+                        u64 n, DevBuf& h, DevBuf& coef, DevBuf& red, hipStream_t st) {
+    std::vector<cplx> G, R, c;
+    std::vector<char> kept;
+    const cplx* const* src = pT;
+    int cur = k;
+    for (int pass = 0; pass < 2; ++pass) {
+        h.resize(sizeof(cplx) * size_t(cur) * cur);
+        vec::block_dot(src, cur, src, cur, n, h.get<cplx>(), red, st);
+        G.resize(size_t(cur) * cur);
+        QK_HIP(hipMemcpyAsync(G.data(), h.get(), sizeof(cplx) * G.size(), hipMemcpyDeviceToHost, st));
+        QK_HIP(hipStreamSynchronize(st));
+        if (!gram_cholesky(G, cur, R, kept, 1e-6)) return -1;
+        const int nk = inverse_coefficients(R, kept, cur, c);
+        if (nk == 0) return 0;
+        coef.resize(sizeof(cplx) * c.size());
+        QK_HIP(hipMemcpyAsync(coef.get(), c.data(), sizeof(cplx) * c.size(), hipMemcpyHostToDevice, st));
+        vec::lincomb_many(src, cur, coef.get<cplx>(), nk, pass == 0 ? tmp : out, n, n, st);
+        QK_HIP(hipStreamSynchronize(st));   // c is reused by the next pass
+        src = pTmp;
+        cur = nk;
+    }
+    return cur;
+}
+
 DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace, double tol,
                                bool want_vectors, Preconditioner* pre, int max_iter) {
     DavidsonResult res;
@@ -550,11 +648,13 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
     auto vp = [&](int i) { return slabV.get<cplx>() + u64(i) * d; };
     auto hvp = [&](int i) { return slabHV.get<cplx>() + u64(i) * d; };
     DevBuf pV(sizeof(void*) * cap), pHV(sizeof(void*) * cap), pI(sizeof(void*) * 2 * cap),
-        pT(sizeof(void*) * nb);
+        pT(sizeof(void*) * nb), pN(sizeof(void*) * nb);
     {
-        std::vector<const void*> a(cap), b(cap), c(2 * size_t(cap)), t(nb);
+        std::vector<const void*> a(cap), b(cap), c(2 * size_t(cap)), t(nb), nn(nb);
         for (int j = 0; j < nb; ++j) t[j] = Tb.get<cplx>() + u64(j) * d;
+        for (int j = 0; j < nb; ++j) nn[j] = Nb.get<cplx>() + u64(j) * d;
         QK_HIP(hipMemcpyAsync(pT.get(), t.data(), sizeof(void*) * nb, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(pN.get(), nn.data(), sizeof(void*) * nb, hipMemcpyHostToDevice, st));
         for (int i = 0; i < cap; ++i) {
             a[i] = vp(i);
             b[i] = hvp(i);
@@ -795,27 +895,41 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
                 cgs2(pV, m0, T + u64(j) * d, d, h, red, st);
                 ++clk.extra;
             }
-        // Among the corrections, in order and without host round trips: each
-        // is orthogonalised against the earlier ones and normalised, or zeroed
-        // when its norm is 1e-10 or less (a zero vector adds nothing to the
-        // later projections, as a dropped one would not).
-        double* inv = tn2 + 2 * nt;
-        for (int j = 0; j < nt; ++j) {
-            cplx* t = T + u64(j) * d;
-            if (j > 0) cgs2(pT.get<const cplx*>(), j, t, d, h, red, st);
-            vec::norm2(t, d, tn2 + j, red, st);
-            vec::inv_norm_or_zero(tn2 + j, inv + j, 1, 1e-10, st);
-            vec::scale_to(t, t, d, inv + j, st);
-        }
-        if (nt > 0) {
-            QK_HIP(hipMemcpyAsync(tn.data(), tn2, sizeof(double) * nt, hipMemcpyDeviceToHost, st));
-            QK_HIP(hipStreamSynchronize(st));
-        }
-        for (int j = 0; j < nt; ++j)
-            if (std::sqrt(tn[j]) > 1e-10) {
-                QK_HIP(hipMemcpyAsync(vp(m), T + u64(j) * d, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
-                ++m;
+        // Among the corrections: Cholesky QR twice (Gram matrix, host
+        // factorisation, Q = T R^-1), the second pass writing straight into
+        // the basis slots.  A correction whose remaining norm after the
+        // earlier ones is 1e-10 or less is dropped, as in sequential
+        // Gram-Schmidt.  When a pivot keeps less than 1e-6 of its column's
+        // squared norm the Gram route would lose accuracy, and the
+        // corrections go through sequential CGS2 instead.
+        const int mq = block && nt > 1 ? cholqr2_into(pT.get<const cplx*>(), pN.get<const cplx*>(), nt,
+                                                      Nb.get<cplx>(), vp(m), d, h, coef, red, st)
+                                       : -1;
+        if (mq >= 0) {
+            m += mq;
+        } else {
+            // in order and without host round trips: each is orthogonalised
+            // against the earlier ones and normalised, or zeroed when its norm
+            // is 1e-10 or less (a zero vector adds nothing to the later
+            // projections, as a dropped one would not)
+            double* inv = tn2 + 2 * nt;
+            for (int j = 0; j < nt; ++j) {
+                cplx* t = T + u64(j) * d;
+                if (j > 0) cgs2(pT.get<const cplx*>(), j, t, d, h, red, st);
+                vec::norm2(t, d, tn2 + j, red, st);
+                vec::inv_norm_or_zero(tn2 + j, inv + j, 1, 1e-10, st);
+                vec::scale_to(t, t, d, inv + j, st);
             }
+            if (nt > 0) {
+                QK_HIP(hipMemcpyAsync(tn.data(), tn2, sizeof(double) * nt, hipMemcpyDeviceToHost, st));
+                QK_HIP(hipStreamSynchronize(st));
+            }
+            for (int j = 0; j < nt; ++j)
+                if (std::sqrt(tn[j]) > 1e-10) {
+                    QK_HIP(hipMemcpyAsync(vp(m), T + u64(j) * d, sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
+                    ++m;
+                }
+        }
         clk.mark(st, "restart + orthogonalise");
         if (m > m0) H.apply(vp(m0), hvp(m0), m - m0, d, st);
     }
