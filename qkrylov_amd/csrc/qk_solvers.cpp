@@ -694,7 +694,7 @@ DavidsonResult davidson_lowest(const Hamiltonian& H, int n_eig, int max_subspace
     std::vector<cplx> Hs;   // projected matrix, row-major m*m
     int valid_rows = 0;
     max_iter = std::max(max_iter, 1);
-    static const bool use_block_dot = [] {
+    const bool use_block_dot = [] {
         const char* e = std::getenv("QKRYLOV_DAVIDSON_BLOCK_DOT");
         return !(e && std::string(e) == "0");
     }();

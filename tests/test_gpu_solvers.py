@@ -223,3 +223,27 @@ def test_lanczos_at_scale_j1j2(qk, lat):
           f"res={resb:.1e} steps={b.iterations}", flush=True)
     assert res < 1e-9 and resb < 1e-5
     assert abs(r.energy - b.eigenvalues[0]) < 1e-10
+
+
+@pytest.mark.parametrize("block", ["1", "0"])
+def test_davidson_small_space_dependent_corrections(qk, orc, monkeypatch, block):
+    """Davidson asked for 6 of the 20 eigenpairs of the N=6 chain (Sz=0) with
+    a 14-vector subspace: corrections become dependent and are dropped, and
+    the block path (Cholesky QR among the corrections) and the per-vector
+    path (CGS2) both return eigenvalues of H to within the residual
+    tolerance."""
+    monkeypatch.setenv("QKRYLOV_DAVIDSON_BLOCK_DOT", block)
+    import qkrylov_amd as q
+    from qkrylov_amd import lattices as lt
+    n = 6
+    H = q.MatrixFreeHamiltonian(q.SpinHalfBasis(n, sz=0), q.SpinHalfSite(), lt.heisenberg_chain(n))
+    ob = orc.Basis("spin", n, {"use_sz": True, "sz2": 0})
+    spec = np.linalg.eigvalsh(orc.Ham(ob, "spin", lt.heisenberg_chain(n)).to_scipy().toarray())
+    r = q.davidson_lowest_ex(H, 6, 14, 1e-8, 200)
+    ev = np.asarray(r.eigenvalues)
+    assert r.converged and ev.size == 6
+    dist = np.max(np.min(np.abs(ev[:, None] - spec[None, :]), axis=1))
+    print(f"[davidson small] block={block} iterations={r.iterations} ev={ev} max distance={dist:.1e}",
+          flush=True)
+    assert dist <= 1e-8
+    assert np.all(np.diff(ev) >= -1e-12)
