@@ -182,6 +182,28 @@ def test_ftlm_sweep_all_sectors(qk, orc, lat):
             assert np.all(np.abs(Cv[s] - Cvr) <= 1e-8 * np.abs(Cvr) + 1e-12 * betas**2 * Er**2), (R, s)
 
 
+@pytest.mark.parametrize("batched", [True, False])
+def test_ftlm_sweep_tj_sectors(qk, orc, lat, batched):
+    """FTLM over several (N_up, N_dn) sectors of t-J on 3x2: the row-program
+    apply inside the block recurrence (captured in a hipGraph when batched)
+    and the separate alpha pass, against the oracle per sector."""
+    os_ = lat.tj_model(lat.square_bonds(3, 2), 1.0, 0.4)
+    secs = [(1, 1), (2, 1), (1, 2), (2, 2), (3, 2)]
+    hs, ohs = [], []
+    for nu, nd in secs:
+        sec = {"use_nup": True, "nup": nu, "use_ndn": True, "ndn": nd}
+        hs.append(qk.MatrixFreeHamiltonian(qk.TJBasis(6, sector=make_sector(sec)), qk.TJSite(), os_))
+        ohs.append(orc.Ham(orc.Basis("tj", 6, sec), "tj", os_))
+    assert all(h.info()["row_path"] == 1 for h in hs)
+    betas = np.array([0.3, 3.0])
+    Z, E, Cv = qk.ftlm_sweep(hs, betas, 5, 12, batched=batched)
+    for s in range(len(secs)):
+        Zr, Er, Cvr = ohs[s].ftlm(betas, 5, 12)
+        assert np.all(np.abs(Z[s] - Zr) <= 1e-8 * np.abs(Zr)), s
+        assert np.all(np.abs(E[s] - Er) <= 1e-8 * np.maximum(np.abs(Er), 1e-12)), s
+        assert np.all(np.abs(Cv[s] - Cvr) <= 1e-8 * np.abs(Cvr) + 1e-12 * betas**2 * Er**2), s
+
+
 def test_block_lanczos_matches_lanczos_and_eigsh(qk, orc, lat):
     import scipy.sparse.linalg as sla
     H = qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(16, sz=0), qk.SpinHalfSite(), lat.heisenberg_chain(16))
