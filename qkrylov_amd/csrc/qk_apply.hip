@@ -265,6 +265,28 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
 #pragma unroll
             for (int v = 0; v < B; ++v) cfma(acc[v], amp, x[v * ld + b]);
         }
+        if constexpr (ROWS) {
+            if (q.ment) {
+                // groups across the split: the row's entries (uniform), each
+                // finished per lane on the low half
+                const u32 vr = q.vrow ? q.vrow[hi] : hi;
+                const u32 e1 = q.mptr[vr + 1];
+                for (u32 e = q.mptr[vr]; e < e1; ++e) {
+                    const RowMix M = q.ment[e];
+                    const DevGroup D = q.mdesc[M.desc_sign >> 1];
+                    cplx amp = s_tab[M.tab + config_of(lo, D)];
+                    if (amp.re == 0.0 && amp.im == 0.0) continue;
+                    const u32 tlo = lo ^ u32(D.flip);
+                    const u16 rl = s_jlo[tlo];
+                    if (rl == 0xFFFF) continue;
+                    if (__popc(tlo & u32(p.m1lo)) * p.ncls2 + __popc(tlo & u32(p.m2lo)) != M.cls) continue;
+                    if ((__popc(lo & u32(D.jw)) ^ M.desc_sign) & 1) { amp.re = -amp.re; amp.im = -amp.im; }
+                    const u64 b = M.base + rl;
+#pragma unroll
+                    for (int v = 0; v < B; ++v) cfma(acc[v], amp, x[v * ld + b]);
+                }
+            }
+        }
 #pragma unroll
         for (int v = 0; v < B; ++v) {
             cplx o = acc[v];
@@ -1097,11 +1119,79 @@ void Hamiltonian::setup_rows() {
     }
     rptr[nv] = u32(rent.size());
 
-    // the remaining groups keep the group loop
+    // the remaining groups keep the group loop, except those across the
+    // split, which become per-row entries over low-half sub-tables
     std::vector<Group> restg;
-    for (auto* v : {&rest_diag, &rest_lo, &rest_hi, &rest_mix}) restg.insert(restg.end(), v->begin(), v->end());
+    for (auto* v : {&rest_diag, &rest_lo, &rest_hi}) restg.insert(restg.end(), v->begin(), v->end());
     std::vector<cplx> rtab;
     std::vector<DevGroup> rdev = pack_groups(restg, rtab);
+
+    std::vector<u32> mptr;
+    std::vector<RowMix> ment;
+    std::vector<DevGroup> mdesc;
+    if (!rest_mix.empty()) {
+        struct Split {
+            std::vector<int> lo_k, hi_k;   // config bits on each half
+            std::vector<i64> sub;          // per high-half configuration: rtab offset or -1
+            u64 fhi, jhi_bits;
+        };
+        std::vector<Split> sp(rest_mix.size());
+        for (size_t gi = 0; gi < rest_mix.size(); ++gi) {
+            const Group& g = rest_mix[gi];
+            Split& S = sp[gi];
+            DevGroup D{};
+            for (size_t k = 0; k < g.pos.size(); ++k) {
+                if (g.pos[k] < L) {
+                    D.posw |= u64(g.pos[k] & 63) << (6 * S.lo_k.size());
+                    S.lo_k.push_back(int(k));
+                } else {
+                    S.hi_k.push_back(int(k));
+                }
+            }
+            D.flip = g.flip & lomask;
+            D.jw = g.jw & lomask;
+            D.nsup = u32(S.lo_k.size());
+            D.kind = u32(kMix);
+            mdesc.push_back(D);
+            S.fhi = g.flip >> L;
+            S.jhi_bits = g.jw >> L;
+            const u32 nh = 1u << S.hi_k.size(), nl = 1u << S.lo_k.size();
+            S.sub.assign(nh, -1);
+            for (u32 ch = 0; ch < nh; ++ch) {
+                std::vector<cplx> t(nl);
+                bool any = false;
+                for (u32 cl = 0; cl < nl; ++cl) {
+                    u32 cf = 0;
+                    for (size_t a = 0; a < S.lo_k.size(); ++a) cf |= ((cl >> a) & 1u) << S.lo_k[a];
+                    for (size_t a = 0; a < S.hi_k.size(); ++a) cf |= ((ch >> a) & 1u) << S.hi_k[a];
+                    t[cl] = g.table[cf];
+                    any = any || nonzero(t[cl]);
+                }
+                if (!any) continue;
+                S.sub[ch] = i64(rtab.size());
+                rtab.insert(rtab.end(), t.begin(), t.end());
+            }
+        }
+        mptr.assign(nv + 1, 0u);
+        for (size_t h = 0; h < nrow; ++h) {
+            if (dense) mptr[h] = u32(ment.size());
+            if (jh[h] == ~u64(0)) continue;
+            if (!dense) mptr[vrow[h]] = u32(ment.size());
+            for (size_t gi = 0; gi < rest_mix.size(); ++gi) {
+                const Group& g = rest_mix[gi];
+                const Split& S = sp[gi];
+                u32 ch = 0;
+                for (size_t a = 0; a < S.hi_k.size(); ++a) ch |= u32((h >> (g.pos[S.hi_k[a]] - L)) & 1u) << a;
+                if (S.sub[ch] < 0) continue;
+                const size_t h2 = h ^ size_t(S.fhi);
+                if (jh[h2] == ~u64(0)) continue;
+                const u32 par = u32(__builtin_popcountll(u64(h) & S.jhi_bits) & 1);
+                ment.push_back(RowMix{jh[h2], u32(S.sub[ch]), (u32(gi) << 1) | par, hcls[h2], 0u});
+            }
+            if (ment.size() > kMaxEntries / 4) return;
+        }
+        mptr[nv] = u32(ment.size());
+    }
     const int lds = int(rtab.size() * sizeof(cplx) + ((B.jlo().size() * sizeof(u16) + 15) / 16) * 16 +
                         amps.size() * sizeof(cplx));
     if (lds > 160 * 1024) return;
@@ -1118,6 +1208,11 @@ void Hamiltonian::setup_rows() {
     upload(d_amps, amps.data(), amps.size(), stream);
     upload(d_rptr, rptr.data(), rptr.size(), stream);
     if (!dense) upload(d_rvrow, vrow.data(), vrow.size(), stream);
+    if (!ment.empty()) {
+        upload(d_mptr, mptr.data(), mptr.size(), stream);
+        upload(d_ment, ment.data(), ment.size(), stream);
+        upload(d_mdesc, mdesc.data(), mdesc.size(), stream);
+    }
     if (rent.empty()) rent.push_back(RowEntry{0, cplx{0.0, 0.0}});
     upload(d_rent, rent.data(), rent.size(), stream);
     QK_HIP(hipStreamSynchronize(stream));
@@ -1129,7 +1224,7 @@ void Hamiltonian::setup_rows() {
     rest.n_diag = u32(rest_diag.size());
     rest.n_lo = u32(rest_lo.size());
     rest.n_hi = u32(rest_hi.size());
-    rest.n_mix = u32(rest_mix.size());
+    rest.n_mix = 0;
     rest.ntab = u32(rtab.size());
     RowParams& q = rparams;
     q.dlo = d_rdlo.get<cplx>();
@@ -1140,6 +1235,9 @@ void Hamiltonian::setup_rows() {
     q.amps = d_amps.get<cplx>();
     q.rptr = d_rptr.get<u32>();
     q.vrow = dense ? nullptr : d_rvrow.get<u32>();
+    q.mptr = ment.empty() ? nullptr : d_mptr.get<u32>();
+    q.ment = ment.empty() ? nullptr : d_ment.get<RowMix>();
+    q.mdesc = ment.empty() ? nullptr : d_mdesc.get<DevGroup>();
     q.rent = d_rent.get<RowEntry>();
     q.namp = u32(amps.size());
     q.ndiag = u32(dlo_g.size() + dhi_g.size());

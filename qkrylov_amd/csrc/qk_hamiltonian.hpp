@@ -145,6 +145,19 @@ struct RowEntry {
 };
 static_assert(sizeof(RowEntry) == 24, "RowEntry layout");
 
+// A group across the split, seen from one row: the row fixes the group's
+// high-half bits, so only its low-half sub-table (offset `tab` in the group
+// table), its target row and the parity of its high-half Jordan-Wigner bits
+// remain.  desc_sign = low-half descriptor index << 1 | that parity.
+struct RowMix {
+    u64 base;     // jhi of the target row
+    u32 tab;      // sub-table over the low-half support configuration
+    u32 desc_sign;
+    int cls;      // class of the target row
+    u32 pad;
+};
+static_assert(sizeof(RowMix) == 24, "RowMix layout");
+
 struct RowParams {
     const cplx* dlo;        // lo_list order: diagonal groups inside the low half
     const cplx* dhi;        // per row (slot as rptr): diagonal groups inside the high half
@@ -155,6 +168,9 @@ struct RowParams {
     const u32* rptr;        // row hi: rent[rptr[v] .. rptr[v + 1]), v = hi or vrow[hi]
     const RowEntry* rent;
     const u32* vrow;        // null: dhi and rptr indexed by hi; else by vrow[hi]
+    const u32* mptr;        // groups across the split, row slot v: ment[mptr[v] .. mptr[v + 1])
+    const RowMix* ment;     //   (null: none)
+    const DevGroup* mdesc;  // low-half descriptors: flip, jw, support of the low half
     u32 namp;
     u32 ndiag;              // diagonal groups folded into dlo and dhi
 };
@@ -244,6 +260,7 @@ struct Hamiltonian {
     RowParams rparams{};
     int rlds_bytes = 0;
     DevBuf d_rgroups, d_rtables, d_rdlo, d_rdhi, d_ell, d_elloff, d_ellw, d_amps, d_rptr, d_rent, d_rvrow;
+    DevBuf d_mptr, d_ment, d_mdesc;
     bool state32 = true;
     int lds_bytes = 0;
     int dlds_bytes = 0;
