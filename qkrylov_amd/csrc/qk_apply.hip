@@ -21,6 +21,14 @@
 // reference's convention y[a] = sum conj(amp(a->b)) x[b]
 // (src/hamiltonian/matrix_free_hamiltonian.cpp:75-84, 156-167).
 //
+// Three kernels share this decomposition: the exchange kernels (every
+// off-diagonal group a real two-bit exchange: bonds instead of groups,
+// diagonals from per-half tables), the row-program kernel (k_apply with
+// ROWS: groups inside one half precomputed per class or per row, groups
+// across the split as per-row entries over low-half sub-tables) and the
+// group loop above for everything else.  All three add a state's terms in
+// group order, so block and single-vector launches give identical bits.
+//
 // Tiles are dealt to the 8 XCDs in 8 contiguous chunks (workgroup i runs on
 // XCD i % 8), so each XCD's L2 holds the rows its waves are reading.
 #include "qk_hamiltonian.hpp"
