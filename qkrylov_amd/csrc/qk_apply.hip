@@ -1082,7 +1082,6 @@ void Hamiltonian::setup_rows() {
     // low-half groups: per class, padded column-major entries per rank
     const size_t kMaxEntries = size_t(1) << 26;
     std::vector<u32> ell, ell_off(std::max<size_t>(ncls, 1), 0u), ell_w(std::max<size_t>(ncls, 1), 0u);
-    std::vector<u32> row_e;
     for (size_t c = 0; c < ncls && !lo_g.empty(); ++c) {
         const u32 cnt = cls_cnt[c], off = cls_off[c];
         std::vector<std::vector<u32>> per(cnt);
