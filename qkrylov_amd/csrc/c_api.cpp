@@ -866,8 +866,7 @@ int qkrylov_ftlm_sweep_f64(const qkrylov_hamiltonian_h* hs, int nsec, const doub
 int qkrylov_reference_start_f64(uint64_t n, unsigned seed, double* out) {
     if (!out && n) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {
-        const std::vector<cplx> v = uniform_start(n, seed);
-        std::memcpy(out, v.data(), sizeof(cplx) * n);
+        uniform_start(n, seed, reinterpret_cast<cplx*>(out));
     });
 }
 int qkrylov_reference_normals_f64(uint64_t n, unsigned seed, double* out) {

@@ -171,8 +171,9 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
     if (start) {
         QK_HIP(hipMemcpyAsync(vptr(0), start, sizeof(cplx) * d, hipMemcpyHostToDevice, st));
     } else {
-        const std::vector<cplx> s0 = uniform_start(d, 1234);
-        QK_HIP(hipMemcpyAsync(vptr(0), s0.data(), sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+        std::unique_ptr<cplx[]> s0(new cplx[std::max<u64>(d, 1)]);
+        uniform_start(d, 1234, s0.get());
+        QK_HIP(hipMemcpyAsync(vptr(0), s0.get(), sizeof(cplx) * d, hipMemcpyHostToDevice, st));
         QK_HIP(hipStreamSynchronize(st));
     }
     {   // normalize (vector_ops.hpp:91-97: unchanged when the norm is 0)
@@ -407,8 +408,10 @@ BlockLanczosResult block_lanczos_lowest(const Hamiltonian& H, int b, int nev, in
 
     // start block
     {
-        const std::vector<cplx> s0 = uniform_start(d * u64(b), 1234);
-        QK_HIP(hipMemcpyAsync(W.get(), s0.data(), sizeof(cplx) * d * b, hipMemcpyHostToDevice, st));
+        std::unique_ptr<cplx[]> s0(new cplx[std::max<u64>(d * u64(b), 1)]);
+        uniform_start(d * u64(b), 1234, s0.get());
+        QK_HIP(hipMemcpyAsync(W.get(), s0.get(), sizeof(cplx) * d * b, hipMemcpyHostToDevice, st));
+        QK_HIP(hipStreamSynchronize(st));
         std::vector<cplx> R0;
         if (!cholqr(R0)) fail("block Lanczos: start block is rank deficient");
         push_block();

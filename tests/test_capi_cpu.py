@@ -151,6 +151,12 @@ def test_reference_random_streams(qk, orc):
     assert np.array_equal(a.view(np.float64), orc.uniform_start(n, 1234).view(np.float64))
     assert lib.qkrylov_reference_normals_f64(n, 42, a.ctypes.data) == 0
     assert np.array_equal(a.view(np.float64), orc.normal_stream(n, 42).view(np.float64))
+    # the start vector is converted in chunks of 2^20 elements on several
+    # threads: a length across a chunk boundary, another seed
+    m = (1 << 20) + 77
+    c = np.empty(m, np.complex128)
+    assert lib.qkrylov_reference_start_f64(m, 5, c.ctypes.data) == 0
+    assert np.array_equal(c.view(np.float64), orc.uniform_start(m, 5).view(np.float64))
     # imaginary part first: element 0's imaginary part is the first draw
     import random
     assert abs(a[0].imag) < 10 and abs(a[0].real) < 10
