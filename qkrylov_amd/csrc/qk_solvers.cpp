@@ -183,20 +183,21 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
     put_ptr(ptrs, 0, vptr(0), st);
 
     std::vector<double> alphas, betas;
+    // alpha stays on the device for the subtraction and beta goes back to
+    // it from persistent host storage, so an iteration waits for the device
+    // once (alpha and the norms together) rather than once per scalar
+    std::vector<double> bhost(size_t(std::max(nmax, 1)) + 1, 0.0);
+    std::vector<const void*> hptr(size_t(std::max(nmax, 1)) + 1, nullptr);
+    DevBuf bdev(sizeof(double) * bhost.size());
+    QK_HIP(hipMemcpyAsync(bdev.get(), bhost.data(), sizeof(double) * bhost.size(), hipMemcpyHostToDevice, st));
     double e_old = 1e100;
     for (int iter = 0; iter < nmax; ++iter) {
         res.iterations = iter + 1;
         cplx* v = vptr(iter);
         H.apply(v, w.get<cplx>(), 1, d, st);
-        vec::dot(v, w.get<cplx>(), d, sc.get<cplx>(), red, st);
-        const double alpha = fetch<cplx>(sc.get<cplx>(), st).re;
-        alphas.push_back(alpha);
-        const double bprev = iter > 0 ? betas.back() : 0.0;
-        sc.resize(64 * sizeof(double));
-        double ab[2] = {alpha, bprev};
-        QK_HIP(hipMemcpyAsync(sc.get<double>() + 8, ab, sizeof(ab), hipMemcpyHostToDevice, st));
-        vec::lanczos_sub(w.get<cplx>(), v, iter > 0 ? vptr(iter - 1) : nullptr, d,
-                         sc.get<double>() + 8, sc.get<double>() + 9, st);
+        vec::dot(v, w.get<cplx>(), d, sc.get<cplx>(), red, st);   // alpha = Re sc[0]
+        vec::lanczos_sub(w.get<cplx>(), v, iter > 0 ? vptr(iter - 1) : nullptr, d, sc.get<double>(),
+                         bdev.get<double>() + (iter > 0 ? iter - 1 : bhost.size() - 1), st);
         // Full reorthogonalisation against every stored vector (the
         // reference's MGS, :141-152) as classical Gram-Schmidt with the
         // Daniel-Gragg-Kaufman-Stewart test: a second pass only when the
@@ -208,9 +209,12 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
         vec::norm2(w.get<cplx>(), d, nn, red, st);
         vec::mdot(ptrs.get<const cplx*>(), k, w.get<cplx>(), d, h.get<cplx>(), red, st);
         vec::msub_norm(ptrs.get<const cplx*>(), k, h.get<cplx>(), w.get<cplx>(), d, nn + 1, red, st);
-        double n01[2];
-        QK_HIP(hipMemcpyAsync(n01, nn, sizeof(n01), hipMemcpyDeviceToHost, st));
+        double head[18];   // Re alpha at [0], the norms at [16], [17]
+        QK_HIP(hipMemcpyAsync(head, sc.get(), sizeof(head), hipMemcpyDeviceToHost, st));
         QK_HIP(hipStreamSynchronize(st));
+        const double alpha = head[0];
+        alphas.push_back(alpha);
+        const double n01[2] = {head[16], head[17]};
         double w2 = n01[1];
         if (!(n01[1] > 0.5 * n01[0])) {
             vec::mdot(ptrs.get<const cplx*>(), k, w.get<cplx>(), d, h.get<cplx>(), red, st);
@@ -224,9 +228,13 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
         }
         if (iter + 1 == nmax) break;
         betas.push_back(beta);
+        bhost[iter] = beta;
+        QK_HIP(hipMemcpyAsync(bdev.get<double>() + iter, &bhost[iter], sizeof(double), hipMemcpyHostToDevice, st));
         cplx* vn = vptr(iter + 1);
         vec::scale_copy(w.get<cplx>(), vn, d, cplx{1.0 / beta, 0.0}, st);
-        put_ptr(ptrs, iter + 1, vn, st);
+        hptr[iter + 1] = vn;
+        QK_HIP(hipMemcpyAsync(ptrs.get<char>() + sizeof(void*) * (iter + 1), &hptr[iter + 1], sizeof(void*),
+                              hipMemcpyHostToDevice, st));
         if (iter > 0) {
             const double e = tridiag_ground(alphas, betas, int(alphas.size()), nullptr);
             if (std::abs(e - e_old) < tol) {
