@@ -126,9 +126,21 @@ def test_n28_ground_state_and_cf_block(n28, qk):
         w2 = c.norm_phi0 ** 2
         integral = np.trapz(S, omegas) + w2 * 2 * 0.05 / (math.pi * 100)
         assert abs(integral - w2) <= 2e-3 * w2, (k, integral, w2)
-    # the block equals one recurrence per start vector (first steps to rounding)
+    # the block equals one recurrence per start vector.  The two sum alpha in
+    # different orders (the single-vector run takes it from the apply's
+    # partial sums), and at q = pi the unreorthogonalised recurrence amplifies
+    # that rounding from the sixth step on (measured: 3e-14 at step 5, 9e-12
+    # at step 6, 5e-10 at step 7), so the first six coefficients are compared
+    # at 1e-10 and the spectra of the 12-step fractions at 1e-6.
     one = qk.continued_fraction_coeffs(H, phis[14], 12)
-    assert np.abs(one.alphas[:10] - cf[14].alphas[:10]).max() <= 1e-10
+    assert np.abs(one.alphas[:6] - cf[14].alphas[:6]).max() <= 1e-10 * np.abs(one.alphas[:6]).max()
+    assert np.abs(one.betas[:5] - cf[14].betas[:5]).max() <= 1e-10 * np.abs(one.betas[:5]).max()
+    ws = np.linspace(0.0, 5.0, 201)
+    s1 = np.array([qk.evaluate_spectral_function(one.alphas, one.betas, one.norm_phi0, w, gs.energy, 0.05)
+                   for w in ws])
+    s2 = np.array([qk.evaluate_spectral_function(cf[14].alphas[:12], cf[14].betas[:12], cf[14].norm_phi0,
+                                                 w, gs.energy, 0.05) for w in ws])
+    assert np.abs(s1 - s2).max() <= 1e-6 * np.abs(s1).max()
 
 
 # ---------------------------------------------------------------- t-J 4x4
