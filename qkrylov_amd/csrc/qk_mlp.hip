@@ -12,6 +12,7 @@
 #include "qk_mlp.hpp"
 
 #include "qk_hamiltonian.hpp"
+#include "qk_threads.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -803,11 +804,7 @@ void MlpStates::build(const Hamiltonian& H, hipStream_t st) {
     const bool two = B.kind() == SiteKind::Hubbard || B.kind() == SiteKind::TJ;
     const int ns = B.nsites();
     std::vector<float> sh(5 * n);
-    const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (unsigned w = 0; w < nt; ++w)
-        th.emplace_back([&, w]() {
-            const u64 lo = n * w / nt, hi = n * (w + 1) / nt;
+    run_ranges(n, host_threads(n), [&](u64 lo, u64 hi) {
             for (u64 i = lo; i < hi; ++i) {
                 const u64 s = B.state(i);
                 float occ[4] = {0, 0, 0, 0};
@@ -827,7 +824,6 @@ void MlpStates::build(const Hamiltonian& H, hipStream_t st) {
                 sh[5 * i + 4] = off.empty() ? 0.0f : float(cnt) / float(off.size());
             }
         });
-    for (auto& t : th) t.join();
     QK_HIP(hipMemcpyAsync(diag.get(), dr.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
     QK_HIP(hipMemcpyAsync(stat.get(), sh.data(), sizeof(float) * 5 * n, hipMemcpyHostToDevice, st));
 

@@ -7,12 +7,12 @@
 // -ffp-contract=off so the polar-method arithmetic inside
 // std::normal_distribution rounds as on the reference's baseline build.
 #include "qk_rng.hpp"
+#include "qk_threads.hpp"
 
 #include <random>
 #include <cstring>
 #include <algorithm>
 #include <cmath>
-#include <thread>
 
 namespace qk {
 
@@ -22,7 +22,17 @@ BlockMT19937::BlockMT19937(result_type seed) {
     for (int i = 1; i < kN; ++i) s_[i] = 1812433253u * (s_[i - 1] ^ (s_[i - 1] >> 30)) + std::uint32_t(i);
 }
 
-void BlockMT19937::refill() {
+// The twist and the conversion loops are built twice, for AVX2 and for the
+// baseline x86-64 ISA, and the loader picks the clone the host supports
+// (GNU ifunc); both clones round identically (IEEE adds, multiplies and
+// divides, no contraction), the AVX2 one only vectorises wider.
+#define QK_VECTOR_CLONES __attribute__((target_clones("avx2", "default")))
+
+namespace {
+
+// one block: renew the 624 state words and temper them into out
+QK_VECTOR_CLONES void twist_block(std::uint32_t* s, std::uint32_t* out) {
+    constexpr int kN = 624;
     // twist: word i takes the top bit of word i and the low 31 bits of word
     // i+1, and word i+397 (mod 624) -- the old one for i < 227, the one
     // already renewed in this block after that; then temper
@@ -31,17 +41,23 @@ void BlockMT19937::refill() {
         return c ^ (y >> 1) ^ ((0u - (y & 1u)) & 0x9908b0dfu);
     };
     int i = 0;
-    for (; i < kN - 397; ++i) s_[i] = tw(s_[i], s_[i + 1], s_[i + 397]);
-    for (; i < kN - 1; ++i) s_[i] = tw(s_[i], s_[i + 1], s_[i + 397 - kN]);
-    s_[kN - 1] = tw(s_[kN - 1], s_[0], s_[396]);
+    for (; i < kN - 397; ++i) s[i] = tw(s[i], s[i + 1], s[i + 397]);
+    for (; i < kN - 1; ++i) s[i] = tw(s[i], s[i + 1], s[i + 397 - kN]);
+    s[kN - 1] = tw(s[kN - 1], s[0], s[396]);
     for (int k = 0; k < kN; ++k) {
-        std::uint32_t y = s_[k];
+        std::uint32_t y = s[k];
         y ^= y >> 11;
         y ^= (y << 7) & 0x9d2c5680u;
         y ^= (y << 15) & 0xefc60000u;
         y ^= y >> 18;
-        out_[k] = y;
+        out[k] = y;
     }
+}
+
+}  // namespace
+
+void BlockMT19937::refill() {
+    twist_block(s_, out_);
     pos_ = 0;
 }
 
@@ -71,6 +87,15 @@ inline double uniform_pm1(std::uint32_t w0, std::uint32_t w1) {
     return u * 2.0 + -1.0;
 }
 
+// elements [a, b) of a chunk from its words, element i from words 4i..4i+3
+QK_VECTOR_CLONES void convert_uniform(const std::uint32_t* w, cplx* v, u64 a, u64 b) {
+    for (u64 i = a; i < b; ++i) {
+        const double im = uniform_pm1(w[4 * i], w[4 * i + 1]);
+        const double re = uniform_pm1(w[4 * i + 2], w[4 * i + 3]);
+        v[i] = cplx{re, im};
+    }
+}
+
 }  // namespace
 
 void uniform_start(u64 n, unsigned seed, cplx* v) {
@@ -81,25 +106,12 @@ void uniform_start(u64 n, unsigned seed, cplx* v) {
     BlockMT19937 rng(seed);
     const u64 chunk = u64(1) << 20;
     std::vector<std::uint32_t> w(4 * std::min<u64>(n, chunk));
-    const unsigned hw = std::thread::hardware_concurrency();
-    const unsigned nth = std::max(1u, std::min(hw ? hw : 1u, 16u));
+    const unsigned nth = host_threads(chunk);
     for (u64 c0 = 0; c0 < n; c0 += chunk) {
         const u64 len = std::min(chunk, n - c0);
         rng.fill(w.data(), 4 * len);
-        auto conv = [&](u64 a, u64 b) {
-            for (u64 i = a; i < b; ++i) {
-                const double im = uniform_pm1(w[4 * i], w[4 * i + 1]);
-                const double re = uniform_pm1(w[4 * i + 2], w[4 * i + 3]);
-                v[c0 + i] = cplx{re, im};
-            }
-        };
-        if (nth == 1 || len < 65536) {
-            conv(0, len);
-        } else {
-            std::vector<std::thread> th;
-            for (unsigned t = 0; t < nth; ++t) th.emplace_back(conv, len * t / nth, len * (t + 1) / nth);
-            for (auto& t : th) t.join();
-        }
+        run_ranges(len, len < 65536 ? 1u : nth,
+                   [&](u64 a, u64 b) { convert_uniform(w.data(), v + c0, a, b); });
     }
 }
 

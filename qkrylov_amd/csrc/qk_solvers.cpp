@@ -6,6 +6,7 @@
 #include "qk_hostla.hpp"
 #include "qk_rng.hpp"
 #include "qk_vec.hpp"
+#include "qk_threads.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -112,21 +113,15 @@ double dev_norm(const cplx* x, u64 n, DevBuf& red, DevBuf& sc, hipStream_t st) {
     return std::sqrt(fetch<double>(sc.get<double>(), st));
 }
 
+// f(i) for i < n on the host threads, item i on thread slot i % nt
 template <typename F>
 void parallel_for(int n, F&& f) {
-    unsigned hw = std::thread::hardware_concurrency();
-    int nt = int(std::min<unsigned>(hw ? hw : 1, 16));
-    nt = std::max(1, std::min(nt, n));
-    if (nt <= 1) {
-        for (int i = 0; i < n; ++i) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            for (int i = t; i < n; i += nt) f(i);
-        });
-    for (auto& x : th) x.join();
+    if (n <= 0) return;
+    const unsigned nt = host_threads(u64(n));
+    run_ranges(nt, nt, [&](u64 a, u64 b) {
+        for (u64 t = a; t < b; ++t)
+            for (int i = int(t); i < n; i += int(nt)) f(i);
+    });
 }
 
 // Two classical Gram-Schmidt passes of w against k vectors (pointer array
