@@ -181,6 +181,8 @@ typedef struct {
     double energy;
     int iterations;
     int converged;
+    /* steps whose reorthogonalisation needed the second Gram-Schmidt pass */
+    int second_passes;
 } qkrylov_lanczos_result_f64_t;
 
 typedef struct {
@@ -271,6 +273,42 @@ int qkrylov_lanczos_ground_state_f64(qkrylov_hamiltonian_h h, int maxiter, doubl
                                      const double* start_complex,
                                      qkrylov_lanczos_result_f64_t* result,
                                      double* eigvec_complex);
+
+/* As qkrylov_lanczos_ground_state_f64, with checkpoints of the run (no
+ * reference counterpart: the reference keeps its basis in memory only,
+ * src/solvers/lanczos.cpp:123-126, 169-171).  save_path (may be NULL):
+ * file that receives the run's state (every stored basis vector, alpha,
+ * beta, the last energy; raw binary, checksummed, replaced atomically)
+ * every save_every steps (0: never) and when the run stops at stop_after;
+ * stop_after > 0 ends the run once that many steps are done (result
+ * converged = 0, iterations = stop_after), as a preempted job would;
+ * resume_path (may be NULL): continue the run saved there instead of
+ * starting one (start is then ignored).  A resumed run performs the same
+ * arithmetic as an uninterrupted one: the same energy and iteration count.
+ * The checkpoint records the Hamiltonian's fingerprint and dimension; a
+ * mismatch is an error.  Host memory of (steps + 1) * dim complex doubles is
+ * used while loading. */
+int qkrylov_lanczos_checkpointed_f64(qkrylov_hamiltonian_h h, int maxiter, double tol,
+                                     const double* start_complex, const char* save_path,
+                                     int save_every, int stop_after, const char* resume_path,
+                                     qkrylov_lanczos_result_f64_t* result,
+                                     double* eigvec_complex);
+
+/* Eigenpair files (what the reference's examples/hdf5_storage.py:5-13 keeps
+ * in HDF5, which this build does not depend on): n eigenvalues and, when
+ * vectors_complex is not NULL, n vectors of dim complex doubles one after
+ * another; fingerprint: qkrylov_hamiltonian_fingerprint() of the
+ * Hamiltonian they belong to, or 0.  Raw binary, checksummed. */
+int qkrylov_eigenpairs_save_f64(const char* path, uint64_t dim, int n, const double* values,
+                                const double* vectors_complex, uint64_t fingerprint);
+/* Header of an eigenpair file (any output pointer may be NULL). */
+int qkrylov_eigenpairs_info(const char* path, uint64_t* dim, int* n, int* has_vectors,
+                            uint64_t* fingerprint);
+/* values_out: n doubles; vectors_complex_out: n * 2 * dim doubles or NULL. */
+int qkrylov_eigenpairs_load_f64(const char* path, double* values_out, double* vectors_complex_out);
+/* Hash of what defines h (site kind, size, sector, dimension, every OpSum
+ * term in order); 0 on error. */
+uint64_t qkrylov_hamiltonian_fingerprint(qkrylov_hamiltonian_h h);
 
 /* Block Lanczos with full reorthogonalisation (no reference counterpart; it
  * extends lanczos.cpp:101-202 to blocks of `block` <= 16 vectors): lowest nev

@@ -8,6 +8,7 @@ the HIP kernels; vectors cross the boundary as complex128 numpy arrays.
 from __future__ import annotations
 
 import ctypes as ct
+import os
 import enum
 from typing import List, Optional, Sequence, Tuple, Union
 
@@ -21,7 +22,8 @@ __all__ = [
     "Site", "SpinHalfSite", "FermionSite", "HubbardSite", "TJSite",
     "OpSum", "Sz", "Sp", "Sm", "Sx", "Sy", "CdagUp", "CUp", "CdagDn", "CDn",
     "Nup", "Ndn", "Nupdn", "N", "C", "Cdag", "Id",
-    "MatrixFreeHamiltonian", "LanczosResult", "lanczos_ground_state",
+    "MatrixFreeHamiltonian", "LanczosResult", "lanczos_ground_state", "lanczos_checkpointed",
+    "save_eigenpairs", "load_eigenpairs", "save_result",
     "DavidsonResult", "davidson_lowest", "DynamicsResult", "continued_fraction_coeffs",
     "continued_fraction_block", "evaluate_spectral_function", "FTLMResult", "ftlm",
     "ftlm_sweep", "device_info", "gpu_count", "find_gpu", "Bdag", "B",
@@ -618,9 +620,10 @@ class MatrixFreeHamiltonian:
 
 # --------------------------------------------------------------------------- solvers
 class LanczosResult:
-    def __init__(self, energy, eigenvector, iterations, converged):
+    def __init__(self, energy, eigenvector, iterations, converged, second_passes=0):
         self.energy, self.eigenvector = energy, eigenvector
         self.iterations, self.converged = iterations, converged
+        self.second_passes = second_passes   # steps that needed a second Gram-Schmidt pass
 
     def __iter__(self):
         yield self.energy
@@ -640,7 +643,78 @@ def lanczos_ground_state(H: MatrixFreeHamiltonian, maxiter: int = 200, tol: floa
     check(load().qkrylov_lanczos_ground_state_f64(
         H._h, int(maxiter), float(tol), None if st is None else _ptr(st), ct.byref(res),
         None if vec is None else _ptr(vec)), "lanczos")
-    return LanczosResult(res.energy, vec, res.iterations, bool(res.converged))
+    return LanczosResult(res.energy, vec, res.iterations, bool(res.converged), res.second_passes)
+
+
+def lanczos_checkpointed(H: MatrixFreeHamiltonian, maxiter: int = 200, tol: float = 1e-12,
+                         save_path: Optional[str] = None, save_every: int = 0, stop_after: int = 0,
+                         resume_path: Optional[str] = None, start: Optional[np.ndarray] = None,
+                         return_state: bool = True):
+    """lanczos_ground_state with checkpoints: the run's state goes to
+    save_path every save_every steps and when it stops at stop_after (> 0:
+    end after that many steps, converged False); resume_path continues a
+    saved run.  A resumed run gives the same energy and iteration count as
+    an uninterrupted one.  (No reference counterpart: the reference keeps
+    its Lanczos basis in memory, src/solvers/lanczos.cpp:123-126.)"""
+    d = H.dimension
+    res = _lib.LanczosResultF64()
+    vec = np.empty(d, dtype=np.complex128) if return_state and not stop_after else None
+    st = None if start is None else np.ascontiguousarray(start, dtype=np.complex128)
+    enc = (lambda p: None if p is None else os.fsencode(os.fspath(p)))
+    check(load().qkrylov_lanczos_checkpointed_f64(
+        H._h, int(maxiter), float(tol), None if st is None else _ptr(st), enc(save_path),
+        int(save_every), int(stop_after), enc(resume_path), ct.byref(res),
+        None if vec is None else _ptr(vec)), "lanczos")
+    return LanczosResult(res.energy, vec, res.iterations, bool(res.converged), res.second_passes)
+
+
+def save_eigenpairs(path, values, vectors=None, H: Optional[MatrixFreeHamiltonian] = None):
+    """Eigenvalues and (optionally) eigenvectors to a raw-binary file; with H
+    the file records H's fingerprint."""
+    vals = np.ascontiguousarray(np.atleast_1d(values), dtype=np.float64)
+    n = vals.size
+    vecs = None
+    dim = H.dimension if H is not None else 0
+    if vectors is not None:
+        vecs = np.ascontiguousarray(np.asarray(vectors, dtype=np.complex128).reshape(n, -1))
+        dim = vecs.shape[1]
+        if H is not None and dim != H.dimension:
+            raise ValueError("eigenvectors do not match the Hamiltonian's dimension")
+    fp = int(load().qkrylov_hamiltonian_fingerprint(H._h)) if H is not None else 0
+    check(load().qkrylov_eigenpairs_save_f64(os.fsencode(os.fspath(path)), dim, n, _ptr(vals),
+                                             None if vecs is None else _ptr(vecs), fp), "eigenpairs")
+
+
+def load_eigenpairs(path, vectors: bool = True, H: Optional[MatrixFreeHamiltonian] = None):
+    """(values, vectors or None) from save_eigenpairs' file; with H, checks
+    that the file was written for it."""
+    p = os.fsencode(os.fspath(path))
+    dim, n, hv, fp = ct.c_uint64(), ct.c_int(), ct.c_int(), ct.c_uint64()
+    check(load().qkrylov_eigenpairs_info(p, ct.byref(dim), ct.byref(n), ct.byref(hv), ct.byref(fp)),
+          "eigenpairs")
+    if H is not None and (fp.value != int(load().qkrylov_hamiltonian_fingerprint(H._h))
+                          or dim.value != H.dimension):
+        raise RuntimeError("eigenpairs: the file was written for a different Hamiltonian")
+    vals = np.empty(max(n.value, 1), dtype=np.float64)
+    vecs = np.empty((n.value, dim.value), dtype=np.complex128) if vectors and hv.value else None
+    check(load().qkrylov_eigenpairs_load_f64(p, _ptr(vals), None if vecs is None else _ptr(vecs)),
+          "eigenpairs")
+    return vals[:n.value], vecs
+
+
+def save_result(path, result, H: Optional[MatrixFreeHamiltonian] = None):
+    """A LanczosResult (energy, eigenvector) or DavidsonResult (eigenvalues,
+    eigenvectors) to a file, as the reference's examples/hdf5_storage.py
+    stores them in HDF5."""
+    if isinstance(result, LanczosResult):
+        vec = None if result.eigenvector is None else [result.eigenvector]
+        save_eigenpairs(path, [result.energy], vec, H)
+    elif isinstance(result, DavidsonResult):
+        vecs = result.eigenvectors
+        save_eigenpairs(path, result.eigenvalues,
+                        None if vecs is None or len(vecs) == 0 else np.asarray(vecs), H)
+    else:
+        raise TypeError("save_result takes a LanczosResult or a DavidsonResult")
 
 
 def block_lanczos(H: MatrixFreeHamiltonian, block: int = 4, nev: int = 1, maxiter: int = 100,

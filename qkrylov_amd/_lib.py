@@ -41,7 +41,8 @@ class FTLMResultC(C.Structure):
 
 
 class LanczosResultF64(C.Structure):
-    _fields_ = [("energy", c_double), ("iterations", c_int), ("converged", c_int)]
+    _fields_ = [("energy", c_double), ("iterations", c_int), ("converged", c_int),
+                ("second_passes", c_int)]
 
 
 class FTLMResultF64(C.Structure):
@@ -124,6 +125,13 @@ SIGNATURES = {
     "qkrylov_hamiltonian_autotune": (c_int, [c_void_p, c_int, C.c_char_p, c_size_t]),
     "qkrylov_lanczos_ground_state_f64": (c_int, [c_void_p, c_int, c_double, c_void_p,
                                                  P(LanczosResultF64), c_void_p]),
+    "qkrylov_lanczos_checkpointed_f64": (c_int, [c_void_p, c_int, c_double, c_void_p, C.c_char_p, c_int,
+                                                 c_int, C.c_char_p, P(LanczosResultF64), c_void_p]),
+    "qkrylov_eigenpairs_save_f64": (c_int, [C.c_char_p, C.c_uint64, c_int, c_void_p, c_void_p,
+                                            C.c_uint64]),
+    "qkrylov_eigenpairs_info": (c_int, [C.c_char_p, P(C.c_uint64), P(c_int), P(c_int), P(C.c_uint64)]),
+    "qkrylov_eigenpairs_load_f64": (c_int, [C.c_char_p, c_void_p, c_void_p]),
+    "qkrylov_hamiltonian_fingerprint": (C.c_uint64, [c_void_p]),
     "qkrylov_block_lanczos_f64": (c_int, [c_void_p, c_int, c_int, c_int, c_double, c_void_p, c_void_p,
                                           P(c_int), P(c_int)]),
     "qkrylov_davidson_lowest_f64": (c_int, [c_void_p, c_int, c_int, c_double, c_void_p, c_void_p,

@@ -11,6 +11,7 @@
 #include "qk_learned.hpp"
 #include "qk_rng.hpp"
 #include "qk_solvers.hpp"
+#include "qk_store.hpp"
 #include "qk_vec.hpp"
 
 #include <algorithm>
@@ -519,9 +520,72 @@ int qkrylov_lanczos_ground_state_f64(qkrylov_hamiltonian_h h, int maxiter, doubl
         result->energy = r.energy;
         result->iterations = r.iterations;
         result->converged = r.converged ? 1 : 0;
+        result->second_passes = r.second_passes;
         if (eig && !r.eigenvector.empty())
             std::memcpy(eig, r.eigenvector.data(), sizeof(cplx) * r.eigenvector.size());
     });
+}
+
+int qkrylov_lanczos_checkpointed_f64(qkrylov_hamiltonian_h h, int maxiter, double tol,
+                                     const double* start, const char* save_path, int save_every,
+                                     int stop_after, const char* resume_path,
+                                     qkrylov_lanczos_result_f64_t* result, double* eig) {
+    if (!hok(h) || !result || save_every < 0 || stop_after < 0 || (stop_after > 0 && !save_path))
+        return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        LanczosIO io;
+        if (save_path) io.save_path = save_path;
+        if (resume_path) io.resume_path = resume_path;
+        io.save_every = save_every;
+        io.stop_after = stop_after;
+        const LanczosResult r = lanczos_run(*h->ptr, maxiter, tol, reinterpret_cast<const cplx*>(start),
+                                            eig != nullptr, io);
+        result->energy = r.energy;
+        result->iterations = r.iterations;
+        result->converged = r.converged ? 1 : 0;
+        result->second_passes = r.second_passes;
+        if (eig && !r.eigenvector.empty())
+            std::memcpy(eig, r.eigenvector.data(), sizeof(cplx) * r.eigenvector.size());
+    });
+}
+
+int qkrylov_eigenpairs_save_f64(const char* path, uint64_t dim, int n, const double* values,
+                                const double* vectors_complex, uint64_t fingerprint) {
+    if (!path || n < 0 || (n > 0 && !values)) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        save_eigenpairs(path, dim, fingerprint, n, values, reinterpret_cast<const cplx*>(vectors_complex));
+    });
+}
+
+int qkrylov_eigenpairs_info(const char* path, uint64_t* dim, int* n, int* has_vectors,
+                            uint64_t* fingerprint) {
+    if (!path) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const EigenFile e = load_eigenpairs(path, false);
+        if (dim) *dim = e.dim;
+        if (n) *n = int(e.values.size());
+        if (has_vectors) *has_vectors = e.has_vectors ? 1 : 0;
+        if (fingerprint) *fingerprint = e.fingerprint;
+    });
+}
+
+int qkrylov_eigenpairs_load_f64(const char* path, double* values_out, double* vectors_complex_out) {
+    if (!path || !values_out) return QKRYLOV_ERROR_INVALID_ARG;
+    return guarded([&] {
+        const EigenFile e = load_eigenpairs(path, true);
+        if (!e.values.empty()) std::memcpy(values_out, e.values.data(), sizeof(double) * e.values.size());
+        if (vectors_complex_out) {
+            if (!e.has_vectors) fail("eigenpairs: the file holds no vectors");
+            std::memcpy(vectors_complex_out, e.vectors.data(), sizeof(cplx) * e.vectors.size());
+        }
+    });
+}
+
+uint64_t qkrylov_hamiltonian_fingerprint(qkrylov_hamiltonian_h h) {
+    if (!hok(h)) return 0;
+    u64 fp = 0;
+    (void)guarded([&] { fp = hamiltonian_fingerprint(*h->ptr); });
+    return fp;
 }
 
 int qkrylov_davidson_lowest_complex(qkrylov_hamiltonian_h h, int n_eig, int max_subspace, float tol,

@@ -7,6 +7,7 @@
 #include "qk_rng.hpp"
 #include "qk_vec.hpp"
 #include "qk_threads.hpp"
+#include "qk_store.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -101,12 +102,6 @@ T fetch(const void* dev, hipStream_t st) {
     return v;
 }
 
-void put_ptr(DevBuf& arr, int slot, const void* p, hipStream_t st) {
-    // pointer arrays are small; a synchronous copy keeps the host value alive
-    QK_HIP(hipMemcpyAsync(arr.get<char>() + sizeof(void*) * slot, &p, sizeof(void*),
-                          hipMemcpyHostToDevice, st));
-    QK_HIP(hipStreamSynchronize(st));
-}
 
 double dev_norm(const cplx* x, u64 n, DevBuf& red, DevBuf& sc, hipStream_t st) {
     vec::norm2(x, n, sc.get<double>(), red, st);
@@ -144,113 +139,259 @@ void cgs2(const DevBuf& ptrs, int k, cplx* w, u64 n, DevBuf& h, DevBuf& red, hip
 // ---------------------------------------------------------------------------
 // Lanczos ground state (reference src/solvers/lanczos.cpp:101-202)
 // ---------------------------------------------------------------------------
-LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol,
-                                   const cplx* start, bool want_vector) {
+namespace {
+
+// A captured one-step graph: replayed once per step (all indices on the device).
+struct StepGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    ~StepGraph() {
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+    }
+    template <typename F>
+    void capture(hipStream_t st, F&& body) {
+        QK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &g);
+            throw;
+        }
+        QK_HIP(hipStreamEndCapture(st, &g));
+        QK_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    }
+};
+
+// pinned host memory (step records read back while later steps run)
+struct PinnedBuf {
+    void* p = nullptr;
+    explicit PinnedBuf(size_t bytes) { QK_HIP(hipHostMalloc(&p, std::max<size_t>(bytes, 16))); }
+    ~PinnedBuf() { (void)hipHostFree(p); }
+    template <typename T>
+    T* get() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+LanczosResult lanczos_run(const Hamiltonian& H, int maxiter, double tol, const cplx* start,
+                          bool want_vector, const LanczosIO& io) {
     LanczosResult res;
     const u64 d = H.dim();
     if (d == 0) return res;
     hipStream_t st = H.stream;
     const int nmax = int(std::min<i64>(i64(maxiter), i64(d)));
+    if (nmax <= 0) return res;
+    constexpr int R = vec::LzStep::kRecFields;
 
     // The basis lives in slabs of kSlab vectors, allocated as the run needs
-    // them (one hipMalloc per kSlab steps instead of one per step).
+    // them; V (device) holds the pointer of every allocated vector.
     constexpr int kSlab = 8;
     std::vector<DevBuf> slabs;
+    std::vector<cplx*> hptr(size_t(nmax), nullptr);   // persistent: source of async copies
+    DevBuf ptrs(sizeof(void*) * size_t(nmax));
     auto vptr = [&](int i) -> cplx* {
-        while (size_t(i / kSlab) >= slabs.size()) slabs.emplace_back(sizeof(cplx) * d * kSlab);
-        return slabs[size_t(i / kSlab)].get<cplx>() + u64(i % kSlab) * d;
+        while (size_t(i / kSlab) >= slabs.size()) {
+            const int first = int(slabs.size()) * kSlab;
+            const int cnt = std::min(kSlab, nmax - first);
+            slabs.emplace_back(sizeof(cplx) * d * size_t(cnt));
+            for (int j = 0; j < cnt; ++j) hptr[size_t(first + j)] = slabs.back().get<cplx>() + u64(j) * d;
+            QK_HIP(hipMemcpyAsync(ptrs.get<cplx*>() + first, &hptr[size_t(first)], sizeof(void*) * cnt,
+                                  hipMemcpyHostToDevice, st));
+        }
+        return hptr[size_t(i)];
     };
-    DevBuf w(sizeof(cplx) * d), red, sc(64 * sizeof(double)), h;
-    DevBuf ptrs(sizeof(void*) * (size_t(std::max(nmax, 1)) + 1));
+    DevBuf w(sizeof(cplx) * d), red, sc(64 * sizeof(double));
+    vec::LzStep lz(d, nmax);
 
-    if (start) {
-        QK_HIP(hipMemcpyAsync(vptr(0), start, sizeof(cplx) * d, hipMemcpyHostToDevice, st));
-    } else {
-        std::unique_ptr<cplx[]> s0(new cplx[std::max<u64>(d, 1)]);
-        uniform_start(d, 1234, s0.get());
-        QK_HIP(hipMemcpyAsync(vptr(0), s0.get(), sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+    std::vector<double> alphas, betas;
+    double e_old = 1e100;
+    int first = 0;   // first step this call runs
+    if (!io.resume_path.empty()) {
+        LanczosState ck = load_lanczos_checkpoint(io.resume_path, H);
+        if (ck.next_step > nmax) fail("lanczos resume: the checkpoint is past maxiter");
+        first = ck.next_step;
+        alphas = std::move(ck.alphas);
+        betas = std::move(ck.betas);
+        e_old = ck.e_old;
+        for (int j = 0; j <= std::min(first, nmax - 1); ++j)
+            QK_HIP(hipMemcpyAsync(vptr(j), ck.vectors.data() + u64(j) * d, sizeof(cplx) * d,
+                                  hipMemcpyHostToDevice, st));
         QK_HIP(hipStreamSynchronize(st));
-    }
-    {   // normalize (vector_ops.hpp:91-97: unchanged when the norm is 0)
+    } else {
+        if (start) {
+            QK_HIP(hipMemcpyAsync(vptr(0), start, sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+        } else {
+            std::unique_ptr<cplx[]> s0(new cplx[d]);
+            uniform_start(d, 1234, s0.get());
+            QK_HIP(hipMemcpyAsync(vptr(0), s0.get(), sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+            QK_HIP(hipStreamSynchronize(st));
+        }
+        // normalize (vector_ops.hpp:91-97: unchanged when the norm is 0)
         const double nrm = dev_norm(vptr(0), d, red, sc, st);
         if (nrm != 0.0) vec::scale(vptr(0), d, cplx{1.0 / nrm, 0.0}, st);
     }
-    put_ptr(ptrs, 0, vptr(0), st);
-
-    std::vector<double> alphas, betas;
-    // alpha stays on the device for the subtraction and beta goes back to
-    // it from persistent host storage, so an iteration waits for the device
-    // once (alpha and the norms together) rather than once per scalar
-    std::vector<double> bhost(size_t(std::max(nmax, 1)) + 1, 0.0);
-    std::vector<const void*> hptr(size_t(std::max(nmax, 1)) + 1, nullptr);
-    DevBuf bdev(sizeof(double) * bhost.size());
-    QK_HIP(hipMemcpyAsync(bdev.get(), bhost.data(), sizeof(double) * bhost.size(), hipMemcpyHostToDevice, st));
-    double e_old = 1e100;
-    for (int iter = 0; iter < nmax; ++iter) {
-        res.iterations = iter + 1;
-        cplx* v = vptr(iter);
-        H.apply(v, w.get<cplx>(), 1, d, st);
-        vec::dot(v, w.get<cplx>(), d, sc.get<cplx>(), red, st);   // alpha = Re sc[0]
-        vec::lanczos_sub(w.get<cplx>(), v, iter > 0 ? vptr(iter - 1) : nullptr, d, sc.get<double>(),
-                         bdev.get<double>() + (iter > 0 ? iter - 1 : bhost.size() - 1), st);
-        // Full reorthogonalisation against every stored vector (the
-        // reference's MGS, :141-152) as classical Gram-Schmidt with the
-        // Daniel-Gragg-Kaufman-Stewart test: a second pass only when the
-        // first removed more than half of |w|^2 (the squared norm after each
-        // pass comes from the subtraction pass itself)
-        const int k = iter + 1;
-        h.resize(sizeof(cplx) * k);
-        double* nn = sc.get<double>() + 16;   // |w|^2 before, after pass 1, after pass 2
-        vec::norm2(w.get<cplx>(), d, nn, red, st);
-        vec::mdot(ptrs.get<const cplx*>(), k, w.get<cplx>(), d, h.get<cplx>(), red, st);
-        vec::msub_norm(ptrs.get<const cplx*>(), k, h.get<cplx>(), w.get<cplx>(), d, nn + 1, red, st);
-        double head[18];   // Re alpha at [0], the norms at [16], [17]
-        QK_HIP(hipMemcpyAsync(head, sc.get(), sizeof(head), hipMemcpyDeviceToHost, st));
-        QK_HIP(hipStreamSynchronize(st));
-        const double alpha = head[0];
-        alphas.push_back(alpha);
-        const double n01[2] = {head[16], head[17]};
-        double w2 = n01[1];
-        if (!(n01[1] > 0.5 * n01[0])) {
-            vec::mdot(ptrs.get<const cplx*>(), k, w.get<cplx>(), d, h.get<cplx>(), red, st);
-            vec::msub_norm(ptrs.get<const cplx*>(), k, h.get<cplx>(), w.get<cplx>(), d, nn + 2, red, st);
-            w2 = fetch<double>(nn + 2, st);
-        }
-        const double beta = std::sqrt(w2);
-        if (beta < 1e-15) {
-            res.converged = true;
-            break;
-        }
-        if (iter + 1 == nmax) break;
-        betas.push_back(beta);
-        bhost[iter] = beta;
-        QK_HIP(hipMemcpyAsync(bdev.get<double>() + iter, &bhost[iter], sizeof(double), hipMemcpyHostToDevice, st));
-        cplx* vn = vptr(iter + 1);
-        vec::scale_copy(w.get<cplx>(), vn, d, cplx{1.0 / beta, 0.0}, st);
-        hptr[iter + 1] = vn;
-        QK_HIP(hipMemcpyAsync(ptrs.get<char>() + sizeof(void*) * (iter + 1), &hptr[iter + 1], sizeof(void*),
+    lz.reset(u32(first), st);
+    if (first > 0 && first <= nmax) {
+        // the previous step's beta, read by the next step's three-term subtraction
+        const double bprev = betas.empty() ? 0.0 : betas.back();
+        QK_HIP(hipMemcpyAsync(lz.rec() + size_t(R) * (first - 1) + 4, &bprev, sizeof(double),
                               hipMemcpyHostToDevice, st));
-        if (iter > 0) {
-            const double e = tridiag_ground(alphas, betas, int(alphas.size()), nullptr);
-            if (std::abs(e - e_old) < tol) {
-                res.converged = true;
-                break;
-            }
-            e_old = e;
-        }
+        QK_HIP(hipStreamSynchronize(st));
     }
+
+    // Short vectors: one step is one captured graph replayed with the step
+    // index on the device (the apply reads a fixed copy of v_k); long
+    // vectors: the same kernels launched directly, the apply reading v_k.
+    const bool use_graph = d <= (u64(1) << 20) && std::getenv("QKRYLOV_LANCZOS_GRAPH") == nullptr
+                               ? true
+                               : (std::getenv("QKRYLOV_LANCZOS_GRAPH") != nullptr &&
+                                  std::getenv("QKRYLOV_LANCZOS_GRAPH")[0] == '1');
+    const bool fused = H.fused_dot(1);
+    DevBuf apart(sizeof(double) * size_t(std::max(H.dot_parts(), 1)));
+    DevBuf vcur;
+    cplx* const* V = ptrs.get<cplx*>();
+    auto step_body = [&](const cplx* x, cplx* vc, int coef_blocks) {
+        if (fused) {
+            H.apply_dot(x, w.get<cplx>(), apart.get<double>(), st);
+            lz.alpha_from_parts(apart.get<double>(), u32(H.dot_parts()), st);
+        } else {
+            H.apply(x, w.get<cplx>(), 1, d, st);
+            lz.alpha_dot(x, w.get<cplx>(), st);
+        }
+        lz.reorth(V, w.get<cplx>(), 1, coef_blocks, st);
+        lz.reorth(V, w.get<cplx>(), 2, coef_blocks, st);
+        lz.finish(V, w.get<cplx>(), vc, st);
+    };
+    StepGraph graph;
+    if (use_graph && first < nmax) {
+        vcur.resize(sizeof(cplx) * d);
+        QK_HIP(hipMemcpyAsync(vcur.get(), vptr(first), sizeof(cplx) * d, hipMemcpyDeviceToDevice, st));
+        QK_HIP(hipStreamSynchronize(st));
+        graph.capture(st, [&] { step_body(vcur.get<cplx>(), vcur.get<cplx>(), nmax + 1); });
+    }
+
+    // Steps are queued ahead of the host's stopping test (the reference's
+    // rule, lanczos.cpp:153-182, evaluated step by step on the records read
+    // back), so the device never waits for the host; a step queued past the
+    // stopping step is discarded (it changes nothing the result uses).
+    PinnedBuf hrec(sizeof(double) * size_t(R) * size_t(nmax));
+    struct Batch {
+        int k0, k1;
+        hipEvent_t ev;
+    };
+    std::vector<Batch> inflight;
+    std::vector<hipEvent_t> evpool;
+    auto get_event = [&]() {
+        if (!evpool.empty()) {
+            hipEvent_t e = evpool.back();
+            evpool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        QK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        return e;
+    };
+    auto cleanup_events = [&]() {
+        for (auto& b : inflight) evpool.push_back(b.ev);
+        inflight.clear();
+        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+        evpool.clear();
+    };
+    const bool small = d <= (u64(1) << 20);
+    const int batch = small ? 4 : 1;
+    int enq = first;
+    bool done = first >= nmax;
+    double last_change = 1e100;
+    res.iterations = first;
+    try {
+        while (!done) {
+            const int evald = inflight.empty() ? enq : inflight.front().k0;
+            int depth = small ? 2 * batch : (tol <= 0.0 || last_change > 1e3 * tol ? 2 : 1);
+            if (tol <= 0.0) depth = std::max(depth, 16);
+            while (enq < nmax && enq - evald < depth) {
+                const int k1 = std::min(nmax, enq + batch);
+                for (int k = enq; k < k1; ++k) {
+                    if (k + 1 < nmax) vptr(k + 1);   // slot written by this step
+                    if (use_graph) QK_HIP(hipGraphLaunch(graph.ge, st));
+                    else step_body(vptr(k), nullptr, k + 2);
+                }
+                QK_HIP(hipMemcpyAsync(hrec.get<double>() + size_t(R) * enq, lz.rec() + size_t(R) * enq,
+                                      sizeof(double) * size_t(R) * size_t(k1 - enq), hipMemcpyDeviceToHost,
+                                      st));
+                hipEvent_t ev = get_event();
+                QK_HIP(hipEventRecord(ev, st));
+                inflight.push_back(Batch{enq, k1, ev});
+                enq = k1;
+            }
+            Batch b = inflight.front();
+            inflight.erase(inflight.begin());
+            QK_HIP(hipEventSynchronize(b.ev));
+            evpool.push_back(b.ev);
+            for (int iter = b.k0; iter < b.k1 && !done; ++iter) {
+                const double* r = hrec.get<double>() + size_t(R) * iter;
+                res.iterations = iter + 1;
+                alphas.push_back(r[0]);
+                if (r[5] != 0.0) ++res.second_passes;
+                const double beta = r[4];
+                if (beta < 1e-15) {
+                    res.converged = true;
+                    done = true;
+                    break;
+                }
+                if (iter + 1 == nmax) {
+                    done = true;
+                    break;
+                }
+                betas.push_back(beta);
+                if (iter > 0) {
+                    const double e = tridiag_ground(alphas, betas, int(alphas.size()), nullptr);
+                    last_change = std::abs(e - e_old);
+                    if (last_change < tol) {
+                        res.converged = true;
+                        done = true;
+                        break;
+                    }
+                    e_old = e;
+                }
+                const bool stop_here = io.stop_after > 0 && iter + 1 >= io.stop_after;
+                if (!io.save_path.empty() &&
+                    (stop_here || (io.save_every > 0 && (iter + 1) % io.save_every == 0))) {
+                    // vectors v_0..v_{iter+1} are final once step iter is done
+                    QK_HIP(hipStreamSynchronize(st));
+                    LanczosState ck;
+                    ck.next_step = iter + 1;
+                    ck.alphas = alphas;
+                    ck.betas = betas;
+                    ck.e_old = e_old;
+                    save_lanczos_checkpoint(io.save_path, H, ck, hptr.data(), st);
+                }
+                if (stop_here) {
+                    done = true;
+                    res.stopped = true;
+                    break;
+                }
+            }
+        }
+        QK_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        cleanup_events();
+        throw;
+    }
+    cleanup_events();
 
     std::vector<double> z;
     const int n = int(alphas.size());
-    res.energy = tridiag_ground(alphas, betas, n, want_vector ? &z : nullptr);
-    if (want_vector) {
+    res.energy = tridiag_ground(alphas, betas, n, want_vector && !res.stopped ? &z : nullptr);
+    if (want_vector && !res.stopped) {
         DevBuf ritz(sizeof(cplx) * d), coef(sizeof(cplx) * std::max(n, 1));
         QK_HIP(hipMemsetAsync(ritz.get(), 0, sizeof(cplx) * d, st));
         if (n > 0) {
             std::vector<cplx> cz(n);
             for (int i = 0; i < n; ++i) cz[i] = cplx{z[i], 0.0};
             QK_HIP(hipMemcpyAsync(coef.get(), cz.data(), sizeof(cplx) * n, hipMemcpyHostToDevice, st));
-            vec::lincomb(ptrs.get<const cplx*>(), n, coef.get<cplx>(), ritz.get<cplx>(), d, st);
+            vec::lincomb(V, n, coef.get<cplx>(), ritz.get<cplx>(), d, st);
             QK_HIP(hipStreamSynchronize(st));
         }
         const double nrm = dev_norm(ritz.get<cplx>(), d, red, sc, st);
@@ -261,6 +402,11 @@ LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol
         QK_HIP(hipStreamSynchronize(st));
     }
     return res;
+}
+
+LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol,
+                                   const cplx* start, bool want_vector) {
+    return lanczos_run(H, maxiter, tol, start, want_vector, LanczosIO{});
 }
 
 // ---------------------------------------------------------------------------

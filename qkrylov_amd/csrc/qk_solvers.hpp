@@ -5,6 +5,7 @@
 
 #include "qk_hamiltonian.hpp"
 
+#include <string>
 #include <vector>
 
 namespace qk {
@@ -13,13 +14,31 @@ struct LanczosResult {
     double energy = 0.0;
     int iterations = 0;
     bool converged = false;
+    bool stopped = false;            // ended at LanczosIO::stop_after (not converged)
+    int second_passes = 0;           // steps that ran the second Gram-Schmidt pass
     std::vector<cplx> eigenvector;   // host, empty unless requested
+};
+
+// Checkpointing of a Lanczos run.  save_path: write the run's state (every
+// stored basis vector, alpha, beta, the last energy) every save_every steps
+// and at stop_after; stop_after > 0: end the run once that many steps are
+// done (as a preempted job would), result.stopped = true; resume_path:
+// continue a run from its checkpoint instead of starting it.  A resumed run
+// performs the same arithmetic as an uninterrupted one (same energy, same
+// iteration count).
+struct LanczosIO {
+    std::string save_path;
+    int save_every = 0;
+    int stop_after = 0;
+    std::string resume_path;
 };
 
 // start: host vector of dim() entries, or null for the reference start
 // (std::mt19937(1234), uniform(-1,1)).
 LanczosResult lanczos_ground_state(const Hamiltonian& H, int maxiter, double tol,
                                    const cplx* start, bool want_vector);
+LanczosResult lanczos_run(const Hamiltonian& H, int maxiter, double tol, const cplx* start,
+                          bool want_vector, const LanczosIO& io);
 
 struct BlockLanczosResult {
     std::vector<double> eigenvalues;               // lowest nev Ritz values

@@ -574,6 +574,240 @@ __global__ void __launch_bounds__(kT) k_block_sub(const cplx* const* __restrict_
         if (o < nw) W[o][i] = acc[o];
 }
 
+// ---------------- Lanczos step with device-resident indices ----------------
+// The step index k (*kd) and every scalar of step j (rec[kRec*j + f]) live
+// on the device, so a step launches the same kernels with the same
+// arguments whatever k is: steps can be queued ahead of the host's stopping
+// test or replayed from one captured graph.  Record fields:
+//   0 alpha = Re<v_k, H v_k>
+//   1 |w|^2 after the three-term subtraction (before reorthogonalisation)
+//   2 |w|^2 after the first projection pass
+//   3 |w|^2 after the second pass (when it runs)
+//   4 beta = |w| at the end of the step
+//   5 1 when the second pass ran
+// Pass 1 folds the three-term subtraction into the projection: with
+// h_j = <V_j, w - alpha v_k - beta_{k-1} v_{k-1}>, the update is
+// w -= sum_j c_j V_j with c_k = h_k + alpha, c_{k-1} = h_{k-1} + beta_{k-1}.
+// Pass 2 (classical Gram-Schmidt once more) runs only when pass 1 removed
+// more than half of |w|^2 (the Daniel-Gragg-Kaufman-Stewart test); every
+// workgroup reads the same two recorded values, so the test is uniform.
+constexpr int kRec = 8;
+
+__global__ void k_fill_u32(u32* __restrict__ p, u64 n, u32 v) {
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ bool lz_second(const double* rec, u32 k) {
+    return !(rec[kRec * k + 2] > 0.5 * rec[kRec * k + 1]);
+}
+
+// alpha from the apply kernel's per-workgroup partial sums of Re<x, Hx>
+__global__ void __launch_bounds__(kT) k_lz_alpha_parts(const double* __restrict__ part, u32 np,
+                                                       const u32* __restrict__ kd,
+                                                       double* __restrict__ rec) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < np; b += kT) s += part[b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) rec[kRec * kd[0]] = s;
+}
+
+// alpha from k_dot_part's complex partials (real part)
+__global__ void __launch_bounds__(kT) k_lz_alpha_dot(const cplx* __restrict__ part, u32 nb,
+                                                     const u32* __restrict__ kd,
+                                                     double* __restrict__ rec) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < nb; b += kT) s += part[b].re;
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) rec[kRec * kd[0]] = s;
+}
+
+// projection partials of pass `pass` over V_0..V_k: part[j*nb + blk]; pass 1
+// also the block's partial of |w - alpha v_k - beta v_{k-1}|^2 in npart[blk]
+template <int PER>
+__global__ void __launch_bounds__(kT) k_lz_mdot(const cplx* const* __restrict__ V,
+                                                const u32* __restrict__ kd,
+                                                const cplx* __restrict__ w, u64 n,
+                                                cplx* __restrict__ part, u32 nb,
+                                                double* __restrict__ npart,
+                                                const double* __restrict__ rec, int pass) {
+    __shared__ double sh[4];
+    const u32 k = kd[0];
+    if (pass == 2 && !lz_second(rec, k)) return;
+    const u64 start = u64(blockIdx.x) * (PER * kT);
+    cplx wl[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const u64 i = start + u64(e) * kT + threadIdx.x;
+        wl[e] = (i < n) ? w[i] : cplx{0.0, 0.0};
+    }
+    if (pass == 1) {
+        const double a = rec[kRec * k];
+        const cplx* vk = V[k];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const u64 i = start + u64(e) * kT + threadIdx.x;
+            if (i < n) {
+                const cplx x = vk[i];
+                wl[e].re -= a * x.re;
+                wl[e].im -= a * x.im;
+            }
+        }
+        if (k > 0) {
+            const double b = rec[kRec * (k - 1) + 4];
+            const cplx* vp = V[k - 1];
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const u64 i = start + u64(e) * kT + threadIdx.x;
+                if (i < n) {
+                    const cplx x = vp[i];
+                    wl[e].re -= b * x.re;
+                    wl[e].im -= b * x.im;
+                }
+            }
+        }
+        double s = 0.0;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) s += wl[e].re * wl[e].re + wl[e].im * wl[e].im;
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) npart[blockIdx.x] = s;
+    }
+    for (u32 j = 0; j <= k; ++j) {
+        const cplx* vj = V[j];
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const u64 i = start + u64(e) * kT + threadIdx.x;
+            if (i < n) {
+                const cplx x = vj[i];
+                re += x.re * wl[e].re + x.im * wl[e].im;
+                im += x.re * wl[e].im - x.im * wl[e].re;
+            }
+        }
+        re = block_sum(re, sh);
+        im = block_sum(im, sh);
+        if (threadIdx.x == 0) part[u64(j) * nb + blockIdx.x] = cplx{re, im};
+    }
+}
+
+// coefficients of pass `pass`: workgroup j <= k sums the partials of h_j
+// (pass 1 adds alpha at j = k and beta_{k-1} at j = k - 1); workgroup k + 1
+// of pass 1 records |w|^2 before the projection; the others have no work
+__global__ void __launch_bounds__(kT) k_lz_coef(const cplx* __restrict__ part, u32 nb,
+                                                const double* __restrict__ npart,
+                                                const u32* __restrict__ kd,
+                                                double* __restrict__ rec, cplx* __restrict__ c,
+                                                int pass) {
+    __shared__ double sh[4];
+    const u32 k = kd[0];
+    const u32 j = blockIdx.x;
+    if (j > k + 1 || (pass == 2 && (j > k || !lz_second(rec, k)))) return;
+    if (j == k + 1) {
+        double s = 0.0;
+        for (u32 b = threadIdx.x; b < nb; b += kT) s += npart[b];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) rec[kRec * k + 1] = s;
+        return;
+    }
+    const cplx* p = part + u64(j) * nb;
+    double re = 0.0, im = 0.0;
+    for (u32 b = threadIdx.x; b < nb; b += kT) {
+        re += p[b].re;
+        im += p[b].im;
+    }
+    re = block_sum(re, sh);
+    im = block_sum(im, sh);
+    if (threadIdx.x == 0) {
+        if (pass == 1 && j == k) re += rec[kRec * k];
+        if (pass == 1 && k > 0 && j == k - 1) re += rec[kRec * (k - 1) + 4];
+        c[j] = cplx{re, im};
+    }
+}
+
+// w -= sum_{j <= k} c_j V_j with the block's partial of |w|^2 afterwards
+template <int PER>
+__global__ void __launch_bounds__(kT) k_lz_msub(const cplx* const* __restrict__ V,
+                                                const u32* __restrict__ kd,
+                                                const cplx* __restrict__ c,
+                                                cplx* __restrict__ w, u64 n,
+                                                double* __restrict__ part,
+                                                const double* __restrict__ rec, int pass) {
+    __shared__ double sh[4];
+    const u32 k = kd[0];
+    if (pass == 2 && !lz_second(rec, k)) return;
+    const u64 start = u64(blockIdx.x) * (PER * kT);
+    const bool full = start + PER * kT <= n;
+    cplx r[PER];
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const u64 i = start + u64(e) * kT + threadIdx.x;
+        r[e] = (full || i < n) ? w[i] : cplx{0.0, 0.0};
+    }
+    for (u32 j = 0; j <= k; ++j) {
+        const cplx* vj = V[j];
+        const cplx cj = c[j];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const u64 i = start + u64(e) * kT + threadIdx.x;
+            if (full || i < n) {
+                const cplx x = vj[i];
+                r[e].re -= cj.re * x.re - cj.im * x.im;
+                r[e].im -= cj.re * x.im + cj.im * x.re;
+            }
+        }
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const u64 i = start + u64(e) * kT + threadIdx.x;
+        if (full || i < n) {
+            w[i] = r[e];
+            acc += r[e].re * r[e].re + r[e].im * r[e].im;
+        }
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// |w|^2 after pass `pass` into the record (field 2 or 3)
+__global__ void __launch_bounds__(kT) k_lz_norm(const double* __restrict__ part, u32 nb,
+                                                const u32* __restrict__ kd,
+                                                double* __restrict__ rec, int pass) {
+    __shared__ double sh[4];
+    const u32 k = kd[0];
+    if (pass == 2 && !lz_second(rec, k)) return;
+    double s = 0.0;
+    for (u32 b = threadIdx.x; b < nb; b += kT) s += part[b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) rec[kRec * k + 1 + pass] = s;
+}
+
+// beta = |w| (the second pass's norm when it ran); V_{k+1} = w / beta (0
+// when beta < 1e-15, the reference's breakdown threshold, so steps queued
+// past a breakdown stay finite) and the same into vcur when given
+__global__ void __launch_bounds__(kT) k_lz_next(cplx* const* __restrict__ V, const u32* __restrict__ kd,
+                                                u32 nslots, const cplx* __restrict__ w, u64 n,
+                                                double* __restrict__ rec, cplx* __restrict__ vcur) {
+    const u32 k = kd[0];
+    const bool second = lz_second(rec, k);
+    const double beta = sqrt(second ? rec[kRec * k + 3] : rec[kRec * k + 2]);
+    const double s = beta < 1e-15 ? 0.0 : 1.0 / beta;
+    const u64 i = u64(blockIdx.x) * kT + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        rec[kRec * k + 4] = beta;
+        rec[kRec * k + 5] = second ? 1.0 : 0.0;
+    }
+    if (i >= n) return;
+    const cplx x = w[i];
+    const cplx y = cplx{x.re * s, x.im * s};
+    if (k + 1 < nslots) V[k + 1][i] = y;
+    if (vcur) vcur[i] = y;
+}
+
+__global__ void k_lz_advance(u32* kd) { kd[0] += 1; }
+
 inline u32 nblk(u64 n) { return static_cast<u32>((n + kCH - 1) / kCH); }
 inline u32 ngrid(u64 n) { return static_cast<u32>((n + kT - 1) / kT); }
 
@@ -721,7 +955,7 @@ void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n,
     // keep long slices, short ones still spread over the chip
     const u64 waves = std::max<u64>(1, std::min<u64>((n + kBDMin - 1) / kBDMin, 4096));
     u64 per_wave = (n + waves - 1) / waves;
-    per_wave = (per_wave + 7) & ~u64(7);                // whole rounds of 8 entries
+    per_wave = std::max<u64>(8, (per_wave + 7) & ~u64(7));   // whole rounds of 8 entries
     const u32 nparts = u32((((n + per_wave - 1) / per_wave) + 3) / 4);
     const int MB = (m + 15) / 16;
     const int stride = MB * 16 * 16;
@@ -766,6 +1000,83 @@ void msub_norm(const cplx* const* V, int k, const cplx* h, cplx* w, u64 n, doubl
     u32 nb = 0;
     launch_msub(V, k, h, w, n, scratch.get<double>(), &nb, st);
     hipLaunchKernelGGL(k_finish_r, dim3(1), dim3(kT), 0, st, scratch.get<double>(), nb, out);
+    QK_HIP(hipGetLastError());
+}
+
+LzStep::LzStep(u64 n_, int nslots_) : n(n_), nslots(nslots_) {
+    per = per_thread(n);
+    const u64 chunk = u64(per) * kT;
+    nb = std::max<u32>(1, u32((n + chunk - 1) / chunk));
+    nbd = std::max<u32>(1, nblk(n));
+    const size_t ns = size_t(nslots) + 1;
+    state.resize(sizeof(u32) * 64 + sizeof(double) * kRec * ns + sizeof(cplx) * ns);
+    hpart.resize(sizeof(cplx) * (size_t(nb) * ns + std::max(nb, nbd)));
+    npart.resize(sizeof(double) * nb);
+}
+
+u32* LzStep::kdev() const { return state.get<u32>(); }
+double* LzStep::rec() const { return reinterpret_cast<double*>(state.get<char>() + sizeof(u32) * 64); }
+cplx* LzStep::coef() const {
+    return reinterpret_cast<cplx*>(state.get<char>() + sizeof(u32) * 64 +
+                                   sizeof(double) * kRec * (size_t(nslots) + 1));
+}
+
+void LzStep::reset(u32 k, hipStream_t st) {
+    // a fill kernel, not hipMemsetAsync, so the whole step stays capturable
+    const size_t words = state.bytes() / sizeof(u32);
+    hipLaunchKernelGGL(k_fill_u32, dim3(u32((words + kT - 1) / kT)), dim3(kT), 0, st, state.get<u32>(),
+                       words, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kT), 0, st, state.get<u32>(), u64(1), k);
+    QK_HIP(hipGetLastError());
+}
+
+void LzStep::alpha_from_parts(const double* part, u32 np, hipStream_t st) const {
+    hipLaunchKernelGGL(k_lz_alpha_parts, dim3(1), dim3(kT), 0, st, part, np, kdev(), rec());
+    QK_HIP(hipGetLastError());
+}
+
+void LzStep::alpha_dot(const cplx* v, const cplx* w, hipStream_t st) const {
+    cplx* part = reinterpret_cast<cplx*>(hpart.get<cplx>() + size_t(nb) * (size_t(nslots) + 1));
+    hipLaunchKernelGGL(k_dot_part, dim3(nbd), dim3(kT), 0, st, v, w, n, part);
+    hipLaunchKernelGGL(k_lz_alpha_dot, dim3(1), dim3(kT), 0, st, part, nbd, kdev(), rec());
+    QK_HIP(hipGetLastError());
+}
+
+void LzStep::reorth(const cplx* const* V, cplx* w, int pass, int coef_blocks, hipStream_t st) const {
+    cplx* part = hpart.get<cplx>();
+    double* np = npart.get<double>();
+    switch (per) {
+#define QK_LZ_MDOT(P)                                                                              \
+    case P:                                                                                        \
+        hipLaunchKernelGGL(k_lz_mdot<P>, dim3(nb), dim3(kT), 0, st, V, kdev(), w, n, part, nb, np, \
+                           rec(), pass);                                                          \
+        break;
+        QK_LZ_MDOT(16) QK_LZ_MDOT(8) QK_LZ_MDOT(4) QK_LZ_MDOT(2)
+        default: hipLaunchKernelGGL(k_lz_mdot<1>, dim3(nb), dim3(kT), 0, st, V, kdev(), w, n, part, nb,
+                                    np, rec(), pass);
+#undef QK_LZ_MDOT
+    }
+    hipLaunchKernelGGL(k_lz_coef, dim3(u32(coef_blocks)), dim3(kT), 0, st, part, nb, np, kdev(), rec(),
+                       coef(), pass);
+    switch (per) {
+#define QK_LZ_MSUB(P)                                                                                    \
+    case P:                                                                                              \
+        hipLaunchKernelGGL(k_lz_msub<P>, dim3(nb), dim3(kT), 0, st, V, kdev(), coef(), w, n, np, rec(), \
+                           pass);                                                                        \
+        break;
+        QK_LZ_MSUB(16) QK_LZ_MSUB(8) QK_LZ_MSUB(4) QK_LZ_MSUB(2)
+        default: hipLaunchKernelGGL(k_lz_msub<1>, dim3(nb), dim3(kT), 0, st, V, kdev(), coef(), w, n, np,
+                                    rec(), pass);
+#undef QK_LZ_MSUB
+    }
+    hipLaunchKernelGGL(k_lz_norm, dim3(1), dim3(kT), 0, st, np, nb, kdev(), rec(), pass);
+    QK_HIP(hipGetLastError());
+}
+
+void LzStep::finish(cplx* const* V, const cplx* w, cplx* vcur, hipStream_t st) const {
+    hipLaunchKernelGGL(k_lz_next, dim3(std::max<u32>(1, ngrid(n))), dim3(kT), 0, st, V, kdev(),
+                       u32(nslots), w, n, rec(), vcur);
+    hipLaunchKernelGGL(k_lz_advance, dim3(1), dim3(1), 0, st, kdev());
     QK_HIP(hipGetLastError());
 }
 

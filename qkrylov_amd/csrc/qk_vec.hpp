@@ -76,6 +76,38 @@ void lincomb_many(const cplx* const* V, int m, const cplx* c, int nout, cplx* ou
 // (reference src/solvers/davidson.cpp:171-175); d: complex diagonal.
 void jacobi_precond(const cplx* r, const cplx* d, double lambda, cplx* t, u64 n, hipStream_t st);
 
+// ---- one Lanczos step with full reorthogonalisation, every index and
+// scalar on the device (the step index and a record of kRecFields doubles
+// per step: alpha, |w|^2 before / after each projection pass, beta, whether
+// the second pass ran).  The kernels of a step take the same arguments for
+// every step, so steps can be queued ahead of the host's stopping test or
+// replayed from one captured graph.  V: device array of nslots vector
+// pointers (V_0..V_k read at step k, V_{k+1} written). ----
+struct LzStep {
+    static constexpr int kRecFields = 8;
+    LzStep(u64 n, int nslots);
+    u32* kdev() const;       // the step index
+    double* rec() const;     // rec[kRecFields * j + f]
+    cplx* coef() const;      // projection coefficients of the current pass
+    // zero every record and set the step index to k (fill kernels, capturable)
+    void reset(u32 k, hipStream_t st);
+    // alpha of the current step from the apply's np partial sums of Re<x, Hx>
+    void alpha_from_parts(const double* part, u32 np, hipStream_t st) const;
+    // alpha of the current step as Re<v, w>
+    void alpha_dot(const cplx* v, const cplx* w, hipStream_t st) const;
+    // projection pass 1 (three-term subtraction folded in) or 2 (runs only
+    // when pass 1 removed more than half of |w|^2); coef_blocks >= k + 2
+    void reorth(const cplx* const* V, cplx* w, int pass, int coef_blocks, hipStream_t st) const;
+    // beta, V_{k+1} = w / beta (and vcur when non-null), then k += 1
+    void finish(cplx* const* V, const cplx* w, cplx* vcur, hipStream_t st) const;
+
+    u64 n;
+    int nslots;
+    int per = 1;
+    u32 nb = 1, nbd = 1;
+    DevBuf state, hpart, npart;
+};
+
 // ---- batched recurrences over Segments (FTLM / continued fractions) ----
 // Buffers U, Y, P are flat (seg.total elements).  Per-sample device arrays:
 // c (scale of the current vector), cp (scale of the previous vector),
