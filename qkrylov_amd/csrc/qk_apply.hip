@@ -814,10 +814,6 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
 template <typename S>
 void launch_apply_s(const Hamiltonian& h, const cplx* x, cplx* y, int nvec, u64 ld,
                     hipStream_t st) {
-    if (h.xchg && h.xpass) {   // the two-pass apply, vector by vector
-        for (int v = 0; v < nvec; ++v) h.apply_xpass(x + u64(v) * ld, y + u64(v) * ld, nullptr, st);
-        return;
-    }
     // the exchange path may run blocks as groups of fewer vectors (1: one
     // single-vector launch per vector), as chosen at set-up
     const int gmax = h.xchg ? h.vec_group : h.gen_group;
@@ -1433,7 +1429,6 @@ void Hamiltonian::setup_exchange() {
     xchg = true;
     set_single_kernel(s0, nb0);
     xchg = true;
-    setup_xpass();
 }
 
 // Tiles of `span` consecutive ranks of one row, rows in the order of
@@ -1564,8 +1559,6 @@ std::string Hamiltonian::autotune(int nvec) {
     }
     double best = 1e300;
     int bs = spl, bn = bond_batch;
-    const bool xp_was = xpass;
-    xpass = false;   // the single-pass kernels first
     os << "\"single\": {";
     bool first = true;
     for (int s : {1, 2, 4})
@@ -1583,18 +1576,7 @@ std::string Hamiltonian::autotune(int nvec) {
         }
     set_single_kernel(bs, bn);
     os << "}, \"states_per_lane\": " << bs << ", \"bond_batch\": " << bn;
-    if (xpass_ok) {
-        // the two-pass apply against the best single-pass kernel
-        xpass = true;
-        const double ms = time_ms(1, reps);
-        os << ", \"two_pass\": " << ms;
-        xpass = ms < best;
-        if (const char* e = std::getenv("QKRYLOV_XPASS")) xpass = std::atoi(e) != 0 && xpass_ok;
-        os << ", \"xpass\": " << (xpass ? 1 : 0);
-    } else {
-        xpass = xp_was;
-    }
-    if (nvec > 1 && !xpass) {
+    if (nvec > 1) {
         double bestv = 1e300;
         int bg = vec_group;
         os << ", \"groups\": {";
@@ -1716,12 +1698,8 @@ Hamiltonian::~Hamiltonian() {
 }
 
 bool Hamiltonian::apply_dot(const cplx* x, cplx* y, double* part, hipStream_t st) const {
-    if (!st) st = stream;
-    if (xchg && xpass && dim() > 0) {
-        apply_xpass(x, y, part, st);
-        return true;
-    }
     if (!xchg || !xs_fn || dim() == 0) return false;
+    if (!st) st = stream;
     XParams q = xparams_s;
     q.dot_part = part;
     const cplx* xa = x;
@@ -1859,11 +1837,7 @@ std::string Hamiltonian::info() const {
        << ", \"row_path\": " << (rows ? 1 : 0) << ", \"row_single\": " << (rows && rows_1 ? 1 : 0)
        << ", \"row_block\": " << (rows && rows_b ? 1 : 0) << ", \"gen_group\": " << gen_group
        << ", \"loop_groups\": " << (rows ? rest.ngroups : params.ngroups)
-       << ", \"row_lds\": " << rlds_bytes << ", \"two_pass\": " << (xpass ? 1 : 0)
-       << ", \"two_pass_ok\": " << (xpass_ok ? 1 : 0) << ", \"pair_mask\": " << xpp.pair_mhi
-       << ", \"row_groups\": " << xp_groups_n << ", \"col_tiles\": " << xp_ctiles_n
-       << ", \"row_lds_x\": " << xp_lds << ", \"grid_r\": " << xp_grid_r << ", \"grid_c\": " << xp_grid_c
-       << "}";
+       << ", \"row_lds\": " << rlds_bytes << "}";
     return os.str();
 }
 

@@ -89,60 +89,6 @@ struct XParams {
     const u32* vrow;        // null: records indexed by hi; else by vrow[hi]
 };
 
-// Two-pass exchange apply (qk_xpass.hip).  Row group of pass R: row hi and,
-// when the pairing bit P is set up, row hi ^ P, staged together in LDS
-// (cnt[1] = 0: one row).
-struct XGroup {
-    u64 base[2];   // jhi of each row
-    u32 hi[2];
-    u32 pos0[2];   // lo_list offset of each row's class
-    u32 cnt[2];    // states per row
-    u32 vr[2];     // per-row record slot (hnb, hcnt, mnb)
-    double dh[2];  // high-half diagonal of each row
-};
-static_assert(sizeof(XGroup) == 64, "XGroup layout");
-
-// Band of pass C: columns [c0, c0 + w) (w <= 4) of every row of one class,
-// rows crow[row0 .. row0 + nrows), staged together in LDS by one workgroup.
-struct XCBand {
-    u32 row0, nrows, c0, w;
-};
-static_assert(sizeof(XCBand) == 16, "XCBand layout");
-
-// Row of a class for pass C: its states' base index and its high-half
-// partners, cpart[pptr .. pptr + pcnt): partner row (index within the class)
-// in bits 0-23, amplitude id (cpamp) in bits 24-31.
-struct XCRow {
-    u64 base;
-    u32 pptr, pcnt;
-};
-static_assert(sizeof(XCRow) == 16, "XCRow layout");
-
-struct XPassParams {
-    const XGroup* groups;
-    const u32* gchunk_off;    // row-group chunks: chunk k = groups [off[k], off[k+1]), XCD k % 8
-    const XCBand* cbands;
-    const XCRow* crow;
-    const u32* cpart;
-    const double* cpamp;
-    const u32* cchunk_off;    // band chunks: bands [off[k], off[k+1]), XCD k % 8
-    const u16* jlo;
-    const u32* lo_list;
-    const double* dlo;
-    const XBond* bonds;       // lo, hi, mixed (as XParams)
-    const DevGroup* dmix;
-    const cplx* dmix_tab;
-    const XRowNb* hnb;
-    const u32* hcnt;
-    const u64* mnb;
-    double* dot_part;         // pass R: workgroup partial sums of Re<x, y> (may be null)
-    u32 ngchunks, ncchunks, ncpamp;
-    u32 n_lo, n_hi, n_mix, n_dmix, ndmix_tab, nlo;
-    int L;
-    u32 pair_mhi;             // high-half mask of the pairing bit (0: none)
-    u32 lds_x_off;            // byte offset of the staged states in LDS
-};
-
 // Tile of the multi-sector exchange kernel: 64 ranks of one row of sector
 // `sec`, applied to all nvec vectors of that sector (vector v at
 // xoff + v*ld of the concatenated buffers).
@@ -251,15 +197,12 @@ struct Hamiltonian {
     // dot_parts() partial sums of Re<x, y> into part; false (nothing launched)
     // when that kernel is not this Hamiltonian's single-vector path.
     bool apply_dot(const cplx* x, cplx* y, double* part, hipStream_t st) const;
-    int dot_parts() const { return xpass ? xp_grid_r : xgrid_s; }
+    int dot_parts() const { return xgrid_s; }
     // apply_dot is available and apply() of nvec vectors runs that same
     // kernel vector by vector
     bool fused_dot(int nvec) const {
-        return xchg && dim() > 0 && (xpass || (xs_fn && (vec_group == 1 || nvec == 1)));
+        return xchg && xs_fn && dim() > 0 && (vec_group == 1 || nvec == 1);
     }
-    // the two-pass exchange apply of one vector (qk_xpass.hip); dot_part
-    // (may be null) receives dot_parts() partial sums of Re<x, y>
-    void apply_xpass(const cplx* x, cplx* y, double* dot_part, hipStream_t st) const;
     // Full complex diagonal in OpSum order (device output, dim entries).
     void diagonal(cplx* d, hipStream_t st) const;
     // phi[k*ld + a] = N^-1/2 sum_j e^{i 2 pi k j / N} Sz_j psi[a], k < nq.
@@ -272,7 +215,6 @@ struct Hamiltonian {
     // vectors and keep the fastest; returns the timings (JSON).
     std::string autotune(int nvec);
     void setup_exchange();
-    void setup_xpass();
     void setup_rows();
     void set_single_kernel(int states_per_lane, int bond_batch);
     std::vector<XTile> xtiles_for(u32 span) const;
@@ -307,14 +249,6 @@ struct Hamiltonian {
     bool xchg = false;          // exchange fast path selected
     int xlds_bytes = 0;
     int xgrid = 0;
-    // two-pass exchange apply (setup_xpass): used for every exchange-path
-    // apply when xpass is set (autotune() compares it with the single-pass
-    // kernel)
-    bool xpass = false, xpass_ok = false;
-    XPassParams xpp{};
-    int xp_grid_r = 0, xp_grid_c = 0, xp_lds = 0, xp_lds_c = 0;
-    u32 xp_groups_n = 0, xp_ctiles_n = 0;
-    DevBuf d_xgroups, d_xgchunk, d_xcbands, d_xcrow, d_xcpart, d_xcpamp, d_xcchunk;
     bool force_generic = false; // QKRYLOV_APPLY=generic
     // row-program path (setup_rows): the remaining groups, their tables and
     // the row-program tables

@@ -22,17 +22,14 @@ def _rand(d, seed, nvec=None):
 EXCHANGE_MODELS = {"heis_chain10_sz0", "heis_obc4_sz0", "j1j2_4x3_sz0", "heis_chain20_sz0"}
 
 
-@pytest.mark.parametrize("path", ["auto", "generic", "rows", "two_pass"])
+@pytest.mark.parametrize("path", ["auto", "generic", "rows"])
 @pytest.mark.parametrize("name", sorted(MODELS))
 def test_apply_matches_oracle(qk, orc, name, path, monkeypatch):
     """auto: exchange kernel where it applies, else the row-program kernel;
     generic: every group in the group loop; rows: the row-program kernel
-    also for exchange models; two_pass: the two-pass exchange apply (column
-    bands for high-half bonds, LDS-staged row groups for the rest)."""
+    also for exchange models."""
     print(f"[apply] {name} path={path}", flush=True)
-    if path == "two_pass":
-        monkeypatch.setenv("QKRYLOV_XPASS", "1")
-    elif path != "auto":
+    if path != "auto":
         monkeypatch.setenv("QKRYLOV_APPLY", path)
     H, oh, _ = build_pair(orc, name)
     info = H.info()
@@ -43,8 +40,6 @@ def test_apply_matches_oracle(qk, orc, name, path, monkeypatch):
     if path == "rows":
         assert info["exchange_path"] == 0
         assert info["row_path"] == 1
-    if path == "two_pass" and name in EXCHANGE_MODELS:
-        assert info["two_pass"] == 1
     assert H.dimension == oh.dim
     x = _rand(H.dimension, 11)
     y = H.apply(x)
@@ -95,19 +90,6 @@ def test_block_apply_equals_single(qk, orc, nvec, name):
     Y = H.apply(X)
     for v in range(nvec):
         assert np.array_equal(Y[v], H.apply(X[v]))
-
-
-def test_two_pass_block_equals_single(qk, orc, monkeypatch):
-    """The two-pass exchange apply runs blocks vector by vector: identical bits."""
-    monkeypatch.setenv("QKRYLOV_XPASS", "1")
-    H, oh, _ = build_pair(orc, "j1j2_4x3_sz0")
-    assert H.info()["two_pass"] == 1
-    X = _rand(H.dimension, 6, 5)
-    Y = H.apply(X)
-    for v in range(5):
-        assert np.array_equal(Y[v], H.apply(X[v]))
-    y_ref = oh.apply(X[2])
-    assert np.abs(Y[2] - y_ref).max() <= 1e-12 * np.abs(y_ref).max()
 
 
 def test_float_abi_matches(qk, orc):
