@@ -456,6 +456,21 @@ int qkrylov_sz_q_device_f64(qkrylov_hamiltonian_h h, const void* psi_dev, void* 
 int qkrylov_block_dot_f64(const double* V_complex, int m, const double* W_complex, int nw,
                           uint64_t n, double* G_complex_out);
 
+/* G = V^H W as qkrylov_block_dot_f64 computes it (same kernels, bit for
+ * bit), with device buffers and no allocation of any kind: no reference
+ * counterpart; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * V_i = V_dev + i*ldv and W_j = W_dev + j*ldw complex doubles (i < m <= 64,
+ * j < nw <= 16, ldv, ldw >= n); G_dev receives m*nw complex doubles,
+ * G[i*nw + j] = sum_k conj(V_i[k]) W_j[k].  Asynchronous on stream (a
+ * hipStream_t, NULL: the default stream).  Its scratch (pointer tables and
+ * partial sums) is one device array of 16 MiB + 1 KiB per GPU, held until
+ * the process ends and shared by every call on that GPU: calls that overlap
+ * in time on one GPU race.  Sizes beyond the array (none within the m, nw
+ * limits) return QKRYLOV_ERROR_INVALID_ARG. */
+int qkrylov_block_dot_noalloc_f64(const void* V_dev, uint64_t ldv, int m, const void* W_dev,
+                                  uint64_t ldw, int nw, uint64_t n, void* G_dev, void* stream);
+
 /* Synchronise the handle's stream. */
 int qkrylov_hamiltonian_sync(qkrylov_hamiltonian_h h);
 

@@ -125,6 +125,8 @@ SIGNATURES = {
     "qkrylov_hamiltonian_autotune": (c_int, [c_void_p, c_int, C.c_char_p, c_size_t]),
     "qkrylov_lanczos_ground_state_f64": (c_int, [c_void_p, c_int, c_double, c_void_p,
                                                  P(LanczosResultF64), c_void_p]),
+    "qkrylov_block_dot_noalloc_f64": (c_int, [c_void_p, C.c_uint64, c_int, c_void_p, C.c_uint64, c_int,
+                                              C.c_uint64, c_void_p, c_void_p]),
     "qkrylov_lanczos_checkpointed_f64": (c_int, [c_void_p, c_int, c_double, c_void_p, C.c_char_p, c_int,
                                                  c_int, C.c_char_p, P(LanczosResultF64), c_void_p]),
     "qkrylov_eigenpairs_save_f64": (c_int, [C.c_char_p, C.c_uint64, c_int, c_void_p, c_void_p,

@@ -122,6 +122,12 @@ std::vector<cplx> diag_host(const Hamiltonian& H) {
 
 }  // namespace
 
+namespace qk {
+// the calling thread's message for qkrylov_last_error(), for entry points
+// defined in other files
+void capi_set_error(const char* msg) { set_error(msg); }
+}  // namespace qk
+
 extern "C" {
 
 const char* qkrylov_last_error(void) { return g_last_error.c_str(); }

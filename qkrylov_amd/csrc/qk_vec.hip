@@ -947,29 +947,54 @@ void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& sc
     QK_HIP(hipGetLastError());
 }
 
+namespace {
+// Decomposition of block_dot: at most 4096 waves, each over at least kBDMin
+// entries (long vectors keep long slices, short ones still spread over the
+// chip), whole rounds of 8 entries; 4 waves per workgroup partial.
+struct BDShape {
+    u64 per_wave;
+    u32 parts;    // workgroups (partials)
+    int mb;       // 16-row blocks of V
+    int stride;   // cplx per partial
+};
+BDShape bd_shape(int m, u64 n) {
+    BDShape s{};
+    const u64 waves = std::max<u64>(1, std::min<u64>((n + kBDMin - 1) / kBDMin, 4096));
+    s.per_wave = std::max<u64>(8, (((n + waves - 1) / waves) + 7) & ~u64(7));
+    s.parts = std::max<u32>(1, u32((((n + s.per_wave - 1) / s.per_wave) + 3) / 4));
+    s.mb = (m + 15) / 16;
+    s.stride = s.mb * 16 * 16;
+    return s;
+}
+}  // namespace
+
+size_t block_dot_scratch_bytes(int m, u64 n) {
+    const BDShape s = bd_shape(m, n);
+    return sizeof(cplx) * size_t(s.parts) * size_t(s.stride);
+}
+
+void block_dot_into(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n, cplx* G,
+                    cplx* part, hipStream_t st) {
+    if (m <= 0 || nw <= 0) return;
+    if (m > 64 || nw > 16) fail("block_dot: at most 64 x 16 projections per call");
+    const BDShape s = bd_shape(m, n);
+    const u32 g = s.parts;
+    switch (s.mb) {
+        case 1: hipLaunchKernelGGL((k_block_dot_part<1>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, s.per_wave, part); break;
+        case 2: hipLaunchKernelGGL((k_block_dot_part<2>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, s.per_wave, part); break;
+        case 3: hipLaunchKernelGGL((k_block_dot_part<3>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, s.per_wave, part); break;
+        default: hipLaunchKernelGGL((k_block_dot_part<4>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, s.per_wave, part); break;
+    }
+    hipLaunchKernelGGL(k_block_dot_fin, dim3(m * nw), dim3(kT), 0, st, part, int(g), s.stride, m, nw, G);
+    QK_HIP(hipGetLastError());
+}
+
 void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n, cplx* G,
                DevBuf& scratch, hipStream_t st) {
     if (m <= 0 || nw <= 0) return;
     if (m > 64 || nw > 16) fail("block_dot: at most 64 x 16 projections per call");
-    // at most 4096 waves, each over at least kBDMin entries: long vectors
-    // keep long slices, short ones still spread over the chip
-    const u64 waves = std::max<u64>(1, std::min<u64>((n + kBDMin - 1) / kBDMin, 4096));
-    u64 per_wave = (n + waves - 1) / waves;
-    per_wave = std::max<u64>(8, (per_wave + 7) & ~u64(7));   // whole rounds of 8 entries
-    const u32 nparts = u32((((n + per_wave - 1) / per_wave) + 3) / 4);
-    const int MB = (m + 15) / 16;
-    const int stride = MB * 16 * 16;
-    scratch.resize(sizeof(cplx) * size_t(std::max<u32>(nparts, 1)) * stride);
-    cplx* part = scratch.get<cplx>();
-    const u32 g = std::max<u32>(nparts, 1);
-    switch (MB) {
-        case 1: hipLaunchKernelGGL((k_block_dot_part<1>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
-        case 2: hipLaunchKernelGGL((k_block_dot_part<2>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
-        case 3: hipLaunchKernelGGL((k_block_dot_part<3>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
-        default: hipLaunchKernelGGL((k_block_dot_part<4>), dim3(g), dim3(kT), 0, st, V, m, W, nw, n, per_wave, part); break;
-    }
-    hipLaunchKernelGGL(k_block_dot_fin, dim3(m * nw), dim3(kT), 0, st, part, int(g), stride, m, nw, G);
-    QK_HIP(hipGetLastError());
+    scratch.resize(block_dot_scratch_bytes(m, n));
+    block_dot_into(V, m, W, nw, n, G, scratch.get<cplx>(), st);
 }
 
 void block_sub(const cplx* const* V, int m, cplx* const* W, int nw, const cplx* G, u64 n,

@@ -57,6 +57,11 @@ void mdot(const cplx* const* V, int k, const cplx* w, u64 n, cplx* h, DevBuf& sc
 // (FP64 MFMA; each vector read once; deterministic two-stage reduction).
 void block_dot(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n, cplx* G,
                DevBuf& scratch, hipStream_t st);
+// The same with caller-provided partial storage of block_dot_scratch_bytes(m, n)
+// bytes (every byte is written before it is read: no clearing needed).
+size_t block_dot_scratch_bytes(int m, u64 n);
+void block_dot_into(const cplx* const* V, int m, const cplx* const* W, int nw, u64 n, cplx* G,
+                    cplx* part, hipStream_t st);
 // W_j -= sum_i V_i G[i*nw + j] for nw <= 16 vectors (G on device); each V_i
 // is read once for all W_j
 void block_sub(const cplx* const* V, int m, cplx* const* W, int nw, const cplx* G, u64 n,
