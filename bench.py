@@ -153,15 +153,21 @@ def bench_ftlm(qk, lat):
     t = time.perf_counter()
     qk.ftlm_sweep(hs, betas[:1], R, 2, batched=True)
     qk.ftlm_sweep(hs, betas[:1], 1, 2, batched=False)
+    qk.ftlm_sweep(hs, betas[:1], 1, 2, batched=False, plain_alpha="dot")
     log(f"ftlm warm-up (random stream, kernels): {time.perf_counter() - t:.3f} s")
     out = {}
-    for name, batched in (("batched", True), ("plain", False)):
+    # plain: one vector at a time, alpha from the apply kernel's partial sums
+    # (the fastest plain loop); plain_dot: alpha from a separate dot, as the
+    # reference loop takes it -- the batched sweep's arithmetic per vector
+    for name, batched, how in (("batched", True, "fused"), ("plain", False, "fused"),
+                               ("plain_dot", False, "dot")):
         t = time.perf_counter()
-        Z, E, Cv = qk.ftlm_sweep(hs, betas, R, M, batched=batched)
+        Z, E, Cv = qk.ftlm_sweep(hs, betas, R, M, batched=batched, plain_alpha=how)
         dt = time.perf_counter() - t
         out[name] = (dt, Z, E, Cv)
         log(f"ftlm {name}: {dt:.3f} s")
     (tb, Zb, Eb, Cb), (tp, Zp, Ep, Cp) = out["batched"], out["plain"]
+    (td, Zd, Ed, Cd) = out["plain_dot"]
 
     def reldiff(a, b):
         return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
@@ -174,10 +180,16 @@ def bench_ftlm(qk, lat):
         "samples": nsamp,
         "samples_per_s_batched": nsamp / tb,
         "samples_per_s_plain": nsamp / tp,
+        "samples_per_s_plain_dot": nsamp / td,
         "speedup": tp / tb,
-        "max_rel_diff_Z": reldiff(Zb, Zp),
-        "max_rel_diff_E": reldiff(Eb, Ep),
-        "max_rel_diff_Cv": reldiff(Cb, Cp),
+        "speedup_note": "against the faster plain loop (alpha from the apply kernel's partial sums)",
+        "max_rel_diff_Z": reldiff(Zb, Zd),
+        "max_rel_diff_E": reldiff(Eb, Ed),
+        "max_rel_diff_Cv": reldiff(Cb, Cd),
+        "diff_note": "batched against the plain loop with alpha from a separate dot (same arithmetic "
+                     "per vector); against the fused-alpha plain loop the summation order of alpha "
+                     "differs",
+        "max_rel_diff_Cv_fused_plain": reldiff(Cb, Cp),
         "E_per_site_T0.05": float(Et[0] / n),
     }
 
@@ -260,6 +272,26 @@ def bench_davidson_tj(qk, lat, msub=48, max_iter=400):
     return out
 
 
+def bench_tj_apply(qk, torch, lat, reps=200):
+    """y = H x for the t-J model on the 4x4 periodic lattice (t=1, J=0.4,
+    7 up, 7 down, d=411,840), one vector and blocks of 8, after the
+    autotuner's choice; times from HIP events on the apply's stream."""
+    ops = lat.tj_model(lat.square_bonds(4, 4), 1.0, 0.4)
+    sec = qk.Sector().set_hubbard_particles(7, 7)
+    H = qk.MatrixFreeHamiltonian(qk.TJBasis(16, sector=sec), qk.TJSite(), ops)
+    d = H.dimension
+    tuning = H.autotune(8)
+    out = {"workload": "t-J 4x4 periodic, t=1, J=0.4, 7 up 7 down, d=411,840; y = H x",
+           "dim": d, "autotune": tuning}
+    for nvec in (1, 8):
+        ms, _ = bench_apply(qk, torch, H, d, nvec, reps, 10)
+        gbs = BYTES_PER_STATE * d * nvec / (ms * 1e-3) / 1e9
+        out[f"b{nvec}"] = {"ms_per_launch": ms, "ms_per_vector": ms / nvec, "achieved_GBs": gbs,
+                           "frac": gbs / HBM_PEAK_GBS}
+        log(f"t-J 4x4 apply b={nvec}: {ms:.4f} ms ({ms / nvec:.4f} ms per vector, {gbs:.0f} GB/s)")
+    return out
+
+
 def bench_dynamics(qk, torch, H, d):
     """N=28 chain: ground state by Lanczos (reference start, tol 1e-12), then
     S^z(q)|0> for the 28 momenta q = 2 pi k / 28 as one block of start vectors,
@@ -323,6 +355,8 @@ def main():
     ap.add_argument("--skip-lanczos", action="store_true", help="skip the J1-J2 6x4 Lanczos run")
     ap.add_argument("--skip-davidson", action="store_true", help="skip the t-J 4x4 Davidson runs")
     ap.add_argument("--skip-dynamics", action="store_true", help="skip the N=28 dynamics run")
+    ap.add_argument("--tj-apply", action="store_true",
+                    help="time the t-J 4x4 apply only (one vector and blocks of 8)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device 5 times (calibrates PMC byte counters)")
     args = ap.parse_args()
@@ -350,6 +384,9 @@ def main():
         return
     if args.davidson_only:
         print(json.dumps(bench_davidson_tj(qk, lat)))
+        return
+    if args.tj_apply:
+        print(json.dumps(bench_tj_apply(qk, torch, lat)))
         return
     t = time.time()
     basis = qk.SpinHalfBasis(N_SITES, sz=0)

@@ -439,7 +439,10 @@ int qkrylov_ftlm_f64(qkrylov_hamiltonian_h h, double beta, int n_random, int n_s
  * the outputs Z[s*nbeta+b], E[...], Cv[...] equal qkrylov_ftlm_f64(hs[s],
  * betas[b], n_random, n_steps).  mode 0: every (sector, random vector) pair
  * advanced together in one block recurrence per step; mode 1: one vector at
- * a time (the plain loop), kept for comparison. */
+ * a time (the plain loop), alpha from the apply kernel's partial sums where
+ * the kernel provides them; mode 2: the plain loop with alpha from a separate
+ * dot as the reference's loop takes it (ftlm.cpp:24-53), the same arithmetic
+ * per vector as mode 0. */
 int qkrylov_ftlm_sweep_f64(const qkrylov_hamiltonian_h* hs, int nsec, const double* betas, int nbeta,
                            int n_random, int n_steps, int mode,
                            double* Z_out, double* E_out, double* Cv_out);

@@ -967,8 +967,13 @@ def ftlm(H: MatrixFreeHamiltonian, beta: float, n_random: int = 50, n_steps: int
 
 
 def ftlm_sweep(Hs: Sequence[MatrixFreeHamiltonian], betas, n_random: int = 50,
-               n_steps: int = 100, batched: bool = True):
-    """Z, E, Cv arrays of shape [len(Hs), len(betas)]; row s equals ftlm(Hs[s], beta)."""
+               n_steps: int = 100, batched: bool = True, plain_alpha: str = "fused"):
+    """Z, E, Cv arrays of shape [len(Hs), len(betas)]; row s equals ftlm(Hs[s], beta).
+    batched=False runs one vector at a time; plain_alpha="fused" takes alpha
+    from the apply kernel's partial sums, "dot" from a separate pass (the
+    reference loop's structure, the batched sweep's arithmetic)."""
+    if plain_alpha not in ("fused", "dot"):
+        raise ValueError("plain_alpha is 'fused' or 'dot'")
     b = np.ascontiguousarray(betas, dtype=np.float64)
     ns, nb = len(Hs), b.size
     hs = (ct.c_void_p * ns)(*[h._h for h in Hs])
@@ -976,5 +981,6 @@ def ftlm_sweep(Hs: Sequence[MatrixFreeHamiltonian], betas, n_random: int = 50,
     E = np.zeros((ns, nb))
     Cv = np.zeros((ns, nb))
     check(load().qkrylov_ftlm_sweep_f64(hs, ns, _ptr(b), nb, int(n_random), int(n_steps),
-                                        0 if batched else 1, _ptr(Z), _ptr(E), _ptr(Cv)), "ftlm_sweep")
+                                        0 if batched else (1 if plain_alpha == "fused" else 2),
+                                        _ptr(Z), _ptr(E), _ptr(Cv)), "ftlm_sweep")
     return Z, E, Cv

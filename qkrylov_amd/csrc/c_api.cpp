@@ -680,8 +680,8 @@ int qkrylov_mlp_features(qkrylov_hamiltonian_h h, double theta, float* out) {
     if (!hok(h) || !out) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {
         const Hamiltonian& H = *h->ptr;
-        MlpStates s;
-        s.build(H, H.stream);
+        const std::shared_ptr<const MlpStates> sp = mlp_states_of(H);
+        const MlpStates& s = *sp;
         const u64 d = H.dim();
         DevBuf X(sizeof(u16) * std::max<u64>(d, 1) * kMlpIn);
         mlp_features(s, theta, X.get<u16>(), H.stream);

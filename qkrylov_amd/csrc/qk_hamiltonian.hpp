@@ -183,6 +183,8 @@ struct ColTables {
     u64 ld_out = 0;
 };
 
+struct MlpStates;   // qk_mlp.hpp
+
 struct Hamiltonian {
     Hamiltonian(std::shared_ptr<const Basis> basis, SiteKind site, const OpSum& ops);
     ~Hamiltonian();
@@ -268,6 +270,9 @@ struct Hamiltonian {
 
     // scratch for host-buffer entry points
     mutable DevBuf scratch_x, scratch_y;
+    // per-state inputs of the learned Davidson preconditioner, built on first
+    // use (mlp_states_of) and kept for the Hamiltonian's lifetime
+    mutable std::shared_ptr<MlpStates> mlp_states;
 };
 
 // y = H_s x for several Hamiltonians (sectors of one model) and several

@@ -175,8 +175,8 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
     // K applied to a MINRES solution when cfg.solve_iters > 0; and the two
     // correction terms
     // z = J r, w = J (H z - diag z) of every pair; everything pair-major
-    MlpStates states;
-    states.build(Hs, st);
+    const std::shared_ptr<const MlpStates> states_p = mlp_states_of(Hs);
+    const MlpStates& states = *states_p;
     DevBuf Z(sizeof(cplx) * n * P), W(sizeof(cplx) * n * P), T(sizeof(cplx) * n * P),
         U(sizeof(cplx) * n * P), hz(sizeof(cplx) * n), ts(sizeof(cplx) * n);
     Blas b{st, n};
@@ -258,9 +258,8 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
 // ---------------------------------------------------------------------------
 // the preconditioner
 // ---------------------------------------------------------------------------
-MlpPreconditioner::MlpPreconditioner(const Mlp& net, const Hamiltonian& H) : net_(net) {
-    states_.build(H, H.stream);
-}
+MlpPreconditioner::MlpPreconditioner(const Mlp& net, const Hamiltonian& H)
+    : net_(net), sp_(mlp_states_of(H)), states_(*sp_) {}
 
 void MlpPreconditioner::apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda,
                               const cplx* u, cplx* t, hipStream_t st) {

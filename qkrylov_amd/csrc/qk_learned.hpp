@@ -57,7 +57,8 @@ public:
 
 private:
     const Mlp& net_;
-    MlpStates states_;
+    std::shared_ptr<const MlpStates> sp_;   // H's tables (cached on H)
+    const MlpStates& states_;
     DevBuf X_, o_, z_, w_, red_;
 };
 

@@ -347,6 +347,19 @@ __global__ void k_mlp_sum_parts(const float* __restrict__ part, int nparts, int 
     dst[j] = s;
 }
 
+// first stage of a long sum of partials: chunk c = blockIdx.y adds partials
+// [c*per, (c+1)*per) in order into tmp[c][j]; k_mlp_sum_parts then adds the
+// chunks in order (fixed order, deterministic, many chains in flight)
+__global__ void k_mlp_sum_chunks(const float* __restrict__ part, int nparts, int stride, int len,
+                                 int per, float* __restrict__ tmp) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    const int b0 = int(blockIdx.y) * per, b1 = min(nparts, b0 + per);
+    float s = 0.f;
+    for (int b = b0; b < b1; ++b) s += part[size_t(b) * stride + j];
+    tmp[size_t(blockIdx.y) * len + j] = s;
+}
+
 __global__ void k_mlp_adam(float* __restrict__ p, float* __restrict__ m1, float* __restrict__ m2,
                            const float* __restrict__ g, int n, float lr, float c1, float c2) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -736,13 +749,23 @@ void Mlp::backward(const u16* X, const u16* h1, const u16* h2, const float* go, 
                        w2k.get<u16>(), w3b.get<u16>(), gz2, gz1, p3);
     hipLaunchKernelGGL((k_mlp_wgrad<kIn>), dim3(gw), dim3(kThreads), 0, st, X, gz1, n, p1);
     hipLaunchKernelGGL((k_mlp_wgrad<kHid>), dim3(gw), dim3(kThreads), 0, st, h1, gz2, n, p2);
-    // W1 and b1 are contiguous in the parameter block, as are W2 and b2
-    hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(s1) + 255) / 256), dim3(256), 0, st, p1, gw,
-                       int(s1), int(s1), grads + kMlpW1);
-    hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(s2) + 255) / 256), dim3(256), 0, st, p2, gw,
-                       int(s2), int(s2), grads + kMlpW2);
-    hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(s3) + 255) / 256), dim3(256), 0, st, p3, ga,
-                       int(s3), int(s3), grads + kMlpW3);
+    // W1 and b1 are contiguous in the parameter block, as are W2 and b2;
+    // partials summed in chunks of kPer, then the chunks in order
+    constexpr int kPer = 16;
+    const int c1 = (gw + kPer - 1) / kPer, c3 = (ga + kPer - 1) / kPer;
+    sum_.resize(sizeof(float) * (size_t(c1) * (s1 + s2) + size_t(c3) * s3));
+    float* t1 = sum_.get<float>();
+    float* t2 = t1 + size_t(c1) * s1;
+    float* t3 = t2 + size_t(c1) * s2;
+    auto two_stage = [&](const float* p, int np, size_t len, int nc, float* tmp, float* dst) {
+        hipLaunchKernelGGL(k_mlp_sum_chunks, dim3((unsigned(len) + 255) / 256, unsigned(nc)), dim3(256), 0, st,
+                           p, np, int(len), int(len), kPer, tmp);
+        hipLaunchKernelGGL(k_mlp_sum_parts, dim3((int(len) + 255) / 256), dim3(256), 0, st, tmp, nc,
+                           int(len), int(len), dst);
+    };
+    two_stage(p1, gw, s1, c1, t1, grads + kMlpW1);
+    two_stage(p2, gw, s2, c1, t2, grads + kMlpW2);
+    two_stage(p3, ga, s3, c3, t3, grads + kMlpW3);
     QK_HIP(hipGetLastError());
 }
 
@@ -781,6 +804,16 @@ void Mlp::eval_host(const float* Xh, u64 n, const float* goh, float* oh, float* 
 // ---------------------------------------------------------------------------
 // per-state statistics: occupation fractions and connectivity
 // ---------------------------------------------------------------------------
+std::shared_ptr<const MlpStates> mlp_states_of(const Hamiltonian& H) {
+    if (!H.mlp_states) {
+        auto s = std::make_shared<MlpStates>();
+        s->build(H, H.stream);
+        QK_HIP(hipStreamSynchronize(H.stream));
+        H.mlp_states = std::move(s);
+    }
+    return H.mlp_states;
+}
+
 void MlpStates::build(const Hamiltonian& H, hipStream_t st) {
     const Basis& B = *H.basis;
     n = B.size();

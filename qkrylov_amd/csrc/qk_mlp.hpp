@@ -20,6 +20,8 @@
 // their energy.
 #pragma once
 
+#include <memory>
+
 #include "qk_device.hpp"
 
 #include <vector>
@@ -101,7 +103,7 @@ public:
 private:
     void cast();
     hipStream_t st_ = nullptr;
-    DevBuf part_, wg_;       // backward partial sums
+    DevBuf part_, wg_, sum_;   // backward partial sums, their chunk sums
 };
 
 // Per-state data of one Hamiltonian that the features need, on the device.
@@ -118,6 +120,10 @@ struct MlpStates {
     u64 ncls = 0;
     void build(const Hamiltonian& H, hipStream_t st);
 };
+// H's MlpStates: depend only on H (not on a network or theta), so they are
+// built once per Hamiltonian and shared by every preconditioner and
+// training run on it.
+std::shared_ptr<const MlpStates> mlp_states_of(const Hamiltonian& H);
 
 // The corrections of one Davidson iteration, passed by value to the batched
 // kernels below (grid.y = correction): residuals r, outputs t, Ritz vectors

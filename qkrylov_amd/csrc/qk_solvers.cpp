@@ -1111,7 +1111,11 @@ struct BlockRec {
     hipStream_t st = nullptr;
     MultiApply multi;   // all groups in one apply launch when they allow it
 
-    void setup(const std::vector<std::pair<const Hamiltonian*, int>>& g, hipStream_t stream) {
+    // allow_fused: alpha may come from the apply kernel's partial sums (else
+    // always from the separate pass, whose per-sample reduction is the same
+    // whatever other samples share the recurrence)
+    void setup(const std::vector<std::pair<const Hamiltonian*, int>>& g, hipStream_t stream,
+               bool allow_fused = true) {
         st = stream;
         std::vector<u64> lens;
         u64 off = 0;
@@ -1141,7 +1145,7 @@ struct BlockRec {
         // reading both vectors again every step.
         dstride = 0;
         const char* f = std::getenv("QKRYLOV_FUSED_ALPHA");
-        bool fused = !multi.ok() && !(f && std::string(f) == "0") && !grps.empty();
+        bool fused = allow_fused && !multi.ok() && !(f && std::string(f) == "0") && !grps.empty();
         for (const Grp& x : grps)
             if (!(x.h->fused_dot(x.nvec) || x.h->dim() == 0)) fused = false;
         if (fused) {
@@ -1404,7 +1408,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
             const u64 d = hs[s]->dim();
             if (d == 0) continue;
             BlockRec rec;
-            rec.setup({{hs[s], 1}}, st);
+            rec.setup({{hs[s], 1}}, st, /*allow_fused=*/mode == 1);
             for (int r = 0; r < R; ++r) {
                 QK_HIP(hipMemcpyAsync(rec.bufU.get(), stream + size_t(r) * d,
                                       sizeof(cplx) * d, hipMemcpyHostToDevice, st));

@@ -108,7 +108,11 @@ struct FTLMPoint {
 };
 
 // ftlm over sectors and a beta grid; out[s*nbeta + b].  mode 0: all
-// (sector, vector) pairs advanced together; mode 1: one vector at a time.
+// (sector, vector) pairs advanced together; mode 1: one vector at a time,
+// alpha from the apply kernel's partial sums where it provides them; mode 2:
+// one vector at a time with alpha from a separate pass over the vector and
+// its image, as the reference's loop takes it (ftlm.cpp:24-53) -- the same
+// arithmetic per vector as mode 0.
 std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
                                   const std::vector<double>& betas, int n_random, int n_steps,
                                   int mode);

@@ -269,3 +269,23 @@ def test_davidson_small_space_dependent_corrections(qk, orc, monkeypatch, block)
           flush=True)
     assert dist <= 1e-8
     assert np.all(np.diff(ev) >= -1e-12)
+
+
+def test_ftlm_batched_equals_plain_dot(qk, lat):
+    """The plain loop with alpha from a separate dot (the reference loop's
+    structure, ftlm.cpp:24-53) performs the batched sweep's arithmetic per
+    vector: identical Z, E, Cv; the plain loop with alpha from the apply
+    kernel's partial sums sums alpha in another order (within 1e-8)."""
+    n = 12
+    os_ = lat.xxz_chain(n, 1.0, 1.5)
+    hs = [qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(n, sector=qk.Sector().set_sz(2 * k - n)),
+                                   qk.SpinHalfSite(), os_) for k in range(n + 1)]
+    betas = 1.0 / (0.05 * 200.0 ** (np.arange(40) / 39.0))
+    Zb, Eb, Cb = qk.ftlm_sweep(hs, betas, 4, 40, batched=True)
+    Zd, Ed, Cd = qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="dot")
+    Zf, Ef, Cf = qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="fused")
+    print(f"[ftlm dot] max |Cv batched - Cv dot| = {np.abs(Cb - Cd).max():.1e}, "
+          f"fused: {np.max(np.abs(Cb - Cf) / np.maximum(np.abs(Cb), 1e-300)):.1e}", flush=True)
+    assert np.array_equal(Zb, Zd) and np.array_equal(Eb, Ed) and np.array_equal(Cb, Cd)
+    for a, b in ((Zb, Zf), (Eb, Ef), (Cb, Cf)):
+        assert np.all(np.abs(a - b) <= 1e-8 * np.maximum(np.abs(a), 1e-12))
