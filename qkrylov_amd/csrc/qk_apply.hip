@@ -230,14 +230,28 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
             for (int v = 0; v < B; ++v) cfma(acc[v], amp, x[v * ld + b]);
         }
         if constexpr (ROWS) {
-            // high-half groups of the row program: the row's entries (uniform)
+            // high-half groups of the row program: the row's entries
+            // (uniform), NE at a time with all their gathers issued before
+            // the first is used; added in entry order
+            constexpr int NE = B >= 4 ? 2 : 4;
             const u32 vr = q.vrow ? q.vrow[hi] : hi;
-            const u32 e1 = q.rptr[vr + 1];
-            for (u32 e = q.rptr[vr]; e < e1; ++e) {
-                const RowEntry R = q.rent[e];
-                const u64 b = R.base + r;
+            const u32 e0 = q.rptr[vr], e1 = q.rptr[vr + 1];
+            for (u32 e = e0; e < e1; e += NE) {
+                RowEntry R[NE];
+                cplx xv[NE][B];
 #pragma unroll
-                for (int v = 0; v < B; ++v) cfma(acc[v], R.amp, x[v * ld + b]);
+                for (int k = 0; k < NE; ++k) {
+                    const bool on = e + k < e1;
+                    R[k] = on ? q.rent[e + k] : RowEntry{q.rent[e].base, cplx{0.0, 0.0}};
+#pragma unroll
+                    for (int v = 0; v < B; ++v) xv[k][v] = x[v * ld + R[k].base + r];
+                }
+#pragma unroll
+                for (int k = 0; k < NE; ++k) {
+                    if (e + k >= e1) break;
+#pragma unroll
+                    for (int v = 0; v < B; ++v) cfma(acc[v], R[k].amp, xv[k][v]);
+                }
             }
         }
         // high-half groups: another row, same column
@@ -276,22 +290,44 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
         if constexpr (ROWS) {
             if (q.ment) {
                 // groups across the split: the row's entries (uniform), each
-                // finished per lane on the low half
+                // finished per lane on the low half; NE entries at a time
+                // (records, then every lane's gathers, then the sums, in
+                // entry order).  A lane whose target is not in the basis
+                // reads its own state and adds zero.
+                constexpr int NE = B >= 4 ? 2 : 4;
                 const u32 vr = q.vrow ? q.vrow[hi] : hi;
-                const u32 e1 = q.mptr[vr + 1];
-                for (u32 e = q.mptr[vr]; e < e1; ++e) {
-                    const RowMix M = q.ment[e];
-                    const DevGroup D = q.mdesc[M.desc_sign >> 1];
-                    cplx amp = s_tab[M.tab + config_of(lo, D)];
-                    if (amp.re == 0.0 && amp.im == 0.0) continue;
-                    const u32 tlo = lo ^ u32(D.flip);
-                    const u16 rl = s_jlo[tlo];
-                    if (rl == 0xFFFF) continue;
-                    if (__popc(tlo & u32(p.m1lo)) * p.ncls2 + __popc(tlo & u32(p.m2lo)) != M.cls) continue;
-                    if ((__popc(lo & u32(D.jw)) ^ M.desc_sign) & 1) { amp.re = -amp.re; amp.im = -amp.im; }
-                    const u64 b = M.base + rl;
+                const u32 e0 = q.mptr[vr], e1 = q.mptr[vr + 1];
+                for (u32 e = e0; e < e1; e += NE) {
+                    cplx amp[NE];
+                    u64 tb[NE];
 #pragma unroll
-                    for (int v = 0; v < B; ++v) cfma(acc[v], amp, x[v * ld + b]);
+                    for (int k = 0; k < NE; ++k) {
+                        amp[k] = cplx{0.0, 0.0};
+                        tb[k] = a;
+                        if (e + k >= e1) continue;
+                        const RowMix M = q.ment[e + k];
+                        const DevGroup D = q.mdesc[M.desc_sign >> 1];
+                        cplx am = s_tab[M.tab + config_of(lo, D)];
+                        const u32 tlo = lo ^ u32(D.flip);
+                        const u16 rl = s_jlo[tlo];
+                        const bool ok = (am.re != 0.0 || am.im != 0.0) && rl != 0xFFFF &&
+                                        __popc(tlo & u32(p.m1lo)) * p.ncls2 + __popc(tlo & u32(p.m2lo)) == M.cls;
+                        if (!ok) continue;
+                        if ((__popc(lo & u32(D.jw)) ^ M.desc_sign) & 1) { am.re = -am.re; am.im = -am.im; }
+                        amp[k] = am;
+                        tb[k] = M.base + rl;
+                    }
+                    cplx xv[NE][B];
+#pragma unroll
+                    for (int k = 0; k < NE; ++k)
+#pragma unroll
+                        for (int v = 0; v < B; ++v) xv[k][v] = x[v * ld + tb[k]];
+#pragma unroll
+                    for (int k = 0; k < NE; ++k) {
+                        if (amp[k].re == 0.0 && amp[k].im == 0.0) continue;
+#pragma unroll
+                        for (int v = 0; v < B; ++v) cfma(acc[v], amp[k], xv[k][v]);
+                    }
                 }
             }
         }
