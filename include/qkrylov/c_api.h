@@ -327,14 +327,16 @@ int qkrylov_davidson_lowest_f64(qkrylov_hamiltonian_h h, int n_eig, int max_subs
 
 /* ---- Learned Davidson preconditioner.  No counterpart in the reference,
  * whose Davidson corrects with the Jacobi diagonal (davidson.cpp:166-177).
- * With J_i = 1/(H_ii - theta) (clamped as the reference does), z = J r and
- * w = J (H z - diag(H) z), the correction is t_i = s1_i z_i + s2_i w_i with
- * s1 = 1 + 0.5 tanh(a), s2 = -0.5 + 0.25 tanh(b), where (a_i, b_i) is the
- * output of a 3-layer MLP (32 -> 128 -> 128 -> 2, ReLU, BF16 weights, FP32
- * accumulation on MFMA) on per-state features (diagonal, theta,
- * connectivity, local occupations).  An untrained network (last layer 0)
- * gives the damped first-order Neumann correction t = z - w/2.  Each
- * correction costs one extra apply of H. ---- */
+ * With J_i = 1/(H_ii - theta) (clamped as the reference does), z = J r,
+ * w = J (H z - diag(H) z) and w2 = J (H w - diag(H) w), the correction is
+ * t_i = z_i + s1_i w_i + s2_i w2_i with s1 = -0.5 + 0.5 tanh(a),
+ * s2 = 0.5 tanh(b), where (a_i, b_i) is the output of a 3-layer MLP
+ * (32 -> 128 -> 128 -> 2, ReLU, BF16 weights, FP32 accumulation on MFMA) on
+ * per-state features (diagonal, theta, connectivity, local occupations).
+ * (s1, s2) = (-1, 1) would be the second-order Neumann expansion of
+ * (H - theta)^-1 r; an untrained network (last layer 0) gives the damped
+ * first-order correction t = z - w/2.  Each correction costs two extra
+ * applies of H. ---- */
 typedef struct qkrylov_mlp_t* qkrylov_mlp_h;
 
 typedef struct {
@@ -364,21 +366,23 @@ int qkrylov_mlp_eval(qkrylov_mlp_h m, const float* features, uint64_t n, const f
  * values the network reads). */
 int qkrylov_mlp_features(qkrylov_hamiltonian_h h, double theta, float* features_out);
 /* The learned correction t of a host residual r (complex, dim) at theta, as
- * Davidson applies it; z_out / w_out (may be NULL) receive the two terms. */
+ * Davidson applies it; z_out / w_out / w2_out (may be NULL) receive the
+ * three terms. */
 int qkrylov_mlp_correction_f64(qkrylov_mlp_h m, qkrylov_hamiltonian_h h, const double* r_complex,
                                double theta, double* t_complex_out, double* z_complex_out,
-                               double* w_complex_out);
+                               double* w_complex_out, double* w2_complex_out);
 /* Training loss of npairs pairs on h's basis (host buffers, pair-major, dim
- * states each): o (npairs*dim x 2 floats, network output), z, w, b, u
+ * states each): o (npairs*dim x 2 floats, network output), z, w, w2, b, u
  * (complex), theta (npairs doubles).  With K = P_u (H - theta) P_u,
- * P_u = 1 - u u^H (u unit) and t = P_u (s1 z + s2 w):
+ * P_u = 1 - u u^H (u unit) and t = P_u (z + s1 w + s2 w2):
  * loss = sum over pairs of 1 - |<b, K t>|^2 / |K t|^2, b = K t* / |K t*|
  * for the accurate correction t*; grad_o (npairs*dim x 2, may be NULL) =
  * d(loss / npairs) / d(o). */
 int qkrylov_mlp_pair_loss_f64(qkrylov_hamiltonian_h h, int npairs, const float* o,
                               const double* z_complex, const double* w_complex,
-                              const double* b_complex, const double* u_complex, const double* theta,
-                              float* grad_o, double* loss_out);
+                              const double* w2_complex, const double* b_complex,
+                              const double* u_complex, const double* theta, float* grad_o,
+                              double* loss_out);
 /* Accurate Jacobi-Davidson correction: MINRES on (I-uu^H)(H-theta)(I-uu^H) t
  * = -r, t orthogonal to u (host complex vectors of dim). */
 int qkrylov_jd_correction_f64(qkrylov_hamiltonian_h h, const double* r_complex,

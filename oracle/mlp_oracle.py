@@ -11,17 +11,19 @@ Pinned by: float32->bfloat16 rounding against torch's conversion; the exact
 (float64, unrounded) backward against central finite differences of the
 exact forward; the pair-loss gradient against finite differences and the
 loss at its two ends (0 for t along t*, 1 for K t orthogonal to K t*); the
-correction terms against a dense solve (z - w is the first-order Neumann
-expansion of (H - theta)^-1 r, so its error falls as the square of the
-off-diagonal scale); the network with a zero output layer giving z - w/2.
+correction terms against a dense solve (z - w and z - w + w2 are the first-
+and second-order Neumann expansions of (H - theta)^-1 r, so their errors
+fall as the second and third power of the off-diagonal scale); the network
+with a zero output layer giving z - w/2.
 
 Network: x (32) -> relu(x W1 + b1) (128) -> relu(h1 W2 + b2) (128) -> h2 W3 + b3 (2).
 Parameter block (float32): W1[32][128], b1[128], W2[128][128], b2[128],
 W3[128][2], b3[2], row index = input.
 
-Correction: z = J r, w = J (H z - diag z), J = 1/(H_ii - theta) clamped as the
-reference clamps it; t = s1 z + s2 w, s1 = 1 + tanh(a)/2, s2 = -1/2 + tanh(b)/4,
-(a, b) = network output per state.
+Correction: z = J r, w = J (H z - diag z), w2 = J (H w - diag w), J =
+1/(H_ii - theta) clamped as the reference clamps it; t = z + s1 w + s2 w2,
+s1 = -1/2 + tanh(a)/2, s2 = tanh(b)/2, (a, b) = network output per state
+(a zero output gives the damped first-order form z - w/2).
 """
 from __future__ import annotations
 
@@ -145,34 +147,36 @@ def occupations(states: np.ndarray, nsites: int, two_bit: bool) -> np.ndarray:
 
 
 def neumann_terms(diag, theta, r, apply_h):
-    """z = J r and w = J (H z - diag z); apply_h(x) = H x."""
+    """z = J r, w = J (H z - diag z) and w2 = J (H w - diag w); apply_h(x) = H x."""
+    d = np.asarray(diag, dtype=np.float64)
     J = 1.0 / jacobi_den(diag, theta)
     z = J * r
-    w = J * (apply_h(z) - np.asarray(diag, dtype=np.float64) * z)
-    return z, w
+    w = J * (apply_h(z) - d * z)
+    w2 = J * (apply_h(w) - d * w)
+    return z, w, w2
 
 
 def coefficients(o):
     """(s1, s2) per state from the network output o (n x 2)."""
     o = np.asarray(o, dtype=np.float64).reshape(-1, OUT)
-    return 1.0 + 0.5 * np.tanh(o[:, 0]), -0.5 + 0.25 * np.tanh(o[:, 1])
+    return -0.5 + 0.5 * np.tanh(o[:, 0]), 0.5 * np.tanh(o[:, 1])
 
 
-def combine(o, z, w):
+def combine(o, z, w, w2):
     s1, s2 = coefficients(o)
-    return s1 * z + s2 * w
+    return z + s1 * w + s2 * w2
 
 
-def pair_loss(o, z, w, b, u, theta, apply_h):
+def pair_loss(o, z, w, w2, b, u, theta, apply_h):
     """Training loss of one pair and dL/do (n x 2).  With P = 1 - u u^H (|u| = 1),
-    K = P (H - theta) P and t = P (s1 z + s2 w):
+    K = P (H - theta) P and t = P (z + s1 w + s2 w2):
         L = 1 - |<b, K t>|^2 / (|K t|^2 |b|^2),
     b = K t* for the accurate correction t*.  apply_h(x) = H x."""
     o = np.asarray(o, dtype=np.float64).reshape(-1, OUT)
     ta, tb = np.tanh(o[:, 0]), np.tanh(o[:, 1])
     P = lambda x: x - u * np.vdot(u, x)
     K = lambda x: P(apply_h(x) - theta * x)
-    t = P(combine(o, z, w))
+    t = P(combine(o, z, w, w2))
     q = K(t)
     a = np.vdot(b, q)
     Q = np.vdot(q, q).real
@@ -180,6 +184,6 @@ def pair_loss(o, z, w, b, u, theta, apply_h):
     L = 1.0 - abs(a) ** 2 / (Q * Bn)
     g = -2.0 * a * b / (Q * Bn) + 2.0 * abs(a) ** 2 * q / (Q * Q * Bn)
     h = K(P(g))
-    grad = np.stack([(np.conj(h) * z).real * 0.5 * (1.0 - ta * ta),
-                     (np.conj(h) * w).real * 0.25 * (1.0 - tb * tb)], axis=1)
+    grad = np.stack([(np.conj(h) * w).real * 0.5 * (1.0 - ta * ta),
+                     (np.conj(h) * w2).real * 0.5 * (1.0 - tb * tb)], axis=1)
     return L, grad

@@ -753,12 +753,14 @@ def davidson_lowest(H: MatrixFreeHamiltonian, n_eig: int = 1, max_subspace: int 
 
 
 class MLPPreconditioner:
-    """Learned correction for Davidson: t_i = s1_i z_i + s2_i w_i.
+    """Learned correction for Davidson: t_i = z_i + s1_i w_i + s2_i w2_i.
 
-    z = J r and w = J (H z - diag(H) z) with J_i = 1/(H_ii - theta); the network
-    (32 -> 128 -> 128 -> 2, ReLU, BF16 weights, FP32 accumulation on MFMA) gives
-    (a_i, b_i) and s1 = 1 + 0.5 tanh(a), s2 = -0.5 + 0.25 tanh(b).  It starts with
-    a zero output layer, i.e. as the damped Neumann correction z - w/2.  There is
+    z = J r, w = J (H z - diag(H) z) and w2 = J (H w - diag(H) w) with
+    J_i = 1/(H_ii - theta); the network (32 -> 128 -> 128 -> 2, ReLU, BF16
+    weights, FP32 accumulation on MFMA) gives (a_i, b_i) and
+    s1 = -0.5 + 0.5 tanh(a), s2 = 0.5 tanh(b); (s1, s2) = (-1, 1) would be the
+    second-order Neumann expansion of (H - theta)^-1 r.  It starts with a zero
+    output layer, i.e. as the damped first-order correction z - w/2.  There is
     no counterpart in the reference, whose Davidson uses the Jacobi diagonal
     (src/solvers/davidson.cpp:166-177)."""
 
@@ -831,31 +833,33 @@ def mlp_features(H: MatrixFreeHamiltonian, theta: float) -> np.ndarray:
 
 
 def mlp_correction(net: MLPPreconditioner, H: MatrixFreeHamiltonian, r: np.ndarray, theta: float):
-    """The learned correction (t, z, w) of residual r at Ritz value theta."""
+    """The learned correction (t, z, w, w2) of residual r at Ritz value theta."""
     r = np.ascontiguousarray(r, dtype=np.complex128)
     if r.size != H.dimension:
         raise ValueError("residual length must equal the dimension")
-    t, z, w = np.empty_like(r), np.empty_like(r), np.empty_like(r)
+    t, z, w, w2 = (np.empty_like(r) for _ in range(4))
     check(load().qkrylov_mlp_correction_f64(net._h, H._h, _ptr(r), float(theta), _ptr(t), _ptr(z),
-                                            _ptr(w)), "mlp correction")
-    return t, z, w
+                                            _ptr(w), _ptr(w2)), "mlp correction")
+    return t, z, w, w2
 
 
 def mlp_pair_loss(H: MatrixFreeHamiltonian, o: np.ndarray, z: np.ndarray, w: np.ndarray,
-                  b: np.ndarray, u: np.ndarray, theta, grad: bool = True):
-    """Summed training loss of pairs on H's basis (rows of z, w, b, u; o is
+                  w2: np.ndarray, b: np.ndarray, u: np.ndarray, theta, grad: bool = True):
+    """Summed training loss of pairs on H's basis (rows of z, w, w2, b, u; o is
     (pairs, n, 2); theta one value per pair) and, with grad, its gradient
     d(loss / pairs)/d(o)."""
     n = H.dimension
     z = np.ascontiguousarray(np.reshape(z, (-1, n)), dtype=np.complex128)
     P = z.shape[0]
-    w, b, u = (np.ascontiguousarray(np.reshape(a, (P, n)), dtype=np.complex128) for a in (w, b, u))
+    w, w2, b, u = (np.ascontiguousarray(np.reshape(a, (P, n)), dtype=np.complex128)
+                   for a in (w, w2, b, u))
     o = np.ascontiguousarray(np.reshape(o, (P, n, 2)), dtype=np.float32)
     th = np.ascontiguousarray(np.reshape(theta, (P,)), dtype=np.float64)
     g = np.empty_like(o) if grad else None
     L = ct.c_double()
-    check(load().qkrylov_mlp_pair_loss_f64(H._h, P, _ptr(o), _ptr(z), _ptr(w), _ptr(b), _ptr(u),
-                                           _ptr(th), None if g is None else _ptr(g), ct.byref(L)),
+    check(load().qkrylov_mlp_pair_loss_f64(H._h, P, _ptr(o), _ptr(z), _ptr(w), _ptr(w2), _ptr(b),
+                                           _ptr(u), _ptr(th), None if g is None else _ptr(g),
+                                           ct.byref(L)),
           "mlp pair loss")
     return (L.value, g) if grad else L.value
 

@@ -171,9 +171,9 @@ SIGNATURES = {
                                                   c_void_p]),
     "qkrylov_hermitian_eigh_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p]),
     "qkrylov_mlp_correction_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_void_p,
-                                           c_void_p, c_void_p]),
+                                           c_void_p, c_void_p, c_void_p]),
     "qkrylov_mlp_pair_loss_f64": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
-                                          c_void_p, c_void_p, c_void_p, c_void_p]),
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "qkrylov_jd_correction_f64": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_double,
                                           c_void_p, P(c_double)]),
     "qkrylov_mlp_train_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int,

@@ -692,7 +692,8 @@ int qkrylov_mlp_features(qkrylov_hamiltonian_h h, double theta, float* out) {
     });
 }
 int qkrylov_mlp_correction_f64(qkrylov_mlp_h m, qkrylov_hamiltonian_h h, const double* r,
-                               double theta, double* t_out, double* z_out, double* w_out) {
+                               double theta, double* t_out, double* z_out, double* w_out,
+                               double* w2_out) {
     if (!m || !m->ptr || !hok(h) || !r || !t_out) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {
         const Hamiltonian& H = *h->ptr;
@@ -707,13 +708,14 @@ int qkrylov_mlp_correction_f64(qkrylov_mlp_h m, qkrylov_hamiltonian_h h, const d
         QK_HIP(hipMemcpyAsync(t_out, td, sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
         if (z_out) QK_HIP(hipMemcpyAsync(z_out, pre.z(), sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
         if (w_out) QK_HIP(hipMemcpyAsync(w_out, pre.w(), sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
+        if (w2_out) QK_HIP(hipMemcpyAsync(w2_out, pre.w2(), sizeof(cplx) * d, hipMemcpyDeviceToHost, H.stream));
         QK_HIP(hipStreamSynchronize(H.stream));
     });
 }
 int qkrylov_mlp_pair_loss_f64(qkrylov_hamiltonian_h h, int npairs, const float* o, const double* z,
-                              const double* w, const double* b, const double* u, const double* theta,
-                              float* grad_o, double* loss) {
-    if (!hok(h) || !o || !z || !w || !b || !u || !theta || !loss || npairs <= 0)
+                              const double* w, const double* w2, const double* b, const double* u,
+                              const double* theta, float* grad_o, double* loss) {
+    if (!hok(h) || !o || !z || !w || !w2 || !b || !u || !theta || !loss || npairs <= 0)
         return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {
         const Hamiltonian& H = *h->ptr;
@@ -724,16 +726,16 @@ int qkrylov_mlp_pair_loss_f64(qkrylov_hamiltonian_h h, int npairs, const float* 
             *loss = 0.0;
             return;
         }
-        DevBuf vb(sizeof(cplx) * tot * 4), ob(sizeof(float) * tot * kMlpOut),
+        DevBuf vb(sizeof(cplx) * tot * 5), ob(sizeof(float) * tot * kMlpOut),
             gb(sizeof(float) * tot * kMlpOut);
         cplx* zd = vb.get<cplx>();
-        const double* src[4] = {z, w, b, u};
-        for (int q = 0; q < 4; ++q)
+        const double* src[5] = {z, w, w2, b, u};
+        for (int q = 0; q < 5; ++q)
             QK_HIP(hipMemcpyAsync(zd + u64(q) * tot, src[q], sizeof(cplx) * tot, hipMemcpyHostToDevice, st));
         QK_HIP(hipMemcpyAsync(ob.get(), o, sizeof(float) * tot * kMlpOut, hipMemcpyHostToDevice, st));
         MlpLossWork wk;
-        *loss = mlp_pair_loss(H, zd, zd + tot, zd + 2 * tot, zd + 3 * tot, theta, npairs, ob.get<float>(),
-                              grad_o ? gb.get<float>() : nullptr, wk, st);
+        *loss = mlp_pair_loss(H, zd, zd + tot, zd + 2 * tot, zd + 3 * tot, zd + 4 * tot, theta, npairs,
+                              ob.get<float>(), grad_o ? gb.get<float>() : nullptr, wk, st);
         if (grad_o)
             QK_HIP(hipMemcpyAsync(grad_o, gb.get(), sizeof(float) * tot * kMlpOut, hipMemcpyDeviceToHost, st));
         QK_HIP(hipStreamSynchronize(st));

@@ -172,13 +172,13 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
 
     // 2. accurate corrections t*, stored as the unit vector K t* (the loss
     // target, K = P_u (H - theta) P_u): K t* = -r for the exact solution, or
-    // K applied to a MINRES solution when cfg.solve_iters > 0; and the two
-    // correction terms
-    // z = J r, w = J (H z - diag z) of every pair; everything pair-major
+    // K applied to a MINRES solution when cfg.solve_iters > 0; and the three
+    // correction terms z = J r, w = J (H z - diag z), w2 = J (H w - diag w)
+    // of every pair; everything pair-major
     const std::shared_ptr<const MlpStates> states_p = mlp_states_of(Hs);
     const MlpStates& states = *states_p;
-    DevBuf Z(sizeof(cplx) * n * P), W(sizeof(cplx) * n * P), T(sizeof(cplx) * n * P),
-        U(sizeof(cplx) * n * P), hz(sizeof(cplx) * n), ts(sizeof(cplx) * n);
+    DevBuf Z(sizeof(cplx) * n * P), W(sizeof(cplx) * n * P), W2(sizeof(cplx) * n * P),
+        T(sizeof(cplx) * n * P), U(sizeof(cplx) * n * P), hz(sizeof(cplx) * n), ts(sizeof(cplx) * n);
     Blas b{st, n};
     double worst = 0.0;
     for (int p = 0; p < P; ++p) {
@@ -199,9 +199,12 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
         }
         const double nt = b.norm(tp);
         if (nt > 0.0) b.scale(tp, 1.0 / nt);
+        cplx* wp = W.get<cplx>() + u64(p) * n;
         mlp_jacobi(states, rec.theta[p], rec.r[p].get<cplx>(), zp, st);
         Hs.apply(zp, hz.get<cplx>(), 1, n, st);
-        mlp_neumann(states, rec.theta[p], hz.get<cplx>(), zp, W.get<cplx>() + u64(p) * n, st);
+        mlp_neumann(states, rec.theta[p], hz.get<cplx>(), zp, wp, st);
+        Hs.apply(wp, hz.get<cplx>(), 1, n, st);
+        mlp_neumann(states, rec.theta[p], hz.get<cplx>(), wp, W2.get<cplx>() + u64(p) * n, st);
         QK_HIP(hipMemcpyAsync(U.get<cplx>() + u64(p) * n, rec.u[p].get(), sizeof(cplx) * n,
                               hipMemcpyDeviceToDevice, st));
     }
@@ -224,7 +227,7 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
             const int cnt = std::min(bp, P - p0);
             net.forward(X.get<u16>() + u64(p0) * n * kMlpIn, n * cnt, o.get<float>(), nullptr, nullptr, st);
             L += mlp_pair_loss(Hs, Z.get<cplx>() + u64(p0) * n, W.get<cplx>() + u64(p0) * n,
-                               T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
+                               W2.get<cplx>() + u64(p0) * n, T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
                                rec.theta.data() + p0, cnt, o.get<float>(), nullptr, wk, st);
         }
         return L / P;
@@ -241,7 +244,7 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
             const u16* Xb = X.get<u16>() + u64(p0) * n * kMlpIn;
             net.forward(Xb, n * cnt, o.get<float>(), h1.get<u16>(), h2.get<u16>(), st);
             (void)mlp_pair_loss(Hs, Z.get<cplx>() + u64(p0) * n, W.get<cplx>() + u64(p0) * n,
-                                T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
+                                W2.get<cplx>() + u64(p0) * n, T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
                                 rec.theta.data() + p0, cnt, o.get<float>(), go.get<float>(), wk, st);
             net.backward(Xb, h1.get<u16>(), h2.get<u16>(), go.get<float>(), n * cnt,
                          grads.get<float>(), st);
@@ -280,8 +283,10 @@ void MlpPreconditioner::apply_many(const Hamiltonian& H, int nt, const cplx* con
     o_.resize(sizeof(float) * rows * nt * kMlpOut);
     z_.resize(sizeof(cplx) * tot);
     w_.resize(sizeof(cplx) * tot);
+    w2_.resize(sizeof(cplx) * tot);
     cplx* z = z_.get<cplx>();
     cplx* w = w_.get<cplx>();
+    cplx* w2 = w2_.get<cplx>();
     // the corrections in batches of up to kMlpBatch, one launch per stage
     for (int j0 = 0; j0 < nt; j0 += kMlpBatch) {
         const int nb = std::min(kMlpBatch, nt - j0);
@@ -295,9 +300,12 @@ void MlpPreconditioner::apply_many(const Hamiltonian& H, int nt, const cplx* con
         }
         cplx* zb = z + u64(j0) * n;
         cplx* wb = w + u64(j0) * n;
+        cplx* w2b = w2 + u64(j0) * n;
         mlp_jacobi_many(states_, b, zb, st);
         H.apply(zb, wb, nb, n, st);   // w holds H z until the Neumann term overwrites it
         mlp_neumann_many(states_, b, wb, zb, wb, st);
+        H.apply(wb, w2b, nb, n, st);
+        mlp_neumann_many(states_, b, w2b, wb, w2b, st);
         u16* Xb = X_.get<u16>() + u64(j0) * rows * kMlpIn;
         float* ob = o_.get<float>() + u64(j0) * rows * kMlpOut;
         mlp_features_many(states_, b, Xb, st);
@@ -305,7 +313,7 @@ void MlpPreconditioner::apply_many(const Hamiltonian& H, int nt, const cplx* con
         // t approximates (H - lambda)^-1 r, which is u itself for an exact
         // Ritz pair: the u component is removed here, before Davidson
         // orthogonalises t against its basis
-        mlp_combine_project_many(states_, b, ob, zb, wb, red_, st);
+        mlp_combine_project_many(states_, b, ob, zb, wb, w2b, red_, st);
     }
 }
 

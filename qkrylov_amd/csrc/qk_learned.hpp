@@ -36,30 +36,31 @@ double jd_correction(const Hamiltonian& H, const cplx* r, const cplx* u, double 
 MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& cfg,
                         std::vector<double>* loss_history);
 
-// t = s1 z + s2 w with z = J r, w = J (H z - diag z) and (s1, s2) from the
-// network (qk_mlp.hpp), then the component along the Ritz vector u removed;
-// one extra apply of H per correction.
+// t = z + s1 w + s2 w2 with z = J r, w = J (H z - diag z), w2 = J (H w -
+// diag w) and (s1, s2) from the network (qk_mlp.hpp), then the component
+// along the Ritz vector u removed; two extra applies of H per correction.
 class MlpPreconditioner : public Preconditioner {
 public:
     MlpPreconditioner(const Mlp& net, const Hamiltonian& H);
     void apply(const Hamiltonian& H, const cplx* r, const cplx* diag, double lambda, const cplx* u,
                cplx* t, hipStream_t st) override;
-    // all corrections of an iteration with one batched apply of H and one
+    // all corrections of an iteration with two batched applies of H and one
     // forward pass over every (correction, state) row
     void apply_many(const Hamiltonian& H, int nt, const cplx* const* r, const cplx* diag,
                     const double* lambda, const cplx* const* u, cplx* const* t,
                     hipStream_t st) override;
     bool wants_ritz() const override { return true; }
-    // the two correction terms of the last apply (device; the first
+    // the three correction terms of the last apply (device; the first
     // correction's after apply_many)
     const cplx* z() const { return z_.get<cplx>(); }
     const cplx* w() const { return w_.get<cplx>(); }
+    const cplx* w2() const { return w2_.get<cplx>(); }
 
 private:
     const Mlp& net_;
     std::shared_ptr<const MlpStates> sp_;   // H's tables (cached on H)
     const MlpStates& states_;
-    DevBuf X_, o_, z_, w_, red_;
+    DevBuf X_, o_, z_, w_, w2_, red_;
 };
 
 }  // namespace qk

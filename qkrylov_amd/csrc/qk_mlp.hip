@@ -460,27 +460,23 @@ __device__ __forceinline__ void mlp_coef(const float* o, u64 k, double& s1, doub
                                          double& tb) {
     ta = tanh(double(o[k * kMlpOut]));
     tb = tanh(double(o[k * kMlpOut + 1]));
-    s1 = 1.0 + 0.5 * ta;
-    s2 = -0.5 + 0.25 * tb;
+    s1 = -0.5 + 0.5 * ta;
+    s2 = 0.5 * tb;
 }
 
+__device__ __forceinline__ cplx mlp_term(double s1, double s2, cplx z, cplx w, cplx w2) {
+    return cplx{z.re + s1 * w.re + s2 * w2.re, z.im + s1 * w.im + s2 * w2.im};
+}
+
+// t = z + s1 w + s2 w2 with the coefficients of output row i
 __global__ void k_mlp_combine(const u64 n, const float* __restrict__ o, const cplx* __restrict__ z,
-                              const cplx* __restrict__ w, cplx* __restrict__ t) {
+                              const cplx* __restrict__ w, const cplx* __restrict__ w2,
+                              cplx* __restrict__ t) {
     const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double s1, s2, ta, tb;
     mlp_coef(o, i, s1, s2, ta, tb);
-    t[i] = cplx{s1 * z[i].re + s2 * w[i].re, s1 * z[i].im + s2 * w[i].im};
-}
-
-__global__ void k_mlp_combine_cls(const u64 n, const float* __restrict__ o, const u32* __restrict__ cls,
-                                  const cplx* __restrict__ z, const cplx* __restrict__ w,
-                                  cplx* __restrict__ t) {
-    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double s1, s2, ta, tb;
-    mlp_coef(o, cls[i], s1, s2, ta, tb);
-    t[i] = cplx{s1 * z[i].re + s2 * w[i].re, s1 * z[i].im + s2 * w[i].im};
+    t[i] = mlp_term(s1, s2, z[i], w[i], w2[i]);
 }
 
 // ---- batched over the corrections of one iteration (grid.y = j) ----------
@@ -516,12 +512,12 @@ __global__ void k_mlp_features_many(const double* __restrict__ diag, const float
     mlp_feature_row(diag[i], stat + i * 5, b.theta[j], inv_sites, X + (u64(j) * rows + i) * kIn);
 }
 
-// t_j = s1 z_j + s2 w_j; partial sums of <u_j, t_j> per workgroup
+// t_j = z_j + s1 w_j + s2 w2_j; partial sums of <u_j, t_j> per workgroup
 constexpr int kCombT = 256;
 __global__ void __launch_bounds__(kCombT)
 k_mlp_combine_many(const u64 n, const float* __restrict__ o, const u64 rows, const u32* __restrict__ cls,
-                   const cplx* __restrict__ z, const cplx* __restrict__ w, const MlpBatch b,
-                   double* __restrict__ part) {
+                   const cplx* __restrict__ z, const cplx* __restrict__ w, const cplx* __restrict__ w2,
+                   const MlpBatch b, double* __restrict__ part) {
     const u64 i = u64(blockIdx.x) * kCombT + threadIdx.x;
     const int j = blockIdx.y;
     double dr = 0.0, di = 0.0;
@@ -529,7 +525,7 @@ k_mlp_combine_many(const u64 n, const float* __restrict__ o, const u64 rows, con
         double s1, s2, ta, tb;
         mlp_coef(o + u64(j) * rows * kMlpOut, cls ? cls[i] : i, s1, s2, ta, tb);
         const u64 k = u64(j) * n + i;
-        const cplx t = cplx{s1 * z[k].re + s2 * w[k].re, s1 * z[k].im + s2 * w[k].im};
+        const cplx t = mlp_term(s1, s2, z[k], w[k], w2[k]);
         b.t[j][i] = t;
         if (b.u[j]) {
             const cplx uu = b.u[j][i];
@@ -598,15 +594,6 @@ __global__ void k_mlp_project_many(const u64 n, const MlpBatch b, const cplx* __
     b.t[j][i] = t;
 }
 
-__global__ void k_mlp_project(const cplx* __restrict__ u, const cplx* __restrict__ c,
-                              cplx* __restrict__ t, const u64 n) {
-    const u64 i = u64(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const cplx cc = *c, uu = u[i];
-    t[i].re -= cc.re * uu.re - cc.im * uu.im;
-    t[i].im -= cc.re * uu.im + cc.im * uu.re;
-}
-
 // ---- training loss ----------------------------------------------------------
 // Per-pair reductions and updates over P pairs of n states stored pair-major
 // (grid.y = pair).
@@ -657,18 +644,18 @@ __global__ void k_pair_lincomb(const cplx* __restrict__ ca, const cplx* a, const
                   x.re * av.im + x.im * av.re + y.re * bv.im + y.im * bv.re};
 }
 
-// go = d(loss / P)/d(a, b) from h = K g:  dL/ds1 = Re(conj(h) z), dL/ds2 = Re(conj(h) w)
-__global__ void k_mlp_grad_coef(const float* __restrict__ o, const cplx* __restrict__ z,
-                                const cplx* __restrict__ w, const cplx* __restrict__ h, const u64 cnt,
+// go = d(loss / P)/d(a, b) from h = K g:  dL/ds1 = Re(conj(h) w), dL/ds2 = Re(conj(h) w2)
+__global__ void k_mlp_grad_coef(const float* __restrict__ o, const cplx* __restrict__ w,
+                                const cplx* __restrict__ w2, const cplx* __restrict__ h, const u64 cnt,
                                 const double invP, float* __restrict__ go) {
     const u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= cnt) return;
     double s1, s2, ta, tb;
     mlp_coef(o, k, s1, s2, ta, tb);
-    const cplx hh = h[k], zz = z[k], ww = w[k];
-    const double g1 = hh.re * zz.re + hh.im * zz.im, g2 = hh.re * ww.re + hh.im * ww.im;
+    const cplx hh = h[k], ww = w[k], vv = w2[k];
+    const double g1 = hh.re * ww.re + hh.im * ww.im, g2 = hh.re * vv.re + hh.im * vv.im;
     go[k * kMlpOut] = float(g1 * 0.5 * (1.0 - ta * ta) * invP);
-    go[k * kMlpOut + 1] = float(g2 * 0.25 * (1.0 - tb * tb) * invP);
+    go[k * kMlpOut + 1] = float(g2 * 0.5 * (1.0 - tb * tb) * invP);
 }
 
 }  // namespace
@@ -905,21 +892,6 @@ void MlpStates::build(const Hamiltonian& H, hipStream_t st) {
     QK_HIP(hipStreamSynchronize(st));
 }
 
-void mlp_features_cls(const MlpStates& s, double theta, u16* X, hipStream_t st) {
-    if (!s.ncls) return;
-    hipLaunchKernelGGL(k_mlp_features, dim3(unsigned((s.ncls + 255) / 256)), dim3(256), 0, st,
-                       s.rdiag.get<double>(), s.rstat.get<float>(), s.ncls, theta, 1.0 / s.nsites, X);
-    QK_HIP(hipGetLastError());
-}
-
-void mlp_combine_cls(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
-                     hipStream_t st) {
-    if (!s.n) return;
-    hipLaunchKernelGGL(k_mlp_combine_cls, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st, s.n, o,
-                       s.cls.get<u32>(), z, w, t);
-    QK_HIP(hipGetLastError());
-}
-
 void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st) {
     if (!s.n) return;
     hipLaunchKernelGGL(k_mlp_features, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st,
@@ -939,14 +911,6 @@ void mlp_neumann(const MlpStates& s, double theta, const cplx* hz, const cplx* z
     if (!s.n) return;
     hipLaunchKernelGGL(k_mlp_neumann, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st,
                        s.diag.get<double>(), s.n, theta, hz, z, w);
-    QK_HIP(hipGetLastError());
-}
-
-void mlp_combine(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
-                 hipStream_t st) {
-    if (!s.n) return;
-    hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((s.n + 255) / 256)), dim3(256), 0, st, s.n, o,
-                       z, w, t);
     QK_HIP(hipGetLastError());
 }
 
@@ -976,7 +940,7 @@ void mlp_features_many(const MlpStates& s, const MlpBatch& b, u16* X, hipStream_
 }
 
 void mlp_combine_project_many(const MlpStates& s, const MlpBatch& b, const float* o, const cplx* z,
-                              const cplx* w, DevBuf& scratch, hipStream_t st) {
+                              const cplx* w, const cplx* w2, DevBuf& scratch, hipStream_t st) {
     if (!s.n || b.nt <= 0) return;
     const u64 rows = s.ncls ? s.ncls : s.n;
     const int nblk = int((s.n + kCombT - 1) / kCombT);
@@ -984,7 +948,7 @@ void mlp_combine_project_many(const MlpStates& s, const MlpBatch& b, const float
     double* part = scratch.get<double>();
     cplx* c = reinterpret_cast<cplx*>(part + 2 * size_t(nblk) * b.nt);
     hipLaunchKernelGGL(k_mlp_combine_many, dim3(nblk, b.nt), dim3(kCombT), 0, st, s.n, o, rows,
-                       s.ncls ? s.cls.get<u32>() : nullptr, z, w, b, part);
+                       s.ncls ? s.cls.get<u32>() : nullptr, z, w, w2, b, part);
     bool any_u = false;
     for (int j = 0; j < b.nt; ++j) any_u = any_u || b.u[j];
     if (any_u) {
@@ -992,12 +956,6 @@ void mlp_combine_project_many(const MlpStates& s, const MlpBatch& b, const float
         hipLaunchKernelGGL(k_mlp_project_many, dim3(unsigned((s.n + 255) / 256), b.nt), dim3(256), 0, st,
                            s.n, b, c);
     }
-    QK_HIP(hipGetLastError());
-}
-
-void mlp_project(const cplx* u, const cplx* c, cplx* t, u64 n, hipStream_t st) {
-    if (!n) return;
-    hipLaunchKernelGGL(k_mlp_project, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, u, c, t, n);
     QK_HIP(hipGetLastError());
 }
 
@@ -1060,9 +1018,9 @@ void pair_k(const Hamiltonian& H, const cplx* u, const double* theta, const cplx
 
 }  // namespace
 
-double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* b, const cplx* u,
-                     const double* theta, int P, const float* o, float* go, MlpLossWork& wk,
-                     hipStream_t st) {
+double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* w2, const cplx* b,
+                     const cplx* u, const double* theta, int P, const float* o, float* go,
+                     MlpLossWork& wk, hipStream_t st) {
     const u64 n = H.dim();
     if (!n || P <= 0) return 0.0;
     const u64 tot = u64(P) * n;
@@ -1072,8 +1030,9 @@ double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const c
     cplx* t = wk.t.get<cplx>();
     cplx* q = wk.q.get<cplx>();
     cplx* g = wk.g.get<cplx>();
-    // t = P_u (s1 z + s2 w), q = K t
-    hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, tot, o, z, w, t);
+    // t = P_u (z + s1 w + s2 w2), q = K t
+    hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, tot, o, z, w,
+                       w2, t);
     QK_HIP(hipGetLastError());
     pair_project(u, t, n, P, wk, st);
     pair_k(H, u, theta, t, q, P, wk, st);
@@ -1095,7 +1054,7 @@ double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const c
     pair_lincomb(ca, b, cq, q, g, n, P, wk, st);
     pair_project(u, g, n, P, wk, st);
     pair_k(H, u, theta, g, t, P, wk, st);   // t = h = K g
-    hipLaunchKernelGGL(k_mlp_grad_coef, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, o, z, w, t,
+    hipLaunchKernelGGL(k_mlp_grad_coef, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, o, w, w2, t,
                        tot, 1.0 / double(P), go);
     QK_HIP(hipGetLastError());
     QK_HIP(hipStreamSynchronize(st));

@@ -1,15 +1,18 @@
 // Learned preconditioner for the Davidson correction equation (H - theta) t = -r.
 //
 // With J_i = 1/(H_ii - theta) (the reference's clamped Jacobi factor),
-// z = J r and w = J (O z), O = H - diag(H) (one extra apply), the correction is
-//     t_i = s1_i z_i + s2_i w_i,
-//     s1_i = 1 + 0.5 tanh(a_i),  s2_i = -0.5 + 0.25 tanh(b_i),
+// O = H - diag(H), z = J r, w = J (O z) and w2 = J (O w) (two extra applies),
+// the correction is
+//     t_i = z_i + s1_i w_i + s2_i w2_i,
+//     s1_i = -0.5 + 0.5 tanh(a_i),  s2_i = 0.5 tanh(b_i),
 // where (a_i, b_i) is the output of a three-layer MLP (32 -> 128 -> 128 -> 2,
 // ReLU) reading per-state features (diagonal element, theta, connectivity,
-// local occupations).  An untrained network (zero output layer) gives
-// t = z - w/2: the first-order Neumann expansion of (D + O - theta)^-1 r,
-// damped by one half; the network moves each state's two coefficients within
-// those bounds.  The component along the Ritz vector u is removed from t.
+// local occupations).  (s1, s2) = (-1, 1) is the second-order Neumann
+// expansion of (D + O - theta)^-1 r = J r - J O J r + J O J O J r - ...; an
+// untrained network (zero output layer) gives (-1/2, 0), t = z - w/2, the
+// first-order expansion damped by one half; the network moves each state's
+// two coefficients within s1 in [-1, 0], s2 in [-1/2, 1/2].  The component
+// along the Ritz vector u is removed from t.
 // Weights are kept in FP32 (master copy, Adam moments) and cast to BF16 for
 // the kernels; every matrix product runs on gfx950 MFMA (BF16 inputs, FP32
 // accumulation).  Training minimises, per (residual, accurate correction)
@@ -137,23 +140,18 @@ struct MlpBatch {
     int nt;
 };
 // z_j = J_j r_j and, after hz_j = H z_j, w_j = J_j (hz_j - diag z_j) (w may
-// be hz); z, w hold the nt vectors contiguously (n each)
+// be hz; the same call on (H w, w) gives w2); z, w hold the nt vectors
+// contiguously (n each)
 void mlp_jacobi_many(const MlpStates& s, const MlpBatch& b, cplx* z, hipStream_t st);
 void mlp_neumann_many(const MlpStates& s, const MlpBatch& b, const cplx* hz, const cplx* z, cplx* w,
                       hipStream_t st);
 // features of every network row of every correction: rows = ncls (classes)
 // or n (states) per correction, correction j at X + j * rows * kIn
 void mlp_features_many(const MlpStates& s, const MlpBatch& b, u16* X, hipStream_t st);
-// t_j = s1 z_j + s2 w_j from output rows j * rows + (class of the state or
-// the state), then t_j -= <u_j, t_j> u_j where u_j is given
+// t_j = z_j + s1 w_j + s2 w2_j from output rows j * rows + (class of the
+// state or the state), then t_j -= <u_j, t_j> u_j where u_j is given
 void mlp_combine_project_many(const MlpStates& s, const MlpBatch& b, const float* o, const cplx* z,
-                              const cplx* w, DevBuf& scratch, hipStream_t st);
-
-// Features of the ncls class representatives for theta (device).
-void mlp_features_cls(const MlpStates& s, double theta, u16* X, hipStream_t st);
-// t = s1 z + s2 w with (s1, s2) from the output row of the state's class
-void mlp_combine_cls(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
-                     hipStream_t st);
+                              const cplx* w, const cplx* w2, DevBuf& scratch, hipStream_t st);
 
 // X[i] = features of state i for theta (device).
 void mlp_features(const MlpStates& s, double theta, u16* X, hipStream_t st);
@@ -162,11 +160,6 @@ void mlp_jacobi(const MlpStates& s, double theta, const cplx* r, cplx* z, hipStr
 // w = J (hz - diag z), hz = H z  (device; w may be hz)
 void mlp_neumann(const MlpStates& s, double theta, const cplx* hz, const cplx* z, cplx* w,
                  hipStream_t st);
-// t = s1(o) z + s2(o) w  (device)
-void mlp_combine(const MlpStates& s, const float* o, const cplx* z, const cplx* w, cplx* t,
-                 hipStream_t st);
-// t -= c u, c a device scalar  (device)
-void mlp_project(const cplx* u, const cplx* c, cplx* t, u64 n, hipStream_t st);
 
 // Work buffers of mlp_pair_loss (grown on demand).
 struct MlpLossWork {
@@ -175,15 +168,15 @@ struct MlpLossWork {
 
 // Training loss of a batch of P pairs of H's basis, rows of pair p at
 // [p*n, (p+1)*n).  With K = P_u (H - theta) P_u (the operator of the
-// correction equation, P_u = 1 - u u^H) and t = P_u (s1 z + s2 w):
+// correction equation, P_u = 1 - u u^H) and t = P_u (z + s1 w + s2 w2):
 //     L_p = 1 - |<b, K t>|^2 / |K t|^2,   b = K t* / |K t*|,
 // the cosine between the learned and the accurate correction t* measured in
-// K's norm (K t* = -r up to the solver's tolerance).  z, w, b, u (unit) are
+// K's norm (K t* = -r up to the solver's tolerance).  z, w, w2, b, u (unit) are
 // complex FP64, theta on the host.  Returns the summed loss; with go
 // non-null also go = d(mean loss)/d(o) (FP32, P*n*2): two batched applies of
 // H per call.
-double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* b, const cplx* u,
-                     const double* theta, int P, const float* o, float* go, MlpLossWork& wk,
-                     hipStream_t st);
+double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* w2, const cplx* b,
+                     const cplx* u, const double* theta, int P, const float* o, float* go,
+                     MlpLossWork& wk, hipStream_t st);
 
 }  // namespace qk

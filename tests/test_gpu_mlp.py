@@ -76,19 +76,20 @@ def test_correction_matches_oracle(qk, orc, lat):
         net = _net(qk, 6)
         r = rng.standard_normal(d) + 1j * rng.standard_normal(d)
         for theta in (float(diag.min()) - 0.7, float(np.median(diag))):
-            t, z, w = qk.mlp_correction(net, H, r, theta)
-            z_ref, w_ref = M.neumann_terms(diag, theta, r, lambda x: A @ x)
+            t, z, w, w2 = qk.mlp_correction(net, H, r, theta)
+            z_ref, w_ref, w2_ref = M.neumann_terms(diag, theta, r, lambda x: A @ x)
             ez = np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref)
             ew = np.linalg.norm(w - w_ref) / np.linalg.norm(w_ref)
+            ew2 = np.linalg.norm(w2 - w2_ref) / np.linalg.norm(w2_ref)
             # the combine on the network's own output is exact arithmetic
             o = net.eval(qk.mlp_features(H, theta))
-            et = np.linalg.norm(t - M.combine(o, z, w)) / np.linalg.norm(t)
+            et = np.linalg.norm(t - M.combine(o, z, w, w2)) / np.linalg.norm(t)
             # and the network output agrees with the oracle's forward pass
             o_ref, _ = M.forward(net.get_params(), qk.mlp_features(H, theta))
-            eo = np.linalg.norm(t - M.combine(o_ref, z_ref, w_ref)) / np.linalg.norm(t)
+            eo = np.linalg.norm(t - M.combine(o_ref, z_ref, w_ref, w2_ref)) / np.linalg.norm(t)
             print(f"[mlp correction] {lx}x{ly} d={d} theta={theta:.3f} z {ez:.1e} w {ew:.1e} "
-                  f"t {et:.1e} t(oracle net) {eo:.1e}", flush=True)
-            assert ez < 1e-12 and ew < 1e-12 and et < 1e-12
+                  f"w2 {ew2:.1e} t {et:.1e} t(oracle net) {eo:.1e}", flush=True)
+            assert ez < 1e-12 and ew < 1e-12 and ew2 < 1e-12 and et < 1e-12
             assert eo < 1e-2
 
 
@@ -99,15 +100,15 @@ def test_pair_loss_matches_oracle(qk, orc, lat):
         rng = np.random.default_rng(12)
         P = 3
         c = lambda: rng.standard_normal((P, n)) + 1j * rng.standard_normal((P, n))
-        z, w, u, b = c(), c(), c(), c()
+        z, w, w2, u, b = c(), c(), c(), c(), c()
         u /= np.linalg.norm(u, axis=1, keepdims=True)
         b /= np.linalg.norm(b, axis=1, keepdims=True)
         theta = A.diagonal().real.min() - np.array([0.3, 1.1, 0.05])
         o = (rng.standard_normal((P, n, 2)) * 0.7).astype(np.float32)
-        L, g = qk.mlp_pair_loss(H, o, z, w, b, u, theta)
+        L, g = qk.mlp_pair_loss(H, o, z, w, w2, b, u, theta)
         Lref, gref = 0.0, []
         for p in range(P):
-            lp, gp = M.pair_loss(o[p], z[p], w[p], b[p], u[p], theta[p], lambda x: A @ x)
+            lp, gp = M.pair_loss(o[p], z[p], w[p], w2[p], b[p], u[p], theta[p], lambda x: A @ x)
             Lref += lp
             gref.append(gp / P)
         gref = np.array(gref)
@@ -115,7 +116,7 @@ def test_pair_loss_matches_oracle(qk, orc, lat):
         print(f"[mlp loss] {lx}x{ly} d={n} L={L:.12f} ref={Lref:.12f} grad max rel={eg:.1e}", flush=True)
         assert abs(L - Lref) <= 1e-10 * P
         assert eg < 1e-6      # the gradient is stored in FP32
-        assert qk.mlp_pair_loss(H, o, z, w, b, u, theta, grad=False) == L
+        assert qk.mlp_pair_loss(H, o, z, w, w2, b, u, theta, grad=False) == L
 
 
 def _spin_chain(qk, lat, n):
@@ -200,7 +201,7 @@ def test_training_lowers_loss_and_davidson_agrees(qk, lat):
     # must take fewer iterations.  The 4x4 target lattice is measured by
     # bench.py.
     from oracle import oracle as orc
-    compared = 0
+    compared = compared_untrained = 0
     for (lx, ly, k), tag in (((2, 4, 3), "2x4"), ((2, 5, 4), "2x5")):
         H = tj(lx, ly, k)
         ob = orc.Basis("tj", lx * ly, {"use_nup": True, "nup": k, "use_ndn": True, "ndn": k})
@@ -224,7 +225,12 @@ def test_training_lowers_loss_and_davidson_agrees(qk, lat):
         if same:
             compared += 1
             assert res["mlp"].iterations < res["jacobi"].iterations, tag
-    assert compared >= 1
+        # training must pay: fewer iterations than the untrained network
+        # (the damped first-order correction z - w/2)
+        if np.allclose(res["mlp"].eigenvalues, res["neumann"].eigenvalues, rtol=0, atol=2e-8):
+            compared_untrained += 1
+            assert res["mlp"].iterations < res["neumann"].iterations, tag
+    assert compared >= 1 and compared_untrained >= 1
 
 
 def test_block_dot_matches_numpy(qk):

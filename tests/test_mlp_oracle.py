@@ -65,16 +65,18 @@ def test_zero_output_layer_is_damped_neumann():
     assert o.shape == (50, 2) and np.all(o == 0.0)
     z = rng.standard_normal(50) + 1j * rng.standard_normal(50)
     w = rng.standard_normal(50) + 1j * rng.standard_normal(50)
-    assert np.array_equal(M.combine(o, z, w), z - 0.5 * w)
+    w2 = rng.standard_normal(50) + 1j * rng.standard_normal(50)
+    assert np.array_equal(M.combine(o, z, w, w2), z - 0.5 * w)
     # the coefficients stay inside their bounds for any output
     s1, s2 = M.coefficients(np.array([[1e30, -1e30], [-1e30, 1e30]]))
-    assert np.allclose(s1, [1.5, 0.5]) and np.allclose(s2, [-0.75, -0.25])
+    assert np.allclose(s1, [0.0, -1.0]) and np.allclose(s2, [-0.5, 0.5])
 
 
 def test_neumann_terms_against_dense_solve():
-    """z - w is the first-order expansion of (H - theta)^-1 r in the
-    off-diagonal part: scaling the off-diagonal by eps leaves an error of
-    order eps^2 (a sign or index slip in w leaves order eps)."""
+    """z - w and z - w + w2 are the first- and second-order expansions of
+    (H - theta)^-1 r in the off-diagonal part: scaling the off-diagonal by
+    eps leaves errors of order eps^2 and eps^3 (a sign or index slip in w
+    or w2 leaves a lower order)."""
     rng = np.random.default_rng(8)
     n = 30
     D = np.diag(rng.uniform(1.0, 3.0, n))
@@ -83,15 +85,18 @@ def test_neumann_terms_against_dense_solve():
     np.fill_diagonal(O, 0.0)
     theta = -1.0
     r = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    errs = []
+    errs, errs2 = [], []
     for eps in (1e-2, 1e-3):
         H = D + eps * O
-        z, w = M.neumann_terms(np.diag(H).real, theta, r, lambda x: H @ x)
+        z, w, w2 = M.neumann_terms(np.diag(H).real, theta, r, lambda x: H @ x)
         exact = np.linalg.solve(H - theta * np.eye(n), r)
         errs.append(np.linalg.norm(z - w - exact) / np.linalg.norm(exact))
+        errs2.append(np.linalg.norm(z - w + w2 - exact) / np.linalg.norm(exact))
         # Jacobi alone is first order
         assert np.linalg.norm(z - exact) / np.linalg.norm(exact) > 10 * errs[-1]
+        assert errs[-1] > 10 * errs2[-1]
     assert errs[1] < errs[0] / 50.0, errs
+    assert errs2[1] < errs2[0] / 500.0, errs2
 
 
 def test_pair_loss_gradient_matches_finite_differences():
@@ -101,32 +106,32 @@ def test_pair_loss_gradient_matches_finite_differences():
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     A = A + A.conj().T
     H = lambda x: A @ x
-    z, w, u, b = c(), c(), c(), c()
+    z, w, w2, u, b = c(), c(), c(), c(), c()
     u /= np.linalg.norm(u)
     theta = -2.5
     o = rng.standard_normal((n, 2)) * 0.5
-    L, g = M.pair_loss(o, z, w, b, u, theta, H)
+    L, g = M.pair_loss(o, z, w, w2, b, u, theta, H)
     assert 0.0 <= L <= 1.0 and g.shape == (n, 2)
     h = 1e-7
     for i in range(n):
         for k in range(2):
             e = np.zeros((n, 2))
             e[i, k] = h
-            fd = (M.pair_loss(o + e, z, w, b, u, theta, H)[0]
-                  - M.pair_loss(o - e, z, w, b, u, theta, H)[0]) / (2 * h)
+            fd = (M.pair_loss(o + e, z, w, w2, b, u, theta, H)[0]
+                  - M.pair_loss(o - e, z, w, w2, b, u, theta, H)[0]) / (2 * h)
             assert abs(fd - g[i, k]) < 1e-6 * max(1.0, abs(fd)), (i, k, fd, g[i, k])
     P = lambda x: x - u * np.vdot(u, x)
     K = lambda x: P(A @ x - theta * x)
     # the loss vanishes when t is the accurate correction up to scale (and a
     # component along u, which the projection removes) ...
-    t = P(M.combine(o, z, w))
-    L0, _ = M.pair_loss(o, z, w, K(3.0 * t), u, theta, H)
+    t = P(M.combine(o, z, w, w2))
+    L0, _ = M.pair_loss(o, z, w, w2, K(3.0 * t), u, theta, H)
     assert L0 < 1e-12
     # ... and is 1 when K t is orthogonal to K t*
     q = K(t)
     b1 = c()
     b1 -= q * np.vdot(q, b1) / np.vdot(q, q)
-    L1, _ = M.pair_loss(o, z, w, b1, u, theta, H)
+    L1, _ = M.pair_loss(o, z, w, w2, b1, u, theta, H)
     assert abs(L1 - 1.0) < 1e-12
     # the accurate correction of a Ritz pair gives b = -r (K t* = -r)
     Q, _ = np.linalg.qr(rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3)))
