@@ -552,13 +552,16 @@ k_apply_xchg_multi(const XParams p, const MParams mp, const cplx* __restrict__ x
 
 // Bond batches of the single-vector exchange kernel.  Every load of a batch
 // (bond records, LDS ranks, the SPL gathers of all NB bonds) is issued before
-// the first one is used, and none of them sits under a lane-dependent branch
-// (a lane without a contribution reads rank 0 of its own row and adds zero),
-// so a wave keeps NB*SPL gathers in flight instead of waiting out one bond at
-// a time.  Contributions are added in bond order, as in the one-bond loop.
+// the first one is used, and a lane without a contribution loads nothing and
+// adds zero, so a wave keeps NB*SPL gathers in flight instead of waiting out
+// one bond at a time.  Contributions are added in bond order, as in the
+// one-bond loop.  SYM: every bond has one amplitude for both directions
+// (a01 == a10, e.g. real exchange couplings), taken straight from the bond
+// record instead of selected per lane; a lane without a contribution adds
+// a * 0.
 
 // low-half bonds: same row, rank from the LDS table
-template <int SPL, int NB>
+template <int SPL, int NB, bool SYM>
 __device__ __forceinline__ void xbonds_lo(const XBond* __restrict__ bonds, const u16* s_jlo,
                                           const cplx* __restrict__ x, const u64 base,
                                           const u32 (&lo)[SPL], const bool (&v)[SPL],
@@ -575,9 +578,10 @@ __device__ __forceinline__ void xbonds_lo(const XBond* __restrict__ bonds, const
         for (int k = 0; k < SPL; ++k) {
             const u32 vi = (lo[k] >> X[u].bi) & 1, vj = (lo[k] >> X[u].bj) & 1;
             ok[u][k] = v[k] && vi != vj;
-            amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
+            if constexpr (SYM) amp[u][k] = X[u].a01;
+            else amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
             const u32 t = s_jlo[lo[k] ^ X[u].mlo];   // lo ^ mlo < 2^L: in the table
-            idx[u][k] = ok[u][k] ? t : 0u;
+            idx[u][k] = SYM ? t : (ok[u][k] ? t : 0u);
         }
     cplx xv[NB][SPL];
 #pragma unroll
@@ -621,7 +625,7 @@ __device__ __forceinline__ void xbonds_hi(const XRowNb* __restrict__ nb, const c
 
 // bonds straddling the split: another row (offset from the per-row record)
 // and another low-half class
-template <int SPL, int NB>
+template <int SPL, int NB, bool SYM>
 __device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, const u64* __restrict__ mnb,
                                            const u16* s_jlo, const u32 hi, const u32 L,
                                            const cplx* __restrict__ x, const u64 base,
@@ -644,9 +648,10 @@ __device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, cons
         for (int k = 0; k < SPL; ++k) {
             const u32 vi = (lo[k] >> X[u].bi) & 1;
             ok[u][k] = v[k] && b2[u] != ~u64(0) && vi != vj;
-            amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
+            if constexpr (SYM) amp[u][k] = X[u].a01;
+            else amp[u][k] = ok[u][k] ? (vi ? X[u].a10 : X[u].a01) : 0.0;
             const u32 t = s_jlo[lo[k] ^ X[u].mlo];
-            idx[u][k] = ok[u][k] ? b2[u] + t : base;
+            idx[u][k] = SYM ? b2[u] + t : (ok[u][k] ? b2[u] + t : base);
         }
     }
     cplx xv[NB][SPL];
@@ -669,8 +674,8 @@ __device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, cons
 // Single-vector exchange kernel with SPL states per lane: a tile spans
 // 64*SPL consecutive ranks of one row and lane l holds ranks r0+l+64k.
 // Bonds are taken NB at a time (xbonds_*), so up to NB*SPL gathers per wave
-// are in flight.
-template <int SPL, int NB>
+// are in flight.  SYM: every low-half and straddling bond has a01 == a10.
+template <int SPL, int NB, bool SYM>
 __global__ void __launch_bounds__(64 * kXWaves)
 k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -720,8 +725,8 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
             acc[k] = cplx{dg * xa[k].re, dg * xa[k].im};
         }
         u32 b = 0;
-        for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB>(p.bonds + b, s_jlo, x, base, lo, v, acc);
-        for (; b < p.n_lo; ++b) xbonds_lo<SPL, 1>(p.bonds + b, s_jlo, x, base, lo, v, acc);
+        for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB, SYM>(p.bonds + b, s_jlo, x, base, lo, v, acc);
+        for (; b < p.n_lo; ++b) xbonds_lo<SPL, 1, SYM>(p.bonds + b, s_jlo, x, base, lo, v, acc);
         const u32 vr = p.vrow ? p.vrow[hi] : hi;   // record slot of this row
         {
             const u32 hc = p.hcnt[vr];
@@ -733,9 +738,9 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
         b = hi_end;
         const u64* mnb = p.mnb + size_t(vr) * p.n_mix;
         for (; b + NB <= mix_end; b += NB)
-            xbonds_mix<SPL, NB>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
+            xbonds_mix<SPL, NB, SYM>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
         for (; b < mix_end; ++b)
-            xbonds_mix<SPL, 1>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
+            xbonds_mix<SPL, 1, SYM>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
 #pragma unroll
         for (int k = 0; k < SPL; ++k)
             if (v[k]) {
@@ -1369,6 +1374,11 @@ void Hamiltonian::setup_exchange() {
     bonds.insert(bonds.end(), mix_b.begin(), mix_b.end());
 
     bonds_host = bonds;
+    xsym = true;
+    for (const std::vector<XBond>* v : {&lo_b, &mix_b})
+        for (const XBond& b : *v) xsym = xsym && b.a01 == b.a10;
+    // QKRYLOV_XSYM=0 keeps the per-lane amplitude selection (the general kernel)
+    if (const char* e = std::getenv("QKRYLOV_XSYM")) xsym = xsym && std::atoi(e) != 0;
     upload(d_xbonds, bonds.data(), bonds.size(), stream);
     upload(d_dlo, dlo.data(), dlo.size(), stream);
     upload(d_dhi, dhi.data(), dhi.size(), stream);
@@ -1509,14 +1519,21 @@ void Hamiltonian::set_single_kernel(int s, int nb) {
         xparams_s.xtiles = d_xtiles_s.get<XTile>();
         QK_HIP(hipStreamSynchronize(stream));
     }
-    const void* table[3][3] = {
-        {(const void*)k_apply_xchg_s<1, 1>, (const void*)k_apply_xchg_s<1, 2>, (const void*)k_apply_xchg_s<1, 4>},
-        {(const void*)k_apply_xchg_s<2, 1>, (const void*)k_apply_xchg_s<2, 2>, (const void*)k_apply_xchg_s<2, 4>},
-        {(const void*)k_apply_xchg_s<4, 1>, (const void*)k_apply_xchg_s<4, 2>, nullptr}};
+    const void* table[2][3][3] = {
+        {{(const void*)k_apply_xchg_s<1, 1, false>, (const void*)k_apply_xchg_s<1, 2, false>,
+          (const void*)k_apply_xchg_s<1, 4, false>},
+         {(const void*)k_apply_xchg_s<2, 1, false>, (const void*)k_apply_xchg_s<2, 2, false>,
+          (const void*)k_apply_xchg_s<2, 4, false>},
+         {(const void*)k_apply_xchg_s<4, 1, false>, (const void*)k_apply_xchg_s<4, 2, false>, nullptr}},
+        {{(const void*)k_apply_xchg_s<1, 1, true>, (const void*)k_apply_xchg_s<1, 2, true>,
+          (const void*)k_apply_xchg_s<1, 4, true>},
+         {(const void*)k_apply_xchg_s<2, 1, true>, (const void*)k_apply_xchg_s<2, 2, true>,
+          (const void*)k_apply_xchg_s<2, 4, true>},
+         {(const void*)k_apply_xchg_s<4, 1, true>, (const void*)k_apply_xchg_s<4, 2, true>, nullptr}}};
     const auto slot = [](int v) { return v == 1 ? 0 : v == 2 ? 1 : 2; };
     spl = s;
     bond_batch = nb;
-    xs_fn = table[slot(s)][slot(nb)];
+    xs_fn = table[xsym ? 1 : 0][slot(s)][slot(nb)];
     if (xlds_bytes > 64 * 1024)
         QK_HIP(hipFuncSetAttribute(xs_fn, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
 }
@@ -1868,7 +1885,8 @@ std::string Hamiltonian::info() const {
        << ", \"lds\": " << lds_bytes << ", \"state32\": " << (state32 ? 1 : 0)
        << ", \"max_support\": " << prog.max_support << ", \"exchange_path\": " << (xchg ? 1 : 0)
        << ", \"xbonds\": [" << xparams.n_lo << ", " << xparams.n_hi << ", " << xparams.n_mix
-       << "], \"diag_mixed\": " << xparams.n_dmix << ", \"states_per_lane\": " << spl
+       << "], \"diag_mixed\": " << xparams.n_dmix << ", \"xsym\": " << (xsym ? 1 : 0)
+       << ", \"states_per_lane\": " << spl
        << ", \"bond_batch\": " << bond_batch << ", \"vec_group\": " << vec_group << ", \"xgrid\": " << xgrid_s
        << ", \"row_path\": " << (rows ? 1 : 0) << ", \"row_single\": " << (rows && rows_1 ? 1 : 0)
        << ", \"row_block\": " << (rows && rows_b ? 1 : 0) << ", \"gen_group\": " << gen_group

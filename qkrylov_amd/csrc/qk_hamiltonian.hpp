@@ -239,6 +239,7 @@ struct Hamiltonian {
     int bond_batch = 1;            // bonds whose loads one wave issues together
     int vec_group = 2;             // vectors per block-kernel launch (exchange path)
     const void* xs_fn = nullptr;   // the single-vector kernel for (spl, bond_batch)
+    bool xsym = false;             // every low-half and straddling bond has a01 == a10
     std::vector<double> dhi_host;  // high-half diagonal per row
     std::vector<double> dlo_host;  // low-half diagonal, lo_list order
     std::vector<XBond> bonds_host;

@@ -49,6 +49,25 @@ def test_apply_matches_oracle(qk, orc, name, path, monkeypatch):
     assert np.linalg.norm(y - y_ref) <= 1e-12 * np.linalg.norm(y_ref) + 1e-300
 
 
+@pytest.mark.parametrize("name", sorted(EXCHANGE_MODELS))
+def test_exchange_symmetric_amplitudes_match_general(qk, orc, name, monkeypatch):
+    """Bonds with one amplitude for both directions take it from the bond
+    record (symmetric kernel); QKRYLOV_XSYM=0 selects it per lane (general
+    kernel).  Both add the same terms in the same order: equal values, and
+    both match the oracle."""
+    print(f"[apply xsym] {name}", flush=True)
+    H, oh, _ = build_pair(orc, name)
+    monkeypatch.setenv("QKRYLOV_XSYM", "0")
+    Hg, _, _ = build_pair(orc, name)
+    monkeypatch.delenv("QKRYLOV_XSYM")
+    assert H.info()["xsym"] == 1 and Hg.info()["xsym"] == 0
+    x = _rand(H.dimension, 17)
+    y, yg = H.apply(x), Hg.apply(x)
+    assert np.array_equal(y, yg)
+    y_ref = oh.apply(x)
+    assert np.abs(y - y_ref).max() <= 1e-12 * max(np.abs(y_ref).max(), 1e-300)
+
+
 @pytest.mark.parametrize("name", sorted(MODELS))
 def test_diagonal_bit_exact(qk, orc, name):
     H, oh, _ = build_pair(orc, name)
