@@ -49,8 +49,10 @@ def log(msg):
 
 
 def traffic_from_profiles(kernel: str, bsize: int):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE and
-    WRITE_SIZE, calibrated on a copy kernel), if a summary is committed."""
+    """Bytes per launch between the L2 caches and memory, measured by
+    rocprofv3 --pmc (L2 read and write requests times their sizes, the sizes
+    checked on a copy kernel), if a summary is committed.  Reads served by
+    the memory-side cache (MALL) are included: an upper bound on HBM bytes."""
     p = ROOT / "profiles" / "hx_heisenberg_n28_pmc.md"
     if not p.exists():
         return None
@@ -383,7 +385,8 @@ def main():
     ap.add_argument("--tj-apply", action="store_true",
                     help="time the t-J 4x4 apply only (one vector and blocks of 8)")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="only copy 1 GiB device to device 5 times (calibrates PMC byte counters)")
+                    help="only copy 1 GiB device to device 5 times (calibrates PMC byte counters), "
+                         "then time copies and a read-only reduction of the same 1 GiB")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -392,7 +395,18 @@ def main():
         for _ in range(5):
             y.copy_(x)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30}))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        rates = {}
+        for name, f, nbytes in (("copy_GBs", lambda: y.copy_(x), 2 << 30),
+                                ("read_GBs", lambda: x.sum(), 1 << 30)):
+            f()
+            e0.record()
+            for _ in range(10):
+                f()
+            e1.record()
+            torch.cuda.synchronize()
+            rates[name] = nbytes / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
+        print(json.dumps({"copy_probe_bytes_read": 1 << 30, "copy_probe_bytes_written": 1 << 30, **rates}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the H·x path runs on one device (sharding across GPUs "
@@ -479,7 +493,9 @@ def main():
                    "kernel": {k: H.info()[k] for k in ("states_per_lane", "bond_batch", "vec_group")}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles("k_apply", 1)},
+                     "traffic": traffic_from_profiles("k_apply", 1),
+                     "traffic_scope": "L2<->memory bytes per apply (MALL hits included; "
+                                      "upper bound on HBM bytes)"},
         "cpu_baseline": cpu,
     }
     line.update(extra)

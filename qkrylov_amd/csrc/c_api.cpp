@@ -727,14 +727,15 @@ int qkrylov_mlp_pair_loss_f64(qkrylov_hamiltonian_h h, int npairs, const float* 
             return;
         }
         DevBuf vb(sizeof(cplx) * tot * 5), ob(sizeof(float) * tot * kMlpOut),
-            gb(sizeof(float) * tot * kMlpOut);
+            gb(sizeof(float) * tot * kMlpOut), tb(sizeof(double) * size_t(npairs));
         cplx* zd = vb.get<cplx>();
         const double* src[5] = {z, w, w2, b, u};
         for (int q = 0; q < 5; ++q)
             QK_HIP(hipMemcpyAsync(zd + u64(q) * tot, src[q], sizeof(cplx) * tot, hipMemcpyHostToDevice, st));
         QK_HIP(hipMemcpyAsync(ob.get(), o, sizeof(float) * tot * kMlpOut, hipMemcpyHostToDevice, st));
+        QK_HIP(hipMemcpyAsync(tb.get(), theta, sizeof(double) * size_t(npairs), hipMemcpyHostToDevice, st));
         MlpLossWork wk;
-        *loss = mlp_pair_loss(H, zd, zd + tot, zd + 2 * tot, zd + 3 * tot, zd + 4 * tot, theta, npairs,
+        *loss = mlp_pair_loss(H, zd, zd + tot, zd + 2 * tot, zd + 3 * tot, zd + 4 * tot, tb.get<double>(), npairs,
                               ob.get<float>(), grad_o ? gb.get<float>() : nullptr, wk, st);
         if (grad_o)
             QK_HIP(hipMemcpyAsync(grad_o, gb.get(), sizeof(float) * tot * kMlpOut, hipMemcpyDeviceToHost, st));

@@ -220,6 +220,9 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
         h2(sizeof(u16) * n * bp * kMlpHid), grads(sizeof(float) * kMlpParams);
     MlpLossWork wk;
     for (int p = 0; p < P; ++p) mlp_features(states, rec.theta[p], X.get<u16>() + u64(p) * n * kMlpIn, st);
+    DevBuf TH(sizeof(double) * P);   // the pairs' Ritz values, for the loss
+    QK_HIP(hipMemcpyAsync(TH.get(), rec.theta.data(), sizeof(double) * P, hipMemcpyHostToDevice, st));
+    const double* th = TH.get<double>();
 
     auto eval_all = [&]() {
         double L = 0.0;
@@ -228,7 +231,7 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
             net.forward(X.get<u16>() + u64(p0) * n * kMlpIn, n * cnt, o.get<float>(), nullptr, nullptr, st);
             L += mlp_pair_loss(Hs, Z.get<cplx>() + u64(p0) * n, W.get<cplx>() + u64(p0) * n,
                                W2.get<cplx>() + u64(p0) * n, T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
-                               rec.theta.data() + p0, cnt, o.get<float>(), nullptr, wk, st);
+                               th + p0, cnt, o.get<float>(), nullptr, wk, st);
         }
         return L / P;
     };
@@ -245,7 +248,7 @@ MlpTrainStats train_mlp(Mlp& net, const Hamiltonian& Hs, const MlpTrainConfig& c
             net.forward(Xb, n * cnt, o.get<float>(), h1.get<u16>(), h2.get<u16>(), st);
             (void)mlp_pair_loss(Hs, Z.get<cplx>() + u64(p0) * n, W.get<cplx>() + u64(p0) * n,
                                 W2.get<cplx>() + u64(p0) * n, T.get<cplx>() + u64(p0) * n, U.get<cplx>() + u64(p0) * n,
-                                rec.theta.data() + p0, cnt, o.get<float>(), go.get<float>(), wk, st);
+                                th + p0, cnt, o.get<float>(), go.get<float>(), wk, st, false);
             net.backward(Xb, h1.get<u16>(), h2.get<u16>(), go.get<float>(), n * cnt,
                          grads.get<float>(), st);
             net.adam_step(grads.get<float>(), float(cfg.lr), st);

@@ -633,15 +633,67 @@ k_pair_dot(const cplx* __restrict__ x, const cplx* __restrict__ y, const u64 n, 
     block_sums<3>(s, part + (size_t(p) * nblk + blockIdx.x) * 3);
 }
 
-// out[k] = ca[p] a[k] + cb[p] b[k]  (out may alias b)
-__global__ void k_pair_lincomb(const cplx* __restrict__ ca, const cplx* a, const cplx* __restrict__ cb,
-                               const cplx* b, cplx* out, const u64 n, const int P) {
+// per pair: xy[p] = sum conj(x) y and (yy non-null) yy[p] = sum |y|^2 from
+// k_pair_dot's partial sums, added in block order
+__global__ void k_pair_dot_fin(const double* __restrict__ part, const int nblk, const int P,
+                               cplx* __restrict__ xy, double* __restrict__ yy) {
+    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= P) return;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int q = 0; q < nblk; ++q) {
+        const double* v = part + (size_t(p) * nblk + q) * 3;
+        a += v[0];
+        b += v[1];
+        c += v[2];
+    }
+    xy[p] = cplx{a, b};
+    if (yy) yy[p] = c;
+}
+
+// x_p -= c_p u_p
+__global__ void k_pair_sub(const cplx* __restrict__ c, const cplx* __restrict__ u, cplx* __restrict__ x,
+                           const u64 n, const int P) {
+    const u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= u64(P) * n) return;
+    const cplx cc = c[k / n], uu = u[k];
+    x[k].re -= cc.re * uu.re - cc.im * uu.im;
+    x[k].im -= cc.re * uu.im + cc.im * uu.re;
+}
+
+// y_p -= theta_p x_p
+__global__ void k_pair_shift(const double* __restrict__ theta, const cplx* __restrict__ x,
+                             cplx* __restrict__ y, const u64 n, const int P) {
+    const u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= u64(P) * n) return;
+    const double th = theta[k / n];
+    y[k].re -= th * x[k].re;
+    y[k].im -= th * x[k].im;
+}
+
+// per pair, from a = <b, q> and Q = |q|^2 (b unit): loss 1 - |a|^2 / Q and the
+// coefficients of its gradient in q, g = ca b + cq q with ca = -2 a / Q,
+// cq = 2 |a|^2 / Q^2
+__global__ void k_pair_loss_coef(const cplx* __restrict__ a, const double* __restrict__ Q, const int P,
+                                 double* __restrict__ loss, cplx* __restrict__ ca, double* __restrict__ cq) {
+    const int p = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (p >= P) return;
+    const double A2 = a[p].re * a[p].re + a[p].im * a[p].im;
+    const double Qs = fmax(Q[p], 1e-300);
+    loss[p] = 1.0 - A2 / Qs;
+    ca[p] = cplx{-2.0 * a[p].re / Qs, -2.0 * a[p].im / Qs};
+    cq[p] = 2.0 * A2 / (Qs * Qs);
+}
+
+// g_p = ca_p b_p + cq_p q_p
+__global__ void k_pair_grad(const cplx* __restrict__ ca, const double* __restrict__ cq,
+                            const cplx* __restrict__ b, const cplx* __restrict__ q, cplx* __restrict__ g,
+                            const u64 n, const int P) {
     const u64 k = u64(blockIdx.x) * blockDim.x + threadIdx.x;
     if (k >= u64(P) * n) return;
     const int p = int(k / n);
-    const cplx x = ca[p], y = cb[p], av = a[k], bv = b[k];
-    out[k] = cplx{x.re * av.re - x.im * av.im + y.re * bv.re - y.im * bv.im,
-                  x.re * av.im + x.im * av.re + y.re * bv.im + y.im * bv.re};
+    const cplx x = ca[p], bv = b[k], qv = q[k];
+    const double y = cq[p];
+    g[k] = cplx{x.re * bv.re - x.im * bv.im + y * qv.re, x.re * bv.im + x.im * bv.re + y * qv.im};
 }
 
 // go = d(loss / P)/d(a, b) from h = K g:  dL/ds1 = Re(conj(h) w), dL/ds2 = Re(conj(h) w2)
@@ -961,66 +1013,67 @@ void mlp_combine_project_many(const MlpStates& s, const MlpBatch& b, const float
 
 namespace {
 
-// the P per-pair sums conj(x) y and |y|^2, on the host
-void pair_dot(const cplx* x, const cplx* y, u64 n, int P, std::vector<cplx>& xy,
-              std::vector<double>* yy, MlpLossWork& wk, hipStream_t st) {
+// Per-pair scalars of mlp_pair_loss, all on the device: theta, the dot
+// products and the loss coefficients of the P pairs.
+struct PairScalars {
+    const double* theta;
+    cplx* xy;
+    double* yy;
+    cplx* ca;
+    double* cq;
+    double* loss;
+};
+
+PairScalars pair_scalars(MlpLossWork& wk, int P) {
+    wk.coef.resize((sizeof(double) * 3 + sizeof(cplx) * 2) * size_t(P));
+    unsigned char* b = static_cast<unsigned char*>(wk.coef.get());
+    PairScalars s;
+    s.xy = reinterpret_cast<cplx*>(b);
+    s.ca = s.xy + P;
+    s.yy = reinterpret_cast<double*>(s.ca + P);
+    s.cq = s.yy + P;
+    s.loss = s.cq + P;
+    s.theta = nullptr;
+    return s;
+}
+
+unsigned pair_grid(u64 n, int P) { return unsigned((u64(P) * n + 255) / 256); }
+
+// xy_p = <x_p, y_p> and (yy non-null) yy_p = |y_p|^2, on the device
+void pair_dot(const cplx* x, const cplx* y, u64 n, int P, cplx* xy, double* yy, MlpLossWork& wk,
+              hipStream_t st) {
     const int nblk = int(std::min<u64>(64, (n + kLossT - 1) / kLossT));
     wk.part.resize(sizeof(double) * size_t(P) * nblk * 3);
     hipLaunchKernelGGL(k_pair_dot, dim3(nblk, P), dim3(kLossT), 0, st, x, y, n, nblk,
                        wk.part.get<double>());
-    std::vector<double> h(size_t(P) * nblk * 3);
-    QK_HIP(hipMemcpyAsync(h.data(), wk.part.get(), sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
-    QK_HIP(hipStreamSynchronize(st));
-    xy.assign(P, cplx{0.0, 0.0});
-    if (yy) yy->assign(P, 0.0);
-    for (int p = 0; p < P; ++p)
-        for (int b = 0; b < nblk; ++b) {
-            const double* v = h.data() + (size_t(p) * nblk + b) * 3;
-            xy[p].re += v[0];
-            xy[p].im += v[1];
-            if (yy) (*yy)[p] += v[2];
-        }
-}
-
-// out_p = ca_p a_p + cb_p b_p with host coefficients
-void pair_lincomb(const std::vector<cplx>& ca, const cplx* a, const std::vector<cplx>& cb, const cplx* b,
-                  cplx* out, u64 n, int P, MlpLossWork& wk, hipStream_t st) {
-    wk.coef.resize(sizeof(cplx) * 2 * size_t(P));
-    std::vector<cplx> c(ca);
-    c.insert(c.end(), cb.begin(), cb.end());
-    QK_HIP(hipMemcpyAsync(wk.coef.get(), c.data(), sizeof(cplx) * c.size(), hipMemcpyHostToDevice, st));
-    const u64 tot = u64(P) * n;
-    hipLaunchKernelGGL(k_pair_lincomb, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st,
-                       wk.coef.get<cplx>(), a, wk.coef.get<cplx>() + P, b, out, n, P);
+    hipLaunchKernelGGL(k_pair_dot_fin, dim3(unsigned((P + 63) / 64)), dim3(64), 0, st,
+                       wk.part.get<double>(), nblk, P, xy, yy);
     QK_HIP(hipGetLastError());
-    // the coefficient buffer is reused by the next call
-    QK_HIP(hipStreamSynchronize(st));
 }
 
 // x_p -= <u_p, x_p> u_p
-void pair_project(const cplx* u, cplx* x, u64 n, int P, MlpLossWork& wk, hipStream_t st) {
-    std::vector<cplx> c;
-    pair_dot(u, x, n, P, c, nullptr, wk, st);
-    for (cplx& v : c) v = cplx{-v.re, -v.im};
-    pair_lincomb(c, u, std::vector<cplx>(P, cplx{1.0, 0.0}), x, x, n, P, wk, st);
+void pair_project(const cplx* u, cplx* x, u64 n, int P, const PairScalars& sc, MlpLossWork& wk,
+                  hipStream_t st) {
+    pair_dot(u, x, n, P, sc.xy, nullptr, wk, st);
+    hipLaunchKernelGGL(k_pair_sub, dim3(pair_grid(n, P)), dim3(256), 0, st, sc.xy, u, x, n, P);
+    QK_HIP(hipGetLastError());
 }
 
 // y = P_u (H - theta) x for x orthogonal to u (P pairs)
-void pair_k(const Hamiltonian& H, const cplx* u, const double* theta, const cplx* x, cplx* y, int P,
+void pair_k(const Hamiltonian& H, const cplx* u, const cplx* x, cplx* y, int P, const PairScalars& sc,
             MlpLossWork& wk, hipStream_t st) {
     const u64 n = H.dim();
     H.apply(x, y, P, n, st);
-    std::vector<cplx> mt(P);
-    for (int p = 0; p < P; ++p) mt[p] = cplx{-theta[p], 0.0};
-    pair_lincomb(mt, x, std::vector<cplx>(P, cplx{1.0, 0.0}), y, y, n, P, wk, st);
-    pair_project(u, y, n, P, wk, st);
+    hipLaunchKernelGGL(k_pair_shift, dim3(pair_grid(n, P)), dim3(256), 0, st, sc.theta, x, y, n, P);
+    QK_HIP(hipGetLastError());
+    pair_project(u, y, n, P, sc, wk, st);
 }
 
 }  // namespace
 
 double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* w2, const cplx* b,
                      const cplx* u, const double* theta, int P, const float* o, float* go,
-                     MlpLossWork& wk, hipStream_t st) {
+                     MlpLossWork& wk, hipStream_t st, bool want_loss) {
     const u64 n = H.dim();
     if (!n || P <= 0) return 0.0;
     const u64 tot = u64(P) * n;
@@ -1030,34 +1083,36 @@ double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const c
     cplx* t = wk.t.get<cplx>();
     cplx* q = wk.q.get<cplx>();
     cplx* g = wk.g.get<cplx>();
+    PairScalars sc = pair_scalars(wk, P);
+    sc.theta = theta;
     // t = P_u (z + s1 w + s2 w2), q = K t
     hipLaunchKernelGGL(k_mlp_combine, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, tot, o, z, w,
                        w2, t);
     QK_HIP(hipGetLastError());
-    pair_project(u, t, n, P, wk, st);
-    pair_k(H, u, theta, t, q, P, wk, st);
-    // a = <b, q>, Q = |q|^2 (b unit)
-    std::vector<cplx> a;
-    std::vector<double> Q;
-    pair_dot(b, q, n, P, a, &Q, wk, st);
-    double loss = 0.0;
-    std::vector<cplx> ca(P), cq(P);
-    for (int p = 0; p < P; ++p) {
-        const double A2 = a[p].re * a[p].re + a[p].im * a[p].im;
-        const double Qs = std::max(Q[p], 1e-300);
-        loss += 1.0 - A2 / Qs;
-        // g = -2 a b / Q + 2 |a|^2 q / Q^2
-        ca[p] = cplx{-2.0 * a[p].re / Qs, -2.0 * a[p].im / Qs};
-        cq[p] = cplx{2.0 * A2 / (Qs * Qs), 0.0};
-    }
-    if (!go) return loss;
-    pair_lincomb(ca, b, cq, q, g, n, P, wk, st);
-    pair_project(u, g, n, P, wk, st);
-    pair_k(H, u, theta, g, t, P, wk, st);   // t = h = K g
-    hipLaunchKernelGGL(k_mlp_grad_coef, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, o, w, w2, t,
-                       tot, 1.0 / double(P), go);
+    pair_project(u, t, n, P, sc, wk, st);
+    pair_k(H, u, t, q, P, sc, wk, st);
+    // a = <b, q>, Q = |q|^2 (b unit); the loss and its coefficients per pair
+    pair_dot(b, q, n, P, sc.xy, sc.yy, wk, st);
+    hipLaunchKernelGGL(k_pair_loss_coef, dim3(unsigned((P + 63) / 64)), dim3(64), 0, st, sc.xy, sc.yy, P,
+                       sc.loss, sc.ca, sc.cq);
     QK_HIP(hipGetLastError());
+    if (go) {
+        // g = d loss / d q, projected; t = h = K g; go from h
+        hipLaunchKernelGGL(k_pair_grad, dim3(pair_grid(n, P)), dim3(256), 0, st, sc.ca, sc.cq, b, q, g, n,
+                           P);
+        QK_HIP(hipGetLastError());
+        pair_project(u, g, n, P, sc, wk, st);
+        pair_k(H, u, g, t, P, sc, wk, st);
+        hipLaunchKernelGGL(k_mlp_grad_coef, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, st, o, w, w2,
+                           t, tot, 1.0 / double(P), go);
+        QK_HIP(hipGetLastError());
+    }
+    if (!want_loss) return 0.0;
+    std::vector<double> L(P);
+    QK_HIP(hipMemcpyAsync(L.data(), sc.loss, sizeof(double) * size_t(P), hipMemcpyDeviceToHost, st));
     QK_HIP(hipStreamSynchronize(st));
+    double loss = 0.0;
+    for (int p = 0; p < P; ++p) loss += L[p];
     return loss;
 }
 

@@ -161,7 +161,8 @@ void mlp_jacobi(const MlpStates& s, double theta, const cplx* r, cplx* z, hipStr
 void mlp_neumann(const MlpStates& s, double theta, const cplx* hz, const cplx* z, cplx* w,
                  hipStream_t st);
 
-// Work buffers of mlp_pair_loss (grown on demand).
+// Work buffers of mlp_pair_loss (grown on demand); coef holds the pairs'
+// scalars (theta, dot products, loss coefficients) on the device.
 struct MlpLossWork {
     DevBuf t, q, g, part, coef;
 };
@@ -172,11 +173,12 @@ struct MlpLossWork {
 //     L_p = 1 - |<b, K t>|^2 / |K t|^2,   b = K t* / |K t*|,
 // the cosine between the learned and the accurate correction t* measured in
 // K's norm (K t* = -r up to the solver's tolerance).  z, w, w2, b, u (unit) are
-// complex FP64, theta on the host.  Returns the summed loss; with go
-// non-null also go = d(mean loss)/d(o) (FP32, P*n*2): two batched applies of
-// H per call.
+// complex FP64, theta (P doubles) on the device too.  Returns the summed
+// loss (want_loss; else 0 and the call does not wait for the device); with
+// go non-null also go = d(mean loss)/d(o) (FP32, P*n*2): two batched applies
+// of H per call.  Every per-pair scalar stays on the device.
 double mlp_pair_loss(const Hamiltonian& H, const cplx* z, const cplx* w, const cplx* w2, const cplx* b,
                      const cplx* u, const double* theta, int P, const float* o, float* go,
-                     MlpLossWork& wk, hipStream_t st);
+                     MlpLossWork& wk, hipStream_t st, bool want_loss = true);
 
 }  // namespace qk
