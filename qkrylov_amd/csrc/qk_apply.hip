@@ -135,10 +135,13 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
     u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ntab) * sizeof(cplx));
     cplx* s_amp = reinterpret_cast<cplx*>(smem + size_t(p.ntab) * sizeof(cplx) +
                                           ((size_t(p.nlo) * sizeof(u16) + 15) / 16) * 16);
+    DevGroup* s_mdesc = reinterpret_cast<DevGroup*>(s_amp + (ROWS ? q.namp : 0u));
 
     for (u32 i = threadIdx.x; i < p.ntab; i += kGBlock) s_tab[i] = p.tables[i];
-    if constexpr (ROWS)
+    if constexpr (ROWS) {
         for (u32 i = threadIdx.x; i < q.namp; i += kGBlock) s_amp[i] = q.amps[i];
+        for (u32 i = threadIdx.x; i < q.nmdesc; i += kGBlock) s_mdesc[i] = q.mdesc[i];
+    }
     if ((p.nlo & 7) == 0) {
         const uint4* src = reinterpret_cast<const uint4*>(p.jlo);
         uint4* dst = reinterpret_cast<uint4*>(s_jlo);
@@ -177,10 +180,30 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
 #pragma unroll
         for (int v = 0; v < B; ++v) acc[v] = cplx{0.0, 0.0};
         cplx dsum{0.0, 0.0};
+        // row-program records: the first batch of the row's high-half and
+        // across-the-split entries is loaded here, so that its latency
+        // overlaps the low-half work; each later batch is loaded while the
+        // gathers of the one before are in flight
+        constexpr int NE = B >= 4 ? 2 : 4;
+        u32 vr = 0, he0 = 0, he1 = 0, me0 = 0, me1 = 0;
+        RowEntry R[NE];
+        RowMix MR[NE];
         if constexpr (ROWS) {
-            const cplx d0 = q.dlo[off + r], d1 = q.dhi[q.vrow ? q.vrow[hi] : hi];
+            vr = q.vrow ? q.vrow[hi] : hi;
+            const cplx d0 = q.dlo[off + r], d1 = q.dhi[vr];
             dsum.re = d0.re + d1.re;
             dsum.im = d0.im + d1.im;
+            he0 = q.rptr[vr];
+            he1 = q.rptr[vr + 1];
+#pragma unroll
+            for (int k = 0; k < NE; ++k)
+                R[k] = he0 + k < he1 ? q.rent[he0 + k] : RowEntry{0, cplx{0.0, 0.0}};
+            if (q.ment) {
+                me0 = q.mptr[vr];
+                me1 = q.mptr[vr + 1];
+#pragma unroll
+                for (int k = 0; k < NE; ++k) MR[k] = me0 + k < me1 ? q.ment[me0 + k] : RowMix{};
+            }
         }
 
         u32 g = 0;
@@ -232,25 +255,27 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
         if constexpr (ROWS) {
             // high-half groups of the row program: the row's entries
             // (uniform), NE at a time with all their gathers issued before
-            // the first is used; added in entry order
-            constexpr int NE = B >= 4 ? 2 : 4;
-            const u32 vr = q.vrow ? q.vrow[hi] : hi;
-            const u32 e0 = q.rptr[vr], e1 = q.rptr[vr + 1];
-            for (u32 e = e0; e < e1; e += NE) {
-                RowEntry R[NE];
+            // the first is used; added in entry order.  A slot past the
+            // row's last entry reads its own state and adds nothing.
+            for (u32 e = he0; e < he1; e += NE) {
                 cplx xv[NE][B];
+                cplx amp[NE];
 #pragma unroll
                 for (int k = 0; k < NE; ++k) {
-                    const bool on = e + k < e1;
-                    R[k] = on ? q.rent[e + k] : RowEntry{q.rent[e].base, cplx{0.0, 0.0}};
+                    const bool on = e + k < he1;
+                    amp[k] = R[k].amp;
+                    const u64 b = on ? R[k].base + r : a;
 #pragma unroll
-                    for (int v = 0; v < B; ++v) xv[k][v] = x[v * ld + R[k].base + r];
+                    for (int v = 0; v < B; ++v) xv[k][v] = x[v * ld + b];
                 }
 #pragma unroll
-                for (int k = 0; k < NE; ++k) {
-                    if (e + k >= e1) break;
+                for (int k = 0; k < NE; ++k)
+                    R[k] = e + NE + k < he1 ? q.rent[e + NE + k] : RowEntry{0, cplx{0.0, 0.0}};
 #pragma unroll
-                    for (int v = 0; v < B; ++v) cfma(acc[v], R[k].amp, xv[k][v]);
+                for (int k = 0; k < NE; ++k) {
+                    if (e + k >= he1) break;
+#pragma unroll
+                    for (int v = 0; v < B; ++v) cfma(acc[v], amp[k], xv[k][v]);
                 }
             }
         }
@@ -294,19 +319,16 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
                 // (records, then every lane's gathers, then the sums, in
                 // entry order).  A lane whose target is not in the basis
                 // reads its own state and adds zero.
-                constexpr int NE = B >= 4 ? 2 : 4;
-                const u32 vr = q.vrow ? q.vrow[hi] : hi;
-                const u32 e0 = q.mptr[vr], e1 = q.mptr[vr + 1];
-                for (u32 e = e0; e < e1; e += NE) {
+                for (u32 e = me0; e < me1; e += NE) {
                     cplx amp[NE];
                     u64 tb[NE];
 #pragma unroll
                     for (int k = 0; k < NE; ++k) {
                         amp[k] = cplx{0.0, 0.0};
                         tb[k] = a;
-                        if (e + k >= e1) continue;
-                        const RowMix M = q.ment[e + k];
-                        const DevGroup D = q.mdesc[M.desc_sign >> 1];
+                        if (e + k >= me1) continue;
+                        const RowMix M = MR[k];
+                        const DevGroup D = s_mdesc[M.desc_sign >> 1];
                         cplx am = s_tab[M.tab + config_of(lo, D)];
                         const u32 tlo = lo ^ u32(D.flip);
                         const u16 rl = s_jlo[tlo];
@@ -322,6 +344,8 @@ k_apply(const ApplyParams p, const ColTables c, const RowParams q, const cplx* _
                     for (int k = 0; k < NE; ++k)
 #pragma unroll
                         for (int v = 0; v < B; ++v) xv[k][v] = x[v * ld + tb[k]];
+#pragma unroll
+                    for (int k = 0; k < NE; ++k) MR[k] = e + NE + k < me1 ? q.ment[e + NE + k] : RowMix{};
 #pragma unroll
                     for (int k = 0; k < NE; ++k) {
                         if (amp[k].re == 0.0 && amp[k].im == 0.0) continue;
@@ -893,7 +917,9 @@ std::vector<DevGroup> pack_groups(const std::vector<Group>& gs, std::vector<cplx
 // Resident persistent grid for a kernel using `lds` bytes of LDS per
 // workgroup: as many workgroups as fit on the chip at once (LDS and the
 // 8-waves-per-SIMD register budget), a multiple of the XCD count.  The tile
-// chunk is a few tiles per resident wave of one XCD.
+// chunk is about a few tiles per resident wave of one XCD, sized so that the
+// chunks split evenly over the 8 XCDs (t-J 4x4: 10,578 tiles in chunks of
+// 2,048 left two XCDs idle; a 184,756-state sector was one chunk, one XCD).
 static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
     const int cus = device_cu_count();
     const int by_lds = (160 * 1024) / std::max(lds, 1);
@@ -907,7 +933,10 @@ static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
     blocks = std::max(kXcd, (blocks + kXcd - 1) / kXcd * kXcd);
     int mult = 2;
     if (const char* m = std::getenv("QKRYLOV_CHUNK_MULT")) mult = std::max(1, std::atoi(m));
-    *chunk = u32(std::max(64, mult * (blocks / kXcd) * waves));
+    const u64 c0 = u64(std::max(64, mult * (blocks / kXcd) * waves));
+    const u64 per_xcd = (ntiles + kXcd - 1) / kXcd;             // tiles per XCD
+    const u64 k = std::max<u64>(1, (per_xcd + c0 - 1) / c0);    // chunks per XCD
+    *chunk = u32(std::max<u64>(1, (per_xcd + k - 1) / k));
     return blocks;
 }
 
@@ -1241,7 +1270,7 @@ void Hamiltonian::setup_rows() {
         mptr[nv] = u32(ment.size());
     }
     const int lds = int(rtab.size() * sizeof(cplx) + ((B.jlo().size() * sizeof(u16) + 15) / 16) * 16 +
-                        amps.size() * sizeof(cplx));
+                        std::max<size_t>(amps.size(), 1) * sizeof(cplx) + mdesc.size() * sizeof(DevGroup));
     if (lds > 160 * 1024) return;
 
     upload(d_rgroups, rdev.data(), rdev.size(), stream);
@@ -1286,6 +1315,7 @@ void Hamiltonian::setup_rows() {
     q.mptr = ment.empty() ? nullptr : d_mptr.get<u32>();
     q.ment = ment.empty() ? nullptr : d_ment.get<RowMix>();
     q.mdesc = ment.empty() ? nullptr : d_mdesc.get<DevGroup>();
+    q.nmdesc = ment.empty() ? 0u : u32(mdesc.size());
     q.rent = d_rent.get<RowEntry>();
     q.namp = u32(amps.size());
     q.ndiag = u32(dlo_g.size() + dhi_g.size());

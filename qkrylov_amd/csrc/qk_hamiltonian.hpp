@@ -171,6 +171,7 @@ struct RowParams {
     const u32* mptr;        // groups across the split, row slot v: ment[mptr[v] .. mptr[v + 1])
     const RowMix* ment;     //   (null: none)
     const DevGroup* mdesc;  // low-half descriptors: flip, jw, support of the low half
+    u32 nmdesc;             //   (copied to LDS)
     u32 namp;
     u32 ndiag;              // diagonal groups folded into dlo and dhi
 };
