@@ -917,9 +917,10 @@ std::vector<DevGroup> pack_groups(const std::vector<Group>& gs, std::vector<cplx
 // Resident persistent grid for a kernel using `lds` bytes of LDS per
 // workgroup: as many workgroups as fit on the chip at once (LDS and the
 // 8-waves-per-SIMD register budget), a multiple of the XCD count.  The tile
-// chunk is about a few tiles per resident wave of one XCD, sized so that the
-// chunks split evenly over the 8 XCDs (t-J 4x4: 10,578 tiles in chunks of
-// 2,048 left two XCDs idle; a 184,756-state sector was one chunk, one XCD).
+// chunk is a few tiles per resident wave of one XCD; when that leaves fewer
+// than 16 chunks per XCD, it is resized so that the chunks split evenly
+// over the 8 XCDs (t-J 4x4: 10,578 tiles in chunks of 2,048 left two XCDs
+// idle; a 184,756-state sector was one chunk, one XCD).
 static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
     const int cus = device_cu_count();
     const int by_lds = (160 * 1024) / std::max(lds, 1);
@@ -936,7 +937,7 @@ static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
     const u64 c0 = u64(std::max(64, mult * (blocks / kXcd) * waves));
     const u64 per_xcd = (ntiles + kXcd - 1) / kXcd;             // tiles per XCD
     const u64 k = std::max<u64>(1, (per_xcd + c0 - 1) / c0);    // chunks per XCD
-    *chunk = u32(std::max<u64>(1, (per_xcd + k - 1) / k));
+    *chunk = u32(k >= 16 ? c0 : std::max<u64>(1, (per_xcd + k - 1) / k));
     return blocks;
 }
 
