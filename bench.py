@@ -140,7 +140,10 @@ def cpu_baseline(lat, n_rows=4_000_000, reps=15):
     return out
 
 
-def bench_ftlm(qk, lat):
+def bench_ftlm(qk, lat, batched_only=False):
+    """FTLM sweep over all 21 Sz sectors of the N=20 XXZ chain: the batched
+    sweep against the plain per-vector loops (batched_only: the batched sweep
+    alone, for profiling its kernels)."""
     n = 20
     os_ = lat.xxz_chain(n, 1.0, 1.5)
     hs = []
@@ -155,6 +158,15 @@ def bench_ftlm(qk, lat):
     # drawn once on the host) and both modes read that same cache
     t = time.perf_counter()
     qk.ftlm_sweep(hs, betas[:1], R, 2, batched=True)
+    if batched_only:
+        log(f"ftlm warm-up (random stream, kernels): {time.perf_counter() - t:.3f} s")
+        t = time.perf_counter()
+        qk.ftlm_sweep(hs, betas, R, M, batched=True)
+        dt = time.perf_counter() - t
+        log(f"ftlm batched: {dt:.3f} s")
+        return {"workload": "XXZ chain N=20 (Jxy=1, Jz=1.5), all 21 Sz sectors, R=64, M=100, "
+                            "200 temperatures; batched sweep only",
+                "samples": nsamp, "samples_per_s_batched": nsamp / dt}
     qk.ftlm_sweep(hs, betas[:1], 1, 2, batched=False)
     qk.ftlm_sweep(hs, betas[:1], 1, 2, batched=False, plain_alpha="dot")
     log(f"ftlm warm-up (random stream, kernels): {time.perf_counter() - t:.3f} s")
@@ -373,6 +385,8 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--apply-only", action="store_true", help="time the b=1 apply only")
     ap.add_argument("--ftlm-only", action="store_true", help="time the FTLM sweep only")
+    ap.add_argument("--ftlm-batched-only", action="store_true",
+                    help="with --ftlm-only: the batched sweep alone (for profiling)")
     ap.add_argument("--davidson-only", action="store_true",
                     help="run the t-J 4x4 Davidson leg only (training included)")
     ap.add_argument("--dav-train-only", action="store_true",
@@ -419,7 +433,7 @@ def main():
 
     log(f"device: {qk.device_info()}")
     if args.ftlm_only:
-        print(json.dumps(bench_ftlm(qk, lat)))
+        print(json.dumps(bench_ftlm(qk, lat, batched_only=args.ftlm_batched_only)))
         return
     if args.davidson_only:
         phase = "train" if args.dav_train_only else ("solve" if args.dav_solve_only else "all")
