@@ -921,11 +921,19 @@ std::vector<DevGroup> pack_groups(const std::vector<Group>& gs, std::vector<cplx
 // than 16 chunks per XCD, it is resized so that the chunks split evenly
 // over the 8 XCDs (t-J 4x4: 10,578 tiles in chunks of 2,048 left two XCDs
 // idle; a 184,756-state sector was one chunk, one XCD).
-static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
+static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves, int resident = 0) {
     const int cus = device_cu_count();
     const int by_lds = (160 * 1024) / std::max(lds, 1);
     const int by_waves = 32 / waves;   // 8 waves per SIMD, 4 SIMDs per CU
     int per_cu = std::max(1, std::min({8, by_lds, by_waves}));
+    // a kernel whose registers let fewer workgroups reside per CU gets that
+    // many when the caller passes it (the single-vector exchange kernel:
+    // N=28's 68-VGPR variant holds one 16-wave workgroup per CU, and two per
+    // CU ran the grid in two phases, 3 % slower).  The row-program and
+    // group-loop kernels, whose tiles differ in cost, keep the second phase:
+    // its workgroups start as others finish (t-J 4x4, 8 vectors: 0.209 ms
+    // with it, 0.257 ms without).
+    if (resident > 0) per_cu = std::min(per_cu, resident);
     // fewer resident workgroups keep fewer tiles in flight per XCD (a smaller
     // L2 working set) at the cost of latency hiding; the autotuner tries it
     if (const char* e = std::getenv("QKRYLOV_WG_PER_CU")) per_cu = std::max(1, std::min(per_cu, std::atoi(e)));
@@ -939,6 +947,20 @@ static int grid_for(int lds, size_t ntiles, u32* chunk, int waves = kWaves) {
     const u64 k = std::max<u64>(1, (per_xcd + c0 - 1) / c0);    // chunks per XCD
     *chunk = u32(k >= 16 ? c0 : std::max<u64>(1, (per_xcd + k - 1) / k));
     return blocks;
+}
+
+// Workgroups of `waves` waves that one CU keeps resident for kernel fn:
+// 512 VGPRs per SIMD lane shared by its waves (at most 8 waves per SIMD, 4
+// SIMDs per CU) and 160 KB of LDS; 0 when the kernel's attributes are not
+// available.
+static int resident_wgs(const void* fn, int lds, int waves) {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, fn) != hipSuccess || a.numRegs <= 0) return 0;
+    const int regs = (a.numRegs + 7) / 8 * 8;   // allocation granule
+    const int per_simd = std::min(8, 512 / regs);
+    const int by_regs = per_simd * 4 / waves;
+    const int by_lds = (160 * 1024) / std::max(lds, 1);
+    return std::max(1, std::min(by_regs, by_lds));
 }
 
 int device_cu_count() {
@@ -1544,7 +1566,6 @@ void Hamiltonian::set_single_kernel(int s, int nb) {
     if (s != spl || !d_xtiles_s.get()) {
         const std::vector<XTile> xts = xtiles_for(64u * u32(s));
         xparams_s = xparams;
-        xgrid_s = grid_for(xlds_bytes, xts.size(), &xparams_s.tiles_per_xcd, kXWaves);
         xparams_s.ntiles = u32(xts.size());
         upload(d_xtiles_s, xts.data(), xts.size(), stream);
         xparams_s.xtiles = d_xtiles_s.get<XTile>();
@@ -1567,6 +1588,9 @@ void Hamiltonian::set_single_kernel(int s, int nb) {
     xs_fn = table[xsym ? 1 : 0][slot(s)][slot(nb)];
     if (xlds_bytes > 64 * 1024)
         QK_HIP(hipFuncSetAttribute(xs_fn, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
+    // the grid: workgroups this variant keeps resident per CU (registers, LDS)
+    xs_res = resident_wgs(xs_fn, xlds_bytes, kXWaves);
+    xgrid_s = grid_for(xlds_bytes, xparams_s.ntiles, &xparams_s.tiles_per_xcd, kXWaves, xs_res);
 }
 
 // Autotuner: times every single-vector variant (states per lane x bond
@@ -1917,6 +1941,7 @@ std::string Hamiltonian::info() const {
        << ", \"max_support\": " << prog.max_support << ", \"exchange_path\": " << (xchg ? 1 : 0)
        << ", \"xbonds\": [" << xparams.n_lo << ", " << xparams.n_hi << ", " << xparams.n_mix
        << "], \"diag_mixed\": " << xparams.n_dmix << ", \"xsym\": " << (xsym ? 1 : 0)
+       << ", \"resident_single\": " << xs_res
        << ", \"states_per_lane\": " << spl
        << ", \"bond_batch\": " << bond_batch << ", \"vec_group\": " << vec_group << ", \"xgrid\": " << xgrid_s
        << ", \"row_path\": " << (rows ? 1 : 0) << ", \"row_single\": " << (rows && rows_1 ? 1 : 0)

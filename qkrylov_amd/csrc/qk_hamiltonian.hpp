@@ -269,6 +269,7 @@ struct Hamiltonian {
     int lds_bytes = 0;
     int dlds_bytes = 0;
     int grid = 0;
+    int xs_res = 0;   // workgroups of the single-vector exchange kernel resident per CU
 
     // scratch for host-buffer entry points
     mutable DevBuf scratch_x, scratch_y;
