@@ -695,17 +695,13 @@ __device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, cons
         }
 }
 
-// Single-vector exchange kernel with SPL states per lane: a tile spans
-// 64*SPL consecutive ranks of one row and lane l holds ranks r0+l+64k.
-// Bonds are taken NB at a time (xbonds_*), so up to NB*SPL gathers per wave
-// are in flight.  SYM: every low-half and straddling bond has a01 == a10.
-template <int SPL, int NB, bool SYM>
-__global__ void __launch_bounds__(64 * kXWaves)
-k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y) {
-    extern __shared__ __align__(16) unsigned char smem[];
+// LDS of the exchange kernels: the diagonal tables of the groups spanning
+// both halves, then the low-half rank table.
+__device__ __forceinline__ void xstage_lds(const XParams& p, unsigned char* smem, cplx*& s_tab,
+                                           u16*& s_jlo) {
     constexpr u32 NT = 64 * kXWaves;
-    cplx* s_tab = reinterpret_cast<cplx*>(smem);
-    u16* s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ndmix_tab) * sizeof(cplx));
+    s_tab = reinterpret_cast<cplx*>(smem);
+    s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ndmix_tab) * sizeof(cplx));
     for (u32 i = threadIdx.x; i < p.ndmix_tab; i += NT) s_tab[i] = p.dmix_tab[i];
     if ((p.nlo & 7) == 0) {
         const uint4* src = reinterpret_cast<const uint4*>(p.jlo);
@@ -715,62 +711,87 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
         for (u32 i = threadIdx.x; i < p.nlo; i += NT) s_jlo[i] = p.jlo[i];
     }
     __syncthreads();
+}
+
+// One tile of the single-vector exchange kernels (lane < T.nvalid): y over
+// the tile's states, x and y at the vector's start, bond records R of the
+// vector's sector.  Adds Re<x, y> over this lane's states to dot.
+template <int SPL, int NB, bool SYM>
+__device__ __forceinline__ void xs_tile(const XParams& p, const XTile& T, const u32 hi,
+                                        const XRecs& R, const cplx* __restrict__ x,
+                                        cplx* __restrict__ y, const cplx* s_tab,
+                                        const u16* s_jlo, const u32 lane, double& dot) {
+    const u32 hi_end = p.n_lo + p.n_hi;
+    const u32 mix_end = hi_end + p.n_mix;
+    const u64 base = T.base;
+    __builtin_assume(lane < T.nvalid);   // the caller skips lanes past the tile
+    u32 lo[SPL], r[SPL];
+    bool v[SPL];
+    cplx acc[SPL], xa[SPL];
+#pragma unroll
+    for (int k = 0; k < SPL; ++k) {
+        const u32 j = lane + 64u * k;
+        v[k] = j < T.nvalid;
+        r[k] = T.r0 + j;
+        lo[k] = v[k] ? p.lo_list[T.pos0 + j] : 0u;
+        double dg = v[k] ? p.dlo[T.pos0 + j] + T.dh : 0.0;
+        if (p.n_dmix) {
+            const u64 s = (u64(hi) << p.L) | lo[k];
+            for (u32 g = 0; g < p.n_dmix; ++g) {
+                const DevGroup G = p.dmix[g];
+                dg += s_tab[G.tab + config_of(s, G)].re;
+            }
+        }
+        xa[k] = cplx{0.0, 0.0};
+        if (v[k]) xa[k] = x[base + r[k]];
+        acc[k] = cplx{dg * xa[k].re, dg * xa[k].im};
+    }
+    u32 b = 0;
+    for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB, SYM>(p.bonds + b, s_jlo, x, base, lo, v, acc);
+    for (; b < p.n_lo; ++b) xbonds_lo<SPL, 1, SYM>(p.bonds + b, s_jlo, x, base, lo, v, acc);
+    const u32 vr = R.vrow ? R.vrow[hi] : hi;   // record slot of this row
+    {
+        const u32 hc = R.hcnt[vr];
+        const XRowNb* nb = R.hnb + size_t(vr) * p.n_hi;
+        u32 e = 0;
+        for (; e + NB <= hc; e += NB) xbonds_hi<SPL, NB>(nb + e, x, r, v, acc);
+        for (; e < hc; ++e) xbonds_hi<SPL, 1>(nb + e, x, r, v, acc);
+    }
+    b = hi_end;
+    const u64* mnb = R.mnb + size_t(vr) * p.n_mix;
+    for (; b + NB <= mix_end; b += NB)
+        xbonds_mix<SPL, NB, SYM>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
+    for (; b < mix_end; ++b)
+        xbonds_mix<SPL, 1, SYM>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
+#pragma unroll
+    for (int k = 0; k < SPL; ++k)
+        if (v[k]) {
+            store_nt(&y[base + r[k]], acc[k]);
+            dot += xa[k].re * acc[k].re + xa[k].im * acc[k].im;
+        }
+}
+
+// Single-vector exchange kernel with SPL states per lane: a tile spans
+// 64*SPL consecutive ranks of one row and lane l holds ranks r0+l+64k.
+// Bonds are taken NB at a time (xbonds_*), so up to NB*SPL gathers per wave
+// are in flight.  SYM: every low-half and straddling bond has a01 == a10.
+template <int SPL, int NB, bool SYM>
+__global__ void __launch_bounds__(64 * kXWaves)
+k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx* s_tab;
+    u16* s_jlo;
+    xstage_lds(p, smem, s_tab, s_jlo);
 
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 lane = threadIdx.x & 63;
-    const u32 hi_end = p.n_lo + p.n_hi;
-    const u32 mix_end = hi_end + p.n_mix;
     double dot = 0.0;   // Re<x, y> over this lane's states (used with dot_part)
 
     QK_FOR_TILES_W(p, wave, t, kXWaves) {
         const XTile T = p.xtiles[t];
         const u32 hi = __builtin_amdgcn_readfirstlane(T.hi);
-        const u64 base = T.base;
         if (lane >= T.nvalid) continue;
-        u32 lo[SPL], r[SPL];
-        bool v[SPL];
-        cplx acc[SPL], xa[SPL];
-#pragma unroll
-        for (int k = 0; k < SPL; ++k) {
-            const u32 j = lane + 64u * k;
-            v[k] = j < T.nvalid;
-            r[k] = T.r0 + j;
-            lo[k] = v[k] ? p.lo_list[T.pos0 + j] : 0u;
-            double dg = v[k] ? p.dlo[T.pos0 + j] + T.dh : 0.0;
-            if (p.n_dmix) {
-                const u64 s = (u64(hi) << p.L) | lo[k];
-                for (u32 g = 0; g < p.n_dmix; ++g) {
-                    const DevGroup G = p.dmix[g];
-                    dg += s_tab[G.tab + config_of(s, G)].re;
-                }
-            }
-            xa[k] = cplx{0.0, 0.0};
-            if (v[k]) xa[k] = x[base + r[k]];
-            acc[k] = cplx{dg * xa[k].re, dg * xa[k].im};
-        }
-        u32 b = 0;
-        for (; b + NB <= p.n_lo; b += NB) xbonds_lo<SPL, NB, SYM>(p.bonds + b, s_jlo, x, base, lo, v, acc);
-        for (; b < p.n_lo; ++b) xbonds_lo<SPL, 1, SYM>(p.bonds + b, s_jlo, x, base, lo, v, acc);
-        const u32 vr = p.vrow ? p.vrow[hi] : hi;   // record slot of this row
-        {
-            const u32 hc = p.hcnt[vr];
-            const XRowNb* nb = p.hnb + size_t(vr) * p.n_hi;
-            u32 e = 0;
-            for (; e + NB <= hc; e += NB) xbonds_hi<SPL, NB>(nb + e, x, r, v, acc);
-            for (; e < hc; ++e) xbonds_hi<SPL, 1>(nb + e, x, r, v, acc);
-        }
-        b = hi_end;
-        const u64* mnb = p.mnb + size_t(vr) * p.n_mix;
-        for (; b + NB <= mix_end; b += NB)
-            xbonds_mix<SPL, NB, SYM>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
-        for (; b < mix_end; ++b)
-            xbonds_mix<SPL, 1, SYM>(p.bonds + b, mnb + (b - hi_end), s_jlo, hi, p.L, x, base, lo, v, acc);
-#pragma unroll
-        for (int k = 0; k < SPL; ++k)
-            if (v[k]) {
-                store_nt(&y[base + r[k]], acc[k]);
-                dot += xa[k].re * acc[k].re + xa[k].im * acc[k].im;
-            }
+        xs_tile<SPL, NB, SYM>(p, T, hi, XRecs{p.hnb, p.hcnt, p.mnb, p.vrow}, x, y, s_tab, s_jlo, lane, dot);
     }
     if (p.dot_part) {
         // this workgroup's Re<x, y>, reduced in a fixed order
@@ -783,6 +804,59 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
             for (int w = 0; w < kXWaves; ++w) tsum += s_dot[w];
             p.dot_part[blockIdx.x] = tsum;
         }
+    }
+}
+
+// Multi-sector single-vector kernel: every (sector, vector) pair is a unit
+// whose tiles all run on one XCD, the units spread over the 8 XCDs by size
+// (the tile list holds each XCD's units in its chunks), so the vector an XCD
+// works on stays in its L2 while the unit runs (the N=20 chain's largest
+// sector: 2.9 MB per vector against 4 MB of L2).  Per tile the same work, in
+// the same order, as k_apply_xchg_s: y is bit-identical to per-vector
+// launches.  With mp.part each tile also writes Re<x, y> over its states,
+// summed in a fixed order, so the recurrence's alpha needs no pass of its
+// own.
+template <int SPL, int NB, bool SYM>
+__global__ void __launch_bounds__(64 * kXWaves)
+k_apply_xchg_ms(const XParams p, const MsParams mp, const uint4* __restrict__ tl,
+                const cplx* __restrict__ x, cplx* __restrict__ y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cplx* s_tab;
+    u16* s_jlo;
+    xstage_lds(p, smem, s_tab, s_jlo);
+
+    const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const u32 lane = threadIdx.x & 63;
+    // The tile list is one chunk per XCD: this wave takes every W-th entry
+    // of its XCD's chunk, and loads the next entry (a scalar load from a
+    // read-only argument) while the current tile runs.
+    const u32 xcd = blockIdx.x % kXcd;
+    const u32 W = (gridDim.x / kXcd) * kXWaves;
+    const u32 t_end = min((xcd + 1) * mp.tiles_per_xcd, mp.ntiles);
+    u32 t = xcd * mp.tiles_per_xcd + (blockIdx.x / kXcd) * kXWaves + wave;
+    if (t >= t_end) return;
+    uint4 e = tl[t];
+    for (;;) {
+        const u32 tn = t + W;
+        const bool more = tn < t_end;
+        const uint4 en = tl[more ? tn : t];
+        const u32 unit = __builtin_amdgcn_readfirstlane(e.y);
+        const u32 tile = __builtin_amdgcn_readfirstlane(e.x);
+        const XTile T = mp.xt[tile];                  // T, U and S loads side by side
+        const MsUnit U = mp.units[unit];
+        const MsSec S = mp.secs[__builtin_amdgcn_readfirstlane(e.z)];
+        const u32 hi = __builtin_amdgcn_readfirstlane(T.hi);
+        double dot = 0.0;
+        if (lane < T.nvalid)
+            xs_tile<SPL, NB, SYM>(p, T, hi, S.rec, x + U.xoff, y + U.xoff, s_tab, s_jlo, lane, dot);
+        if (mp.part && T.nvalid) {                    // padding: an empty tile
+            // the tile's Re<x, y>: every lane, fixed butterfly order
+            for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off);
+            if (lane == 0) mp.part[U.poff + (tile - S.t0)] = dot;
+        }
+        if (!more) break;
+        t = tn;
+        e = en;
     }
 }
 
@@ -1732,6 +1806,17 @@ bool MultiApply::build(const std::vector<Group>& groups, hipStream_t st) {
             std::memcmp(h.dmixtab_host.data(), h0.dmixtab_host.data(), sizeof(cplx) * h.dmixtab_host.size()) != 0)
             return false;
     }
+    h0_ = &h0;
+    // QKRYLOV_MULTI_KERNEL=block: the block kernel, every tile applied to
+    // several vectors of its sector; default: one vector per tile, each
+    // vector's tiles on one XCD
+    const char* mk = std::getenv("QKRYLOV_MULTI_KERNEL");
+    if (!(mk && std::string(mk) == "block") && build_units(gs, st)) {
+        units_ = true;
+        ok_ = true;
+        return true;
+    }
+    units_ = false;
     std::vector<MTile> mt;
     std::vector<const u64*> jh;
     // vblk > 0: one tile per (block of vblk vectors, tile), all tiles of the
@@ -1787,8 +1872,145 @@ bool MultiApply::build(const std::vector<Group>& groups, hipStream_t st) {
     return true;
 }
 
-void MultiApply::launch(const cplx* x, cplx* y, hipStream_t st) const {
+// Units: one (group, vector) pair each, costed by its tiles.  With fewer
+// than 32 units each is cut into pieces of consecutive tiles, so that a
+// single vector still spreads over every XCD.  Each piece goes to the XCD
+// with the least work so far, largest first; an XCD's list holds its pieces
+// one after another and is the XCD's one chunk of the tile schedule (chunk c
+// runs on XCD c % 8), padded with empty entries to the longest list.
+bool MultiApply::build_units(const std::vector<const Group*>& gs, hipStream_t st) {
+    const Hamiltonian& h0 = *h0_;
+    for (const Group* g : gs)
+        if (g->h->xsym != h0.xsym) return false;
+    int s = 2, nb = 4;
+    if (const char* e = std::getenv("QKRYLOV_MULTI_SPL")) s = std::atoi(e);
+    if (const char* e = std::getenv("QKRYLOV_MULTI_NB")) nb = std::atoi(e);
+    if (s != 1 && s != 2) s = 2;
+    if (nb != 1 && nb != 2 && nb != 4) nb = 4;
+    const void* table[2][2][3] = {
+        {{(const void*)k_apply_xchg_ms<1, 1, false>, (const void*)k_apply_xchg_ms<1, 2, false>,
+          (const void*)k_apply_xchg_ms<1, 4, false>},
+         {(const void*)k_apply_xchg_ms<2, 1, false>, (const void*)k_apply_xchg_ms<2, 2, false>,
+          (const void*)k_apply_xchg_ms<2, 4, false>}},
+        {{(const void*)k_apply_xchg_ms<1, 1, true>, (const void*)k_apply_xchg_ms<1, 2, true>,
+          (const void*)k_apply_xchg_ms<1, 4, true>},
+         {(const void*)k_apply_xchg_ms<2, 1, true>, (const void*)k_apply_xchg_ms<2, 2, true>,
+          (const void*)k_apply_xchg_ms<2, 4, true>}}};
+    ms_fn_ = table[h0.xsym ? 1 : 0][s == 1 ? 0 : 1][nb == 1 ? 0 : nb == 2 ? 1 : 2];
+    if (h0.xlds_bytes > 64 * 1024)
+        QK_HIP(hipFuncSetAttribute(ms_fn_, hipFuncAttributeMaxDynamicSharedMemorySize, h0.xlds_bytes));
+
+    // per-sector tiles and records
+    std::vector<XTile> xt;
+    std::vector<size_t> xoff;
+    std::vector<u32> ntile;
+    for (const Group* g : gs) {
+        const std::vector<XTile> t = g->h->xtiles_for(64u * u32(s));
+        xoff.push_back(xt.size());
+        ntile.push_back(u32(t.size()));
+        xt.insert(xt.end(), t.begin(), t.end());
+    }
+    // the last tile is empty: the padding entries of short XCD lists
+    const u32 empty = u32(xt.size());
+    xt.push_back(XTile{});
+    if (xt.size() >= (size_t(1) << 32)) return false;
+    upload(ms_xt_, xt.data(), xt.size(), st);
+    std::vector<MsSec> secs;
+    for (size_t i = 0; i < gs.size(); ++i) {
+        const XParams& q = gs[i]->h->xparams;
+        secs.push_back(MsSec{XRecs{q.hnb, q.hcnt, q.mnb, q.vrow}, u32(xoff[i]), 0u});
+    }
+    std::vector<MsUnit> units;
+    utiles_.clear();
+    u32 maxt = 0;
+    for (size_t i = 0; i < gs.size(); ++i) maxt = std::max(maxt, ntile[i]);
+    for (size_t i = 0; i < gs.size(); ++i)
+        for (int v = 0; v < gs[i]->nvec; ++v) {
+            units.push_back(MsUnit{gs[i]->off + u64(v) * gs[i]->h->dim(), u32(i), 0u});
+            utiles_.push_back(ntile[i]);
+        }
+    // tile partials: unit u at u * maxt (when the offsets fit 32 bits)
+    pstride_ = u64(units.size()) * maxt < (u64(1) << 32) ? maxt : 0u;
+    for (size_t u = 0; u < units.size(); ++u) units[u].poff = u32(u * pstride_);
+
+    // pieces (unit, first tile, end tile) and the XCD lists
+    struct Piece {
+        u32 unit, t0, t1;
+    };
+    const u32 npiece = units.size() >= 4 * kXcd ? 1u : u32((4 * kXcd + units.size() - 1) / units.size());
+    std::vector<Piece> pieces;
+    for (u32 u = 0; u < units.size(); ++u) {
+        const u32 nt = ntile[units[u].sec];
+        const u32 np = std::max(1u, std::min(npiece, nt));
+        for (u32 k = 0; k < np; ++k) pieces.push_back(Piece{u, u32(u64(nt) * k / np), u32(u64(nt) * (k + 1) / np)});
+    }
+    std::stable_sort(pieces.begin(), pieces.end(),
+                     [](const Piece& a, const Piece& b) { return a.t1 - a.t0 > b.t1 - b.t0; });
+    std::vector<std::vector<Piece>> xu(kXcd);
+    std::vector<u64> load(kXcd, 0);
+    for (const Piece& q : pieces) {
+        const int j = int(std::min_element(load.begin(), load.end()) - load.begin());
+        xu[j].push_back(q);
+        load[j] += q.t1 - q.t0;
+    }
+    // grid: the workgroups the kernel keeps resident, or fewer for few tiles
+    // (one wave per tile at most: the plain loop's small sectors)
+    const int per_cu = std::max(1, resident_wgs(ms_fn_, h0.xlds_bytes, kXWaves));
+    const int by_lds = (160 * 1024) / std::max(h0.xlds_bytes, 1);
+    const int cus = device_cu_count();
+    u64 real = 0;
+    for (u32 u = 0; u < units.size(); ++u) real += ntile[units[u].sec];
+    const u64 want = (real + kXWaves - 1) / kXWaves;
+    grid_ = int(std::min<u64>(want, u64(cus) * std::max(1, std::min({per_cu, by_lds, 2}))));
+    grid_ = std::max(kXcd, (grid_ + kXcd - 1) / kXcd * kXcd);
+    std::vector<std::vector<uint4>> lists(kXcd);
+    size_t longest = 0;
+    for (int j = 0; j < kXcd; ++j) {
+        std::sort(xu[j].begin(), xu[j].end(), [](const Piece& a, const Piece& b) {
+            return a.unit != b.unit ? a.unit < b.unit : a.t0 < b.t0;
+        });
+        for (const Piece& q : xu[j]) {
+            const u32 b = u32(xoff[units[q.unit].sec]);
+            for (u32 t = q.t0; t < q.t1; ++t) lists[j].push_back(make_uint4(b + t, q.unit, units[q.unit].sec, 0u));
+        }
+        longest = std::max(longest, lists[j].size());
+    }
+    // one chunk per XCD: the lists back to back, each padded to the longest
+    const size_t chunk = std::max<size_t>(longest, 1);
+    if (chunk * kXcd >= (size_t(1) << 32)) return false;
+    std::vector<uint4> tl;
+    tl.reserve(chunk * kXcd);
+    for (int j = 0; j < kXcd; ++j)
+        for (size_t i = 0; i < chunk; ++i)
+            tl.push_back(i < lists[j].size() ? lists[j][i] : make_uint4(empty, 0u, 0u, 0u));
+    upload(ms_tiles_, tl.data(), tl.size(), st);
+    upload(ms_units_, units.data(), units.size(), st);
+    upload(ms_secs_, secs.data(), secs.size(), st);
+    QK_HIP(hipStreamSynchronize(st));
+    msp_.tiles = ms_tiles_.get<uint4>();
+    msp_.xt = ms_xt_.get<XTile>();
+    msp_.units = ms_units_.get<MsUnit>();
+    msp_.secs = ms_secs_.get<MsSec>();
+    msp_.part = nullptr;
+    msp_.ntiles = u32(tl.size());
+    msp_.tiles_per_xcd = u32(chunk);
+    return true;
+}
+
+void MultiApply::launch(const cplx* x, cplx* y, hipStream_t st, double* part) const {
     if (!ok_) fail("MultiApply::launch without a successful build");
+    if (part && !tile_parts()) fail("MultiApply::launch: tile partial sums need the per-vector kernel");
+    if (units_) {
+        XParams p = h0_->xparams;
+        MsParams m = msp_;
+        m.part = part;
+        const cplx* xa = x;
+        cplx* ya = y;
+        const uint4* tl = msp_.tiles;
+        void* args[] = {&p, &m, &tl, &xa, &ya};
+        QK_HIP(hipLaunchKernel(ms_fn_, dim3(grid_), dim3(64 * kXWaves), args, size_t(h0_->xlds_bytes), st));
+        return;
+    }
     if (vb_ == 4)
         hipLaunchKernelGGL((k_apply_xchg_multi<4>), dim3(grid_), dim3(64 * kXWaves), h0_->xlds_bytes,
                            st, h0_->xparams, mp_, x, y);

@@ -108,6 +108,39 @@ struct MParams {
     u32 ntiles, tiles_per_xcd;
 };
 
+// Bond records of one sector's rows for the single-vector exchange kernels
+// (the per-row fields of XParams).
+struct XRecs {
+    const XRowNb* hnb;
+    const u32* hcnt;
+    const u64* mnb;
+    const u32* vrow;
+};
+
+// Multi-sector single-vector kernel: the tiles of every sector in one array
+// (sector s from xt[t0]) and each sector's records, a unit (one vector of
+// one sector, at xoff of the concatenated buffers; its tile partial sums at
+// part[poff + tile - t0]), and the tile list: entry (tile in xt, unit,
+// sector), an empty tile (nvalid 0) for padding; the sector rides in the
+// entry so that the tile, unit and sector loads go out together.
+struct MsSec {
+    XRecs rec;
+    u32 t0, pad;
+};
+struct MsUnit {
+    u64 xoff;
+    u32 sec, poff;
+};
+static_assert(sizeof(MsUnit) == 16, "MsUnit layout");
+struct MsParams {
+    const uint4* tiles;
+    const XTile* xt;
+    const MsUnit* units;
+    const MsSec* secs;
+    double* part;   // null, or per tile Re<x, y> over its states
+    u32 ntiles, tiles_per_xcd;
+};
+
 // Everything one apply launch needs, passed by value as the kernel argument.
 struct ApplyParams {
     const u16* jlo;
@@ -292,16 +325,31 @@ public:
     };
     // false when the groups cannot share one launch (callers then apply per group)
     bool build(const std::vector<Group>& groups, hipStream_t st);
-    void launch(const cplx* x, cplx* y, hipStream_t st) const;
+    // part (tile_parts() only): unit u's tile t writes Re<x, y> over the
+    // tile's states to part[u * part_stride() + t], t < unit_tiles()[u];
+    // units are the groups' vectors in order
+    void launch(const cplx* x, cplx* y, hipStream_t st, double* part = nullptr) const;
     bool ok() const { return ok_; }
+    bool tile_parts() const { return ok_ && units_ && pstride_ > 0; }
+    u32 part_stride() const { return pstride_; }
+    const std::vector<u32>& unit_tiles() const { return utiles_; }
 
 private:
+    // per-vector units on XCD-resident tiles (k_apply_xchg_ms); false: the
+    // block kernel k_apply_xchg_multi
+    bool build_units(const std::vector<const Group*>& gs, hipStream_t st);
     bool ok_ = false;
+    bool units_ = false;
     const Hamiltonian* h0_ = nullptr;
     MParams mp_{};
     DevBuf tiles_, jhis_;
     int grid_ = 0;
     int vb_ = 8;
+    MsParams msp_{};
+    DevBuf ms_tiles_, ms_units_, ms_secs_, ms_xt_;
+    const void* ms_fn_ = nullptr;
+    u32 pstride_ = 0;
+    std::vector<u32> utiles_;
 };
 
 // y = O x for an OpSum O taking states of basis `in` to basis `out`: both of

@@ -94,6 +94,31 @@ std::shared_ptr<const cplx> reference_normals(unsigned seed, size_t n) {
     return *cached;
 }
 
+// The same stream in device memory of the current device, for streams of at
+// most 1 GiB (null beyond: callers stage from the host copy), kept for the
+// process like the host copy, so a repeated sweep copies its start vectors
+// device to device instead of over PCIe (the N=20 FTLM sweep: 1.07 GB of
+// start vectors from a 189 MB stream).
+const cplx* reference_normals_dev(unsigned seed, size_t n, const cplx* host, hipStream_t st) {
+    if (sizeof(cplx) * n > (size_t(1) << 30)) return nullptr;
+    static std::mutex m;
+    static auto* cached = new DevBuf();
+    static unsigned cseed = 0;
+    static size_t cn = 0;
+    static int cdev = -1;
+    int dev = 0;
+    QK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(m);
+    if (cached->get() && cseed == seed && cn >= n && cdev == dev) return cached->get<cplx>();
+    *cached = DevBuf(sizeof(cplx) * std::max<size_t>(n, 1));
+    QK_HIP(hipMemcpyAsync(cached->get(), host, sizeof(cplx) * n, hipMemcpyHostToDevice, st));
+    QK_HIP(hipStreamSynchronize(st));
+    cseed = seed;
+    cn = n;
+    cdev = dev;
+    return cached->get<cplx>();
+}
+
 template <typename T>
 T fetch(const void* dev, hipStream_t st) {
     T v;
@@ -1107,6 +1132,7 @@ struct BlockRec {
     DevBuf bufU, bufY, bufP, c, cp, bp, alive, alpha, beta, nrm;
     DevBuf dparts, dcount;   // alpha partials written by the apply kernel
     u32 dstride = 0;         // 0: alpha from a separate pass over U and Y
+    bool mparts = false;     // partials from the multi-sector launch (per tile)
     int nsamp = 0;
     hipStream_t st = nullptr;
     MultiApply multi;   // all groups in one apply launch when they allow it
@@ -1138,14 +1164,29 @@ struct BlockRec {
             dmax = std::max(dmax, x.h->dim());
             ntot += x.nvec;
         }
+        // A single vector takes the multi-sector launch too when alpha may
+        // come from its tile partials: the same tiles and sums as in a
+        // batch, so one vector alone and the batch agree bit for bit.
         const char* e = std::getenv("QKRYLOV_MULTI_APPLY");
-        if (!(e && std::string(e) == "0") && ntot > 1 && dmax < (u64(1) << 22)) multi.build(mg, st);
-        // Per-vector single-vector launches can return alpha's partial sums
-        // themselves (the kernel holds x and y when it stores), which saves
-        // reading both vectors again every step.
-        dstride = 0;
         const char* f = std::getenv("QKRYLOV_FUSED_ALPHA");
-        bool fused = allow_fused && !multi.ok() && !(f && std::string(f) == "0") && !grps.empty();
+        const bool fuse_ok = allow_fused && !(f && std::string(f) == "0");
+        if (!(e && std::string(e) == "0") && (ntot > 1 || fuse_ok) && ntot >= 1 && dmax < (u64(1) << 22))
+            multi.build(mg, st);
+        // The apply kernels can return alpha's partial sums themselves (they
+        // hold x and y when they store), which saves reading both vectors
+        // again every step: per tile from the multi-sector launch, per
+        // workgroup from per-vector single-vector launches.
+        dstride = 0;
+        mparts = false;
+        if (fuse_ok && multi.tile_parts() && multi.unit_tiles().size() == size_t(nsamp)) {
+            mparts = true;
+            dstride = std::max<u32>(multi.part_stride(), 1);
+            const size_t pb = sizeof(double) * size_t(dstride) * std::max<size_t>(nsamp, 1);
+            dparts.resize(pb);
+            QK_HIP(hipMemsetAsync(dparts.get(), 0, pb, st));
+            upload(dcount, multi.unit_tiles().data(), multi.unit_tiles().size(), st);
+        }
+        bool fused = fuse_ok && !multi.ok() && !grps.empty();
         for (const Grp& x : grps)
             if (!(x.h->fused_dot(x.nvec) || x.h->dim() == 0)) fused = false;
         if (fused) {
@@ -1211,14 +1252,18 @@ struct BlockRec {
                 double* a = alpha.get<double>() + size_t(step) * nsamp;
                 double* b = beta.get<double>() + size_t(step) * nsamp;
                 if (dstride) {
-                    size_t j = 0;
-                    for (auto& g : grps)
-                        for (int r = 0; r < g.nvec; ++r, ++j) {
-                            const u64 o = g.off + u64(r) * g.h->dim();
-                            double* part = dparts.get<double>() + j * dstride;
-                            if (g.h->dim() && !g.h->apply_dot(U + o, Y + o, part, st))
-                                fail("block recurrence: fused apply not available");
-                        }
+                    if (mparts) {
+                        multi.launch(U, Y, st, dparts.get<double>());
+                    } else {
+                        size_t j = 0;
+                        for (auto& g : grps)
+                            for (int r = 0; r < g.nvec; ++r, ++j) {
+                                const u64 o = g.off + u64(r) * g.h->dim();
+                                double* part = dparts.get<double>() + j * dstride;
+                                if (g.h->dim() && !g.h->apply_dot(U + o, Y + o, part, st))
+                                    fail("block recurrence: fused apply not available");
+                            }
+                    }
                     vec::alpha_from_parts(dparts.get<double>(), dstride, dcount.get<u32>(), nsamp,
                                           c.get<double>(), a, alive.get<int>(), st);
                 } else {
@@ -1363,6 +1408,10 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
     Timer tm("ftlm");
     const std::shared_ptr<const cplx> stream_p = reference_normals(42, size_t(R) * dmax);
     const cplx* stream = stream_p.get();
+    hipStream_t st = hs[0]->stream;
+    const cplx* dstream = reference_normals_dev(42, size_t(R) * dmax, stream, st);
+    const hipMemcpyKind kind = dstream ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (dstream) stream = dstream;
     tm.mark("random stream");
 
     // per (sector, vector) tridiagonal results
@@ -1372,7 +1421,6 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
         std::vector<double> a, b;
     };
     std::vector<Sample> samples;
-    hipStream_t st = hs[0]->stream;
     for (auto* h : hs) QK_HIP(hipStreamSynchronize(h->stream));
 
     if (mode == 0) {
@@ -1388,8 +1436,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
             rec.setup(g, st);
             for (size_t gi = 0; gi < g.size(); ++gi)
                 QK_HIP(hipMemcpyAsync(rec.bufU.get<cplx>() + rec.grps[gi].off, stream,
-                                      sizeof(cplx) * g[gi].first->dim() * R,
-                                      hipMemcpyHostToDevice, st));
+                                      sizeof(cplx) * g[gi].first->dim() * R, kind, st));
             const std::vector<double> nrm = rec.start(0.0);
             tm.mark("stage vectors");
             std::vector<double> ah, bh;
@@ -1410,8 +1457,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
             BlockRec rec;
             rec.setup({{hs[s], 1}}, st, /*allow_fused=*/mode == 1);
             for (int r = 0; r < R; ++r) {
-                QK_HIP(hipMemcpyAsync(rec.bufU.get(), stream + size_t(r) * d,
-                                      sizeof(cplx) * d, hipMemcpyHostToDevice, st));
+                QK_HIP(hipMemcpyAsync(rec.bufU.get(), stream + size_t(r) * d, sizeof(cplx) * d, kind, st));
                 const std::vector<double> nrm = rec.start(0.0);
                 std::vector<double> ah, bh;
                 rec.run(n_steps, ah, bh);
@@ -1433,14 +1479,31 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
     });
     tm.mark("tridiagonal eigensolves");
     std::vector<double> Zs(size_t(nsec) * nb, 0.0), Es(Zs.size(), 0.0), E2s(Zs.size(), 0.0);
+    // The sums of one (sector, beta) run in sample order in registers and are
+    // stored when the sector changes: threads of neighbouring betas never
+    // write the same cache line inside the loop.
     parallel_for(nb, [&](int b) {
         const double beta = betas[b];
+        int cur = -1;
+        double Z = 0.0, E = 0.0, E2 = 0.0;
+        auto flush = [&]() {
+            if (cur < 0) return;
+            const size_t o = size_t(cur) * nb + b;
+            Zs[o] = Z;
+            Es[o] = E;
+            E2s[o] = E2;
+        };
         for (size_t i = 0; i < samples.size(); ++i) {
             const int s = samples[i].sec;
+            if (s != cur) {
+                flush();
+                cur = s;
+                const size_t o = size_t(s) * nb + b;
+                Z = Zs[o];
+                E = Es[o];
+                E2 = E2s[o];
+            }
             const double n2 = samples[i].nrm * samples[i].nrm;
-            double& Z = Zs[size_t(s) * nb + b];
-            double& E = Es[size_t(s) * nb + b];
-            double& E2 = E2s[size_t(s) * nb + b];
             for (size_t k = 0; k < eig[i].size(); ++k) {
                 const double w = n2 * z0[i][k] * z0[i][k] * std::exp(-beta * eig[i][k]);
                 Z += w;
@@ -1448,6 +1511,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
                 E2 += eig[i][k] * eig[i][k] * w;
             }
         }
+        flush();
     });
     for (int s = 0; s < nsec; ++s) {
         if (hs[s]->dim() == 0) continue;

@@ -100,9 +100,9 @@ def test_continued_fraction_matches_oracle(qk, orc):
         single = qk.continued_fraction_coeffs(H, phis[j], 30)
         got = block[j]
         assert len(got.alphas) == len(a_ref) and len(got.betas) == len(b_ref)
-        # one vector alone takes alpha from the apply kernel's own partial
-        # sums, the block from a separate pass: same values up to summation
-        # order in the first steps
+        # one vector alone takes alpha from the apply kernel's partial sums
+        # per workgroup, the block from the multi-sector launch's partial
+        # sums per tile: same values up to summation order in the first steps
         assert np.abs(got.alphas[:10] - single.alphas[:10]).max() < 1e-12
         assert abs(got.norm_phi0 - n_ref) <= 1e-13 * n_ref
         # the first steps agree tightly; orthogonality loss later amplifies
@@ -180,6 +180,34 @@ def test_ftlm_sweep_all_sectors(qk, orc, lat):
             # Cv = beta^2 (<E^2> - <E>^2) cancels: rounding is relative to beta^2 <E>^2
             # (one-state sectors have Cv = 0 up to that noise)
             assert np.all(np.abs(Cv[s] - Cvr) <= 1e-8 * np.abs(Cvr) + 1e-12 * betas**2 * Er**2), (R, s)
+
+
+@pytest.mark.parametrize("kernel", ["units:2:4", "units:1:2", "units:2:1", "block"])
+def test_ftlm_multi_sector_kernels_bit_identical(qk, lat, monkeypatch, kernel):
+    """Both multi-sector apply kernels add every state's terms in the order of
+    the single-vector kernel.  One vector per tile with each vector's tiles
+    on one XCD (units: states per lane, bond batch) also returns alpha's
+    partial sums per tile, and the plain loop runs one vector through the
+    same kernel and sums: batched and plain agree bit for bit.  Tiles applied
+    to blocks of vectors (block) take alpha from a separate pass, as the
+    plain loop with alpha from a separate dot does: again bit for bit."""
+    if kernel == "block":
+        monkeypatch.setenv("QKRYLOV_MULTI_KERNEL", "block")
+        plain = "dot"
+    else:
+        _, spl, nb = kernel.split(":")
+        monkeypatch.setenv("QKRYLOV_MULTI_SPL", spl)
+        monkeypatch.setenv("QKRYLOV_MULTI_NB", nb)
+        plain = "fused"
+    n = 12
+    os_ = lat.xxz_chain(n, 1.0, 1.5)
+    hs = [qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(n, sector=make_sector({"use_sz": True, "sz2": 2 * k - n})),
+                                   qk.SpinHalfSite(), os_) for k in range(n + 1)]
+    betas = np.array([0.1, 1.0, 5.0])
+    print(f"[ftlm multi] kernel={kernel}", flush=True)
+    Z, E, Cv = qk.ftlm_sweep(hs, betas, 11, 25, batched=True)
+    Zp, Ep, Cvp = qk.ftlm_sweep(hs, betas, 11, 25, batched=False, plain_alpha=plain)
+    assert np.array_equal(Z, Zp) and np.array_equal(E, Ep) and np.array_equal(Cv, Cvp)
 
 
 @pytest.mark.parametrize("batched", [True, False])
@@ -271,11 +299,12 @@ def test_davidson_small_space_dependent_corrections(qk, orc, monkeypatch, block)
     assert np.all(np.diff(ev) >= -1e-12)
 
 
-def test_ftlm_batched_equals_plain_dot(qk, lat):
-    """The plain loop with alpha from a separate dot (the reference loop's
-    structure, ftlm.cpp:24-53) performs the batched sweep's arithmetic per
-    vector: identical Z, E, Cv; the plain loop with alpha from the apply
-    kernel's partial sums sums alpha in another order (within 1e-8)."""
+def test_ftlm_batched_equals_plain(qk, lat):
+    """The plain loop with alpha from the apply kernel's partial sums runs
+    each vector through the batched sweep's kernel, tiles and sums: identical
+    Z, E, Cv.  The plain loop with alpha from a separate dot (the reference
+    loop's structure, ftlm.cpp:24-53) sums alpha in another order (within
+    1e-8)."""
     n = 12
     os_ = lat.xxz_chain(n, 1.0, 1.5)
     hs = [qk.MatrixFreeHamiltonian(qk.SpinHalfBasis(n, sector=qk.Sector().set_sz(2 * k - n)),
@@ -284,8 +313,11 @@ def test_ftlm_batched_equals_plain_dot(qk, lat):
     Zb, Eb, Cb = qk.ftlm_sweep(hs, betas, 4, 40, batched=True)
     Zd, Ed, Cd = qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="dot")
     Zf, Ef, Cf = qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="fused")
-    print(f"[ftlm dot] max |Cv batched - Cv dot| = {np.abs(Cb - Cd).max():.1e}, "
-          f"fused: {np.max(np.abs(Cb - Cf) / np.maximum(np.abs(Cb), 1e-300)):.1e}", flush=True)
-    assert np.array_equal(Zb, Zd) and np.array_equal(Eb, Ed) and np.array_equal(Cb, Cd)
-    for a, b in ((Zb, Zf), (Eb, Ef), (Cb, Cf)):
+    print(f"[ftlm plain] max |Cv batched - Cv fused| = {np.abs(Cb - Cf).max():.1e}, "
+          f"dot: {np.max(np.abs(Cb - Cd) / np.maximum(np.abs(Cb), 1e-300)):.1e}", flush=True)
+    assert np.array_equal(Zb, Zf) and np.array_equal(Eb, Ef) and np.array_equal(Cb, Cf)
+    for a, b in ((Zb, Zd), (Eb, Ed)):
         assert np.all(np.abs(a - b) <= 1e-8 * np.maximum(np.abs(a), 1e-12))
+    # Cv = beta^2 (<E^2> - <E>^2) cancels: its rounding is relative to
+    # beta^2 <E>^2 (the one-state sectors' Cv is that rounding alone)
+    assert np.all(np.abs(Cb - Cd) <= 1e-8 * np.abs(Cb) + 1e-12 * betas**2 * Eb**2)
