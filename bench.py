@@ -167,25 +167,36 @@ def bench_ftlm(qk, lat, batched_only=False):
         return {"workload": "XXZ chain N=20 (Jxy=1, Jz=1.5), all 21 Sz sectors, R=64, M=100, "
                             "200 temperatures; batched sweep only",
                 "samples": nsamp, "samples_per_s_batched": nsamp / dt}
-    qk.ftlm_sweep(hs, betas[:1], 1, 2, batched=False)
-    qk.ftlm_sweep(hs, betas[:1], 1, 2, batched=False, plain_alpha="dot")
+    for how in ("fused", "dot", "workgroup"):
+        qk.ftlm_sweep(hs, betas[:1], 1, 2, batched=False, plain_alpha=how)
     log(f"ftlm warm-up (random stream, kernels): {time.perf_counter() - t:.3f} s")
     out = {}
-    # plain: one vector at a time, alpha from the apply kernel's partial sums
-    # (the fastest plain loop); plain_dot: alpha from a separate dot, as the
-    # reference loop takes it -- the batched sweep's arithmetic per vector
-    for name, batched, how in (("batched", True, "fused"), ("plain", False, "fused"),
-                               ("plain_dot", False, "dot")):
+    # plain loops, one vector at a time: plain_tile through the batched
+    # sweep's kernel with alpha from its partial sums per tile (the batched
+    # sweep's arithmetic per vector); plain_dot with alpha from a separate
+    # dot, as the reference loop takes it; plain_workgroup through the
+    # single-vector kernel with alpha from its partial sums per workgroup
+    for name, batched, how in (("batched", True, "fused"), ("plain_tile", False, "fused"),
+                               ("plain_dot", False, "dot"), ("plain_workgroup", False, "workgroup")):
         t = time.perf_counter()
         Z, E, Cv = qk.ftlm_sweep(hs, betas, R, M, batched=batched, plain_alpha=how)
         dt = time.perf_counter() - t
         out[name] = (dt, Z, E, Cv)
         log(f"ftlm {name}: {dt:.3f} s")
-    (tb, Zb, Eb, Cb), (tp, Zp, Ep, Cp) = out["batched"], out["plain"]
-    (td, Zd, Ed, Cd) = out["plain_dot"]
+    tb, Zb, Eb, Cb = out["batched"]
+    fastest = min(("plain_tile", "plain_dot", "plain_workgroup"), key=lambda k: out[k][0])
+    tp = out[fastest][0]
+    _, Zt_, Et_, Ct_ = out["plain_tile"]
 
     def reldiff(a, b):
         return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+    def reldiff_cv(a, b):
+        # Cv = beta^2 (<E^2> - <E>^2) cancels: relative to its rounding
+        # floor beta^2 <E>^2 * 1e-12 where it is smaller (one-state sectors,
+        # whose Cv is that rounding alone)
+        floor = 1e-12 * betas**2 * Eb**2
+        return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
 
     # combine sectors (thermodynamics of the whole chain)
     Zt = Zb.sum(0)
@@ -195,16 +206,20 @@ def bench_ftlm(qk, lat, batched_only=False):
         "samples": nsamp,
         "samples_per_s_batched": nsamp / tb,
         "samples_per_s_plain": nsamp / tp,
-        "samples_per_s_plain_dot": nsamp / td,
+        "plain_fastest": fastest,
+        "samples_per_s_plain_tile": nsamp / out["plain_tile"][0],
+        "samples_per_s_plain_dot": nsamp / out["plain_dot"][0],
+        "samples_per_s_plain_workgroup": nsamp / out["plain_workgroup"][0],
         "speedup": tp / tb,
-        "speedup_note": "against the faster plain loop (alpha from the apply kernel's partial sums)",
-        "max_rel_diff_Z": reldiff(Zb, Zp),
-        "max_rel_diff_E": reldiff(Eb, Ep),
-        "max_rel_diff_Cv": reldiff(Cb, Cp),
-        "diff_note": "batched against the plain loop with alpha from the apply kernel's partial "
-                     "sums (the same tiles and sums per vector); against the plain loop with alpha "
-                     "from a separate dot the summation order of alpha differs",
-        "max_rel_diff_Cv_dot_plain": reldiff(Cb, Cd),
+        "speedup_note": "against the fastest of the three plain loops",
+        "max_rel_diff_Z": reldiff(Zb, Zt_),
+        "max_rel_diff_E": reldiff(Eb, Et_),
+        "max_rel_diff_Cv": reldiff_cv(Cb, Ct_),
+        "diff_note": "batched against plain_tile (the same kernel, tiles and sums per vector); the "
+                     "other plain loops sum alpha in another order",
+        "max_rel_diff_Cv_plain_dot": reldiff_cv(Cb, out["plain_dot"][3]),
+        "max_rel_diff_Cv_plain_workgroup": reldiff_cv(Cb, out["plain_workgroup"][3]),
+        "cv_diff_note": "relative to max(|Cv|, 1e-12 beta^2 E^2), Cv's rounding floor",
         "E_per_site_T0.05": float(Et[0] / n),
     }
 

@@ -442,11 +442,13 @@ int qkrylov_ftlm_f64(qkrylov_hamiltonian_h h, double beta, int n_random, int n_s
 /* FTLM over several sectors and a temperature grid.  For sector s and beta b
  * the outputs Z[s*nbeta+b], E[...], Cv[...] equal qkrylov_ftlm_f64(hs[s],
  * betas[b], n_random, n_steps).  mode 0: every (sector, random vector) pair
- * advanced together in one block recurrence per step; mode 1: one vector at
- * a time (the plain loop), alpha from the apply kernel's partial sums where
- * the kernel provides them; mode 2: the plain loop with alpha from a separate
- * dot as the reference's loop takes it (ftlm.cpp:24-53), the same arithmetic
- * per vector as mode 0. */
+ * advanced together in one block recurrence per step; modes 1-3: one vector
+ * at a time (the plain loop).  mode 1: each vector through mode 0's apply
+ * kernel, alpha from its partial sums per tile (the same arithmetic per
+ * vector as mode 0 for sectors below 2^22 states); mode 2: alpha from a
+ * separate dot as the reference's loop takes it (ftlm.cpp:24-53); mode 3:
+ * the single-vector apply kernel, alpha from its partial sums per workgroup.
+ * Other modes: QKRYLOV_ERROR_INVALID_ARG. */
 int qkrylov_ftlm_sweep_f64(const qkrylov_hamiltonian_h* hs, int nsec, const double* betas, int nbeta,
                            int n_random, int n_steps, int mode,
                            double* Z_out, double* E_out, double* Cv_out);

@@ -973,11 +973,14 @@ def ftlm(H: MatrixFreeHamiltonian, beta: float, n_random: int = 50, n_steps: int
 def ftlm_sweep(Hs: Sequence[MatrixFreeHamiltonian], betas, n_random: int = 50,
                n_steps: int = 100, batched: bool = True, plain_alpha: str = "fused"):
     """Z, E, Cv arrays of shape [len(Hs), len(betas)]; row s equals ftlm(Hs[s], beta).
-    batched=False runs one vector at a time; plain_alpha="fused" takes alpha
-    from the apply kernel's partial sums, "dot" from a separate pass (the
-    reference loop's structure, the batched sweep's arithmetic)."""
-    if plain_alpha not in ("fused", "dot"):
-        raise ValueError("plain_alpha is 'fused' or 'dot'")
+    batched=False runs one vector at a time: plain_alpha="fused" through the
+    batched sweep's apply kernel, alpha from its partial sums per tile (the
+    batched sweep's arithmetic per vector); "dot" with alpha from a separate
+    pass (the reference loop's structure); "workgroup" through the
+    single-vector kernel, alpha from its partial sums per workgroup."""
+    modes = {"fused": 1, "dot": 2, "workgroup": 3}
+    if plain_alpha not in modes:
+        raise ValueError("plain_alpha is 'fused', 'dot' or 'workgroup'")
     b = np.ascontiguousarray(betas, dtype=np.float64)
     ns, nb = len(Hs), b.size
     hs = (ct.c_void_p * ns)(*[h._h for h in Hs])
@@ -985,6 +988,6 @@ def ftlm_sweep(Hs: Sequence[MatrixFreeHamiltonian], betas, n_random: int = 50,
     E = np.zeros((ns, nb))
     Cv = np.zeros((ns, nb))
     check(load().qkrylov_ftlm_sweep_f64(hs, ns, _ptr(b), nb, int(n_random), int(n_steps),
-                                        0 if batched else (1 if plain_alpha == "fused" else 2),
+                                        0 if batched else modes[plain_alpha],
                                         _ptr(Z), _ptr(E), _ptr(Cv)), "ftlm_sweep")
     return Z, E, Cv

@@ -921,7 +921,8 @@ int qkrylov_ftlm_f64(qkrylov_hamiltonian_h h, double beta, int n_random, int n_s
 }
 int qkrylov_ftlm_sweep_f64(const qkrylov_hamiltonian_h* hs, int nsec, const double* betas, int nbeta,
                            int n_random, int n_steps, int mode, double* Z, double* E, double* Cv) {
-    if (!hs || nsec <= 0 || !betas || nbeta <= 0 || !Z || !E || !Cv) return QKRYLOV_ERROR_INVALID_ARG;
+    if (!hs || nsec <= 0 || !betas || nbeta <= 0 || !Z || !E || !Cv || mode < 0 || mode > 3)
+        return QKRYLOV_ERROR_INVALID_ARG;
     for (int s = 0; s < nsec; ++s)
         if (!hok(hs[s])) return QKRYLOV_ERROR_INVALID_ARG;
     return guarded([&] {

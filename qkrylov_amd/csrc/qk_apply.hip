@@ -697,9 +697,10 @@ __device__ __forceinline__ void xbonds_mix(const XBond* __restrict__ bonds, cons
 
 // LDS of the exchange kernels: the diagonal tables of the groups spanning
 // both halves, then the low-half rank table.
+template <int WV = kXWaves>
 __device__ __forceinline__ void xstage_lds(const XParams& p, unsigned char* smem, cplx*& s_tab,
                                            u16*& s_jlo) {
-    constexpr u32 NT = 64 * kXWaves;
+    constexpr u32 NT = 64 * WV;
     s_tab = reinterpret_cast<cplx*>(smem);
     s_jlo = reinterpret_cast<u16*>(smem + size_t(p.ndmix_tab) * sizeof(cplx));
     for (u32 i = threadIdx.x; i < p.ndmix_tab; i += NT) s_tab[i] = p.dmix_tab[i];
@@ -815,15 +816,16 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
 // the same order, as k_apply_xchg_s: y is bit-identical to per-vector
 // launches.  With mp.part each tile also writes Re<x, y> over its states,
 // summed in a fixed order, so the recurrence's alpha needs no pass of its
-// own.
-template <int SPL, int NB, bool SYM>
-__global__ void __launch_bounds__(64 * kXWaves)
+// own.  WV waves per workgroup: 8 leaves a wave up to 256 registers, for
+// variants that keep 16 gathers in flight.
+template <int SPL, int NB, bool SYM, int WV>
+__global__ void __launch_bounds__(64 * WV)
 k_apply_xchg_ms(const XParams p, const MsParams mp, const uint4* __restrict__ tl,
                 const cplx* __restrict__ x, cplx* __restrict__ y) {
     extern __shared__ __align__(16) unsigned char smem[];
     cplx* s_tab;
     u16* s_jlo;
-    xstage_lds(p, smem, s_tab, s_jlo);
+    xstage_lds<WV>(p, smem, s_tab, s_jlo);
 
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 lane = threadIdx.x & 63;
@@ -831,9 +833,9 @@ k_apply_xchg_ms(const XParams p, const MsParams mp, const uint4* __restrict__ tl
     // of its XCD's chunk, and loads the next entry (a scalar load from a
     // read-only argument) while the current tile runs.
     const u32 xcd = blockIdx.x % kXcd;
-    const u32 W = (gridDim.x / kXcd) * kXWaves;
+    const u32 W = (gridDim.x / kXcd) * WV;
     const u32 t_end = min((xcd + 1) * mp.tiles_per_xcd, mp.ntiles);
-    u32 t = xcd * mp.tiles_per_xcd + (blockIdx.x / kXcd) * kXWaves + wave;
+    u32 t = xcd * mp.tiles_per_xcd + (blockIdx.x / kXcd) * WV + wave;
     if (t >= t_end) return;
     uint4 e = tl[t];
     for (;;) {
@@ -1882,21 +1884,41 @@ bool MultiApply::build_units(const std::vector<const Group*>& gs, hipStream_t st
     const Hamiltonian& h0 = *h0_;
     for (const Group* g : gs)
         if (g->h->xsym != h0.xsym) return false;
-    int s = 2, nb = 4;
+    // (states per lane, bonds per batch): up to 16 gathers per wave in flight
+    // (states per lane, bonds per batch, waves per workgroup): up to 16
+    // gathers per wave in flight
+    struct Variant {
+        int s, nb, wv;
+        const void* fn[2];   // [SYM]
+    };
+#define QK_MS(S, NB, WV)                                                                    \
+    Variant {                                                                               \
+        S, NB, WV, {                                                                        \
+            (const void*)k_apply_xchg_ms<S, NB, false, WV>, (const void*)k_apply_xchg_ms<S, NB, true, WV> \
+        }                                                                                   \
+    }
+    static const Variant variants[] = {QK_MS(1, 2, 16), QK_MS(1, 4, 16), QK_MS(1, 8, 16), QK_MS(2, 1, 16),
+                                       QK_MS(2, 2, 16), QK_MS(2, 4, 16), QK_MS(2, 4, 8),  QK_MS(2, 8, 8),
+                                       QK_MS(4, 2, 16), QK_MS(4, 4, 8)};
+#undef QK_MS
+    // default (2, 4, 8): N=20 FTLM sweep 4,214 samples/s against 4,075-4,092
+    // with 16-wave workgroups; 16 gathers in flight per wave ran slower
+    // ((2, 8, 8) 3,250, (4, 4, 8) 3,119, (1, 8, 16) 3,388; (4, 2, 16) 4,013)
+    int s = 2, nb = 4, wv = 8;
     if (const char* e = std::getenv("QKRYLOV_MULTI_SPL")) s = std::atoi(e);
     if (const char* e = std::getenv("QKRYLOV_MULTI_NB")) nb = std::atoi(e);
-    if (s != 1 && s != 2) s = 2;
-    if (nb != 1 && nb != 2 && nb != 4) nb = 4;
-    const void* table[2][2][3] = {
-        {{(const void*)k_apply_xchg_ms<1, 1, false>, (const void*)k_apply_xchg_ms<1, 2, false>,
-          (const void*)k_apply_xchg_ms<1, 4, false>},
-         {(const void*)k_apply_xchg_ms<2, 1, false>, (const void*)k_apply_xchg_ms<2, 2, false>,
-          (const void*)k_apply_xchg_ms<2, 4, false>}},
-        {{(const void*)k_apply_xchg_ms<1, 1, true>, (const void*)k_apply_xchg_ms<1, 2, true>,
-          (const void*)k_apply_xchg_ms<1, 4, true>},
-         {(const void*)k_apply_xchg_ms<2, 1, true>, (const void*)k_apply_xchg_ms<2, 2, true>,
-          (const void*)k_apply_xchg_ms<2, 4, true>}}};
-    ms_fn_ = table[h0.xsym ? 1 : 0][s == 1 ? 0 : 1][nb == 1 ? 0 : nb == 2 ? 1 : 2];
+    if (const char* e = std::getenv("QKRYLOV_MULTI_WAVES")) wv = std::atoi(e);
+    ms_fn_ = nullptr;
+    for (const Variant& v : variants)
+        if (v.s == s && v.nb == nb && (v.wv == wv || !ms_fn_)) {
+            ms_fn_ = v.fn[h0.xsym ? 1 : 0];
+            ms_wv_ = v.wv;
+        }
+    if (!ms_fn_) {   // not built: the default
+        s = 2;
+        ms_wv_ = 8;
+        ms_fn_ = h0.xsym ? (const void*)k_apply_xchg_ms<2, 4, true, 8> : (const void*)k_apply_xchg_ms<2, 4, false, 8>;
+    }
     if (h0.xlds_bytes > 64 * 1024)
         QK_HIP(hipFuncSetAttribute(ms_fn_, hipFuncAttributeMaxDynamicSharedMemorySize, h0.xlds_bytes));
 
@@ -1955,13 +1977,13 @@ bool MultiApply::build_units(const std::vector<const Group*>& gs, hipStream_t st
     }
     // grid: the workgroups the kernel keeps resident, or fewer for few tiles
     // (one wave per tile at most: the plain loop's small sectors)
-    const int per_cu = std::max(1, resident_wgs(ms_fn_, h0.xlds_bytes, kXWaves));
+    const int per_cu = std::max(1, resident_wgs(ms_fn_, h0.xlds_bytes, ms_wv_));
     const int by_lds = (160 * 1024) / std::max(h0.xlds_bytes, 1);
     const int cus = device_cu_count();
     u64 real = 0;
     for (u32 u = 0; u < units.size(); ++u) real += ntile[units[u].sec];
-    const u64 want = (real + kXWaves - 1) / kXWaves;
-    grid_ = int(std::min<u64>(want, u64(cus) * std::max(1, std::min({per_cu, by_lds, 2}))));
+    const u64 want = (real + ms_wv_ - 1) / ms_wv_;
+    grid_ = int(std::min<u64>(want, u64(cus) * std::max(1, std::min({per_cu, by_lds, 32 / ms_wv_}))));
     grid_ = std::max(kXcd, (grid_ + kXcd - 1) / kXcd * kXcd);
     std::vector<std::vector<uint4>> lists(kXcd);
     size_t longest = 0;
@@ -2008,7 +2030,7 @@ void MultiApply::launch(const cplx* x, cplx* y, hipStream_t st, double* part) co
         cplx* ya = y;
         const uint4* tl = msp_.tiles;
         void* args[] = {&p, &m, &tl, &xa, &ya};
-        QK_HIP(hipLaunchKernel(ms_fn_, dim3(grid_), dim3(64 * kXWaves), args, size_t(h0_->xlds_bytes), st));
+        QK_HIP(hipLaunchKernel(ms_fn_, dim3(grid_), dim3(64 * ms_wv_), args, size_t(h0_->xlds_bytes), st));
         return;
     }
     if (vb_ == 4)

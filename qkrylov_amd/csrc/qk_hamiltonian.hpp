@@ -348,6 +348,7 @@ private:
     MsParams msp_{};
     DevBuf ms_tiles_, ms_units_, ms_secs_, ms_xt_;
     const void* ms_fn_ = nullptr;
+    int ms_wv_ = 16;   // waves per workgroup of ms_fn_
     u32 pstride_ = 0;
     std::vector<u32> utiles_;
 };

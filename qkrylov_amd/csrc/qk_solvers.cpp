@@ -1139,9 +1139,10 @@ struct BlockRec {
 
     // allow_fused: alpha may come from the apply kernel's partial sums (else
     // always from the separate pass, whose per-sample reduction is the same
-    // whatever other samples share the recurrence)
+    // whatever other samples share the recurrence); multi_single: a single
+    // vector may take the multi-sector launch (else the single-vector kernel)
     void setup(const std::vector<std::pair<const Hamiltonian*, int>>& g, hipStream_t stream,
-               bool allow_fused = true) {
+               bool allow_fused = true, bool multi_single = true) {
         st = stream;
         std::vector<u64> lens;
         u64 off = 0;
@@ -1170,7 +1171,8 @@ struct BlockRec {
         const char* e = std::getenv("QKRYLOV_MULTI_APPLY");
         const char* f = std::getenv("QKRYLOV_FUSED_ALPHA");
         const bool fuse_ok = allow_fused && !(f && std::string(f) == "0");
-        if (!(e && std::string(e) == "0") && (ntot > 1 || fuse_ok) && ntot >= 1 && dmax < (u64(1) << 22))
+        if (!(e && std::string(e) == "0") && (ntot > 1 || (fuse_ok && multi_single)) && ntot >= 1 &&
+            dmax < (u64(1) << 22))
             multi.build(mg, st);
         // The apply kernels can return alpha's partial sums themselves (they
         // hold x and y when they store), which saves reading both vectors
@@ -1455,7 +1457,8 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
             const u64 d = hs[s]->dim();
             if (d == 0) continue;
             BlockRec rec;
-            rec.setup({{hs[s], 1}}, st, /*allow_fused=*/mode == 1);
+            rec.setup({{hs[s], 1}}, st, /*allow_fused=*/mode == 1 || mode == 3,
+                      /*multi_single=*/mode == 1);
             for (int r = 0; r < R; ++r) {
                 QK_HIP(hipMemcpyAsync(rec.bufU.get(), stream + size_t(r) * d, sizeof(cplx) * d, kind, st));
                 const std::vector<double> nrm = rec.start(0.0);

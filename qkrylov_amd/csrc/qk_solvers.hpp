@@ -108,11 +108,12 @@ struct FTLMPoint {
 };
 
 // ftlm over sectors and a beta grid; out[s*nbeta + b].  mode 0: all
-// (sector, vector) pairs advanced together; mode 1: one vector at a time,
-// alpha from the apply kernel's partial sums where it provides them; mode 2:
-// one vector at a time with alpha from a separate pass over the vector and
-// its image, as the reference's loop takes it (ftlm.cpp:24-53) -- the same
-// arithmetic per vector as mode 0.
+// (sector, vector) pairs advanced together; modes 1-3 one vector at a time:
+// 1: through mode 0's multi-sector kernel, alpha from its partial sums per
+// tile (the same arithmetic per vector as mode 0 where the sectors take that
+// launch); 2: alpha from a separate pass over the vector and its image, as
+// the reference's loop takes it (ftlm.cpp:24-53); 3: the single-vector
+// kernel, alpha from its partial sums per workgroup.
 std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
                                   const std::vector<double>& betas, int n_random, int n_steps,
                                   int mode);

@@ -313,11 +313,15 @@ def test_ftlm_batched_equals_plain(qk, lat):
     Zb, Eb, Cb = qk.ftlm_sweep(hs, betas, 4, 40, batched=True)
     Zd, Ed, Cd = qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="dot")
     Zf, Ef, Cf = qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="fused")
+    Zw, Ew, Cw = qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="workgroup")
     print(f"[ftlm plain] max |Cv batched - Cv fused| = {np.abs(Cb - Cf).max():.1e}, "
           f"dot: {np.max(np.abs(Cb - Cd) / np.maximum(np.abs(Cb), 1e-300)):.1e}", flush=True)
     assert np.array_equal(Zb, Zf) and np.array_equal(Eb, Ef) and np.array_equal(Cb, Cf)
-    for a, b in ((Zb, Zd), (Eb, Ed)):
-        assert np.all(np.abs(a - b) <= 1e-8 * np.maximum(np.abs(a), 1e-12))
-    # Cv = beta^2 (<E^2> - <E>^2) cancels: its rounding is relative to
-    # beta^2 <E>^2 (the one-state sectors' Cv is that rounding alone)
-    assert np.all(np.abs(Cb - Cd) <= 1e-8 * np.abs(Cb) + 1e-12 * betas**2 * Eb**2)
+    for Zo, Eo, Co in ((Zd, Ed, Cd), (Zw, Ew, Cw)):
+        for a, b in ((Zb, Zo), (Eb, Eo)):
+            assert np.all(np.abs(a - b) <= 1e-8 * np.maximum(np.abs(a), 1e-12))
+        # Cv = beta^2 (<E^2> - <E>^2) cancels: its rounding is relative to
+        # beta^2 <E>^2 (the one-state sectors' Cv is that rounding alone)
+        assert np.all(np.abs(Cb - Co) <= 1e-8 * np.abs(Cb) + 1e-12 * betas**2 * Eb**2)
+    with pytest.raises(ValueError):
+        qk.ftlm_sweep(hs, betas, 4, 40, batched=False, plain_alpha="sum")
