@@ -776,19 +776,20 @@ __device__ __forceinline__ void xs_tile(const XParams& p, const XTile& T, const 
 // 64*SPL consecutive ranks of one row and lane l holds ranks r0+l+64k.
 // Bonds are taken NB at a time (xbonds_*), so up to NB*SPL gathers per wave
 // are in flight.  SYM: every low-half and straddling bond has a01 == a10.
-template <int SPL, int NB, bool SYM>
-__global__ void __launch_bounds__(64 * kXWaves)
+// WV waves per workgroup.
+template <int SPL, int NB, bool SYM, int WV = kXWaves>
+__global__ void __launch_bounds__(64 * WV)
 k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y) {
     extern __shared__ __align__(16) unsigned char smem[];
     cplx* s_tab;
     u16* s_jlo;
-    xstage_lds(p, smem, s_tab, s_jlo);
+    xstage_lds<WV>(p, smem, s_tab, s_jlo);
 
     const u32 wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const u32 lane = threadIdx.x & 63;
     double dot = 0.0;   // Re<x, y> over this lane's states (used with dot_part)
 
-    QK_FOR_TILES_W(p, wave, t, kXWaves) {
+    QK_FOR_TILES_W(p, wave, t, WV) {
         const XTile T = p.xtiles[t];
         const u32 hi = __builtin_amdgcn_readfirstlane(T.hi);
         if (lane >= T.nvalid) continue;
@@ -796,13 +797,13 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
     }
     if (p.dot_part) {
         // this workgroup's Re<x, y>, reduced in a fixed order
-        __shared__ double s_dot[kXWaves];
+        __shared__ double s_dot[WV];
         for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off);
         if (lane == 0) s_dot[wave] = dot;
         __syncthreads();
         if (threadIdx.x == 0) {
             double tsum = 0.0;
-            for (int w = 0; w < kXWaves; ++w) tsum += s_dot[w];
+            for (int w = 0; w < WV; ++w) tsum += s_dot[w];
             p.dot_part[blockIdx.x] = tsum;
         }
     }
@@ -810,7 +811,7 @@ k_apply_xchg_s(const XParams p, const cplx* __restrict__ x, cplx* __restrict__ y
 
 // Multi-sector single-vector kernel: every (sector, vector) pair is a unit
 // whose tiles all run on one XCD, the units spread over the 8 XCDs by size
-// (the tile list holds each XCD's units in its chunks), so the vector an XCD
+// (the tile list holds each XCD's units in its chunk), so the vector an XCD
 // works on stays in its L2 while the unit runs (the N=20 chain's largest
 // sector: 2.9 MB per vector against 4 MB of L2).  Per tile the same work, in
 // the same order, as k_apply_xchg_s: y is bit-identical to per-vector
@@ -935,7 +936,7 @@ void launch_apply_t(const Hamiltonian& h, const cplx* x, cplx* y, u64 ld, hipStr
             const cplx* xa = x;
             cplx* ya = y;
             void* args[] = {const_cast<XParams*>(&h.xparams_s), &xa, &ya};
-            QK_HIP(hipLaunchKernel(h.xs_fn, dim3(h.xgrid_s), dim3(64 * kXWaves), args,
+            QK_HIP(hipLaunchKernel(h.xs_fn, dim3(h.xgrid_s), dim3(64 * h.xs_wv), args,
                                    size_t(h.xlds_bytes), st));
             return;
         }
@@ -1634,7 +1635,7 @@ std::vector<XTile> Hamiltonian::xtiles_for(u32 span) const {
 
 // Single-vector exchange kernel with s states per lane and bonds in batches
 // of nb (xbonds_*): tiles of 64*s ranks, grid, LDS limit.
-void Hamiltonian::set_single_kernel(int s, int nb) {
+void Hamiltonian::set_single_kernel(int s, int nb, int wv) {
     if (!xchg) return;
     if (s != 1 && s != 2 && s != 4) s = 2;
     if (nb != 1 && nb != 2 && nb != 4) nb = 1;
@@ -1662,11 +1663,26 @@ void Hamiltonian::set_single_kernel(int s, int nb) {
     spl = s;
     bond_batch = nb;
     xs_fn = table[xsym ? 1 : 0][slot(s)][slot(nb)];
+    xs_wv = kXWaves;
+    // 8-wave workgroups where built (2 states per lane, 2 or 4 bonds per
+    // batch): by default below 2^18 states, where launches are short and a
+    // finer grid fills the chip sooner; QKRYLOV_XS_WAVES sets 8 or 16
+    if (wv == 0) {
+        wv = dim() < (u64(1) << 18) ? 8 : 16;
+        if (const char* e = std::getenv("QKRYLOV_XS_WAVES")) wv = std::atoi(e);
+    }
+    if (wv == 8 && s == 2 && nb != 1) {
+        const void* t8[2][2] = {
+            {(const void*)k_apply_xchg_s<2, 2, false, 8>, (const void*)k_apply_xchg_s<2, 4, false, 8>},
+            {(const void*)k_apply_xchg_s<2, 2, true, 8>, (const void*)k_apply_xchg_s<2, 4, true, 8>}};
+        xs_fn = t8[xsym ? 1 : 0][nb == 2 ? 0 : 1];
+        xs_wv = 8;
+    }
     if (xlds_bytes > 64 * 1024)
         QK_HIP(hipFuncSetAttribute(xs_fn, hipFuncAttributeMaxDynamicSharedMemorySize, xlds_bytes));
     // the grid: workgroups this variant keeps resident per CU (registers, LDS)
-    xs_res = resident_wgs(xs_fn, xlds_bytes, kXWaves);
-    xgrid_s = grid_for(xlds_bytes, xparams_s.ntiles, &xparams_s.tiles_per_xcd, kXWaves, xs_res);
+    xs_res = resident_wgs(xs_fn, xlds_bytes, xs_wv);
+    xgrid_s = grid_for(xlds_bytes, xparams_s.ntiles, &xparams_s.tiles_per_xcd, xs_wv, xs_res);
 }
 
 // Autotuner: times every single-vector variant (states per lane x bond
@@ -1742,24 +1758,27 @@ std::string Hamiltonian::autotune(int nvec) {
         return os.str();
     }
     double best = 1e300;
-    int bs = spl, bn = bond_batch;
+    int bs = spl, bn = bond_batch, bw = xs_wv;
     os << "\"single\": {";
     bool first = true;
     for (int s : {1, 2, 4})
-        for (int nb : {1, 2, 4}) {
-            if (s == 4 && nb == 4) continue;
-            set_single_kernel(s, nb);
-            const double ms = time_ms(1, reps);
-            os << (first ? "" : ", ") << "\"" << s << "x" << nb << "\": " << ms;
-            first = false;
-            if (ms < best) {
-                best = ms;
-                bs = s;
-                bn = nb;
+        for (int nb : {1, 2, 4})
+            for (int wv : {16, 8}) {
+                if (s == 4 && nb == 4) continue;
+                if (wv == 8 && !(s == 2 && nb != 1)) continue;   // 8-wave variants built
+                set_single_kernel(s, nb, wv);
+                const double ms = time_ms(1, reps);
+                os << (first ? "" : ", ") << "\"" << s << "x" << nb << (wv == 8 ? "w8" : "") << "\": " << ms;
+                first = false;
+                if (ms < best) {
+                    best = ms;
+                    bs = s;
+                    bn = nb;
+                    bw = wv;
+                }
             }
-        }
-    set_single_kernel(bs, bn);
-    os << "}, \"states_per_lane\": " << bs << ", \"bond_batch\": " << bn;
+    set_single_kernel(bs, bn, bw);
+    os << "}, \"states_per_lane\": " << bs << ", \"bond_batch\": " << bn << ", \"waves\": " << bw;
     if (nvec > 1) {
         double bestv = 1e300;
         int bg = vec_group;
@@ -2057,7 +2076,7 @@ bool Hamiltonian::apply_dot(const cplx* x, cplx* y, double* part, hipStream_t st
     const cplx* xa = x;
     cplx* ya = y;
     void* args[] = {&q, &xa, &ya};
-    QK_HIP(hipLaunchKernel(xs_fn, dim3(xgrid_s), dim3(64 * kXWaves), args, size_t(xlds_bytes), st));
+    QK_HIP(hipLaunchKernel(xs_fn, dim3(xgrid_s), dim3(64 * xs_wv), args, size_t(xlds_bytes), st));
     return true;
 }
 

@@ -252,7 +252,8 @@ struct Hamiltonian {
     std::string autotune(int nvec);
     void setup_exchange();
     void setup_rows();
-    void set_single_kernel(int states_per_lane, int bond_batch);
+    // waves: 8 or 16 per workgroup; 0 chooses by size (QKRYLOV_XS_WAVES)
+    void set_single_kernel(int states_per_lane, int bond_batch, int waves = 0);
     std::vector<XTile> xtiles_for(u32 span) const;
 
     std::shared_ptr<const Basis> basis;
@@ -303,6 +304,7 @@ struct Hamiltonian {
     int dlds_bytes = 0;
     int grid = 0;
     int xs_res = 0;   // workgroups of the single-vector exchange kernel resident per CU
+    int xs_wv = 16;   // waves per workgroup of xs_fn
 
     // scratch for host-buffer entry points
     mutable DevBuf scratch_x, scratch_y;
