@@ -193,9 +193,9 @@ def bench_ftlm(qk, lat, batched_only=False):
 
     def diff_cv(a, b):
         # Cv = beta^2 (<E^2> - <E>^2) is the difference of two terms of size
-        # beta^2 <E>^2; rounding moves it by a few ulps of those terms, so
-        # differences are scaled by beta^2 <E>^2
-        return float(np.max(np.abs(a - b) / np.maximum(betas**2 * Eb**2, 1e-300)))
+        # up to beta^2 <E^2> = Cv + beta^2 <E>^2; rounding moves it by a few
+        # ulps of that, so differences are scaled by it
+        return float(np.max(np.abs(a - b) / np.maximum(Cb + betas**2 * Eb**2, 1e-300)))
 
     # combine sectors (thermodynamics of the whole chain)
     Zt = Zb.sum(0)
@@ -218,7 +218,7 @@ def bench_ftlm(qk, lat, batched_only=False):
                      "other plain loops sum alpha in another order",
         "max_diff_Cv_plain_dot": diff_cv(Cb, out["plain_dot"][3]),
         "max_diff_Cv_plain_workgroup": diff_cv(Cb, out["plain_workgroup"][3]),
-        "cv_diff_note": "batched against the other plain loops: |dCv| / (beta^2 <E>^2), the scale of "
+        "cv_diff_note": "batched against the other plain loops: |dCv| / (beta^2 <E^2>), the scale of "
                         "the two terms whose difference Cv is",
         "E_per_site_T0.05": float(Et[0] / n),
     }

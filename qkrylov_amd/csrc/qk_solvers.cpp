@@ -1239,81 +1239,124 @@ struct BlockRec {
     // M steps; alpha/beta on the host as [step][sample].  With `graph` the
     // M steps (every apply and recurrence kernel, no host work in between:
     // alpha, beta and the scales stay on the device) are captured once into
-    // a hipGraph and launched as one unit.
+    // a hipGraph and launched as one unit.  run() = prepare, launch, collect;
+    // split, they let the host work on one recurrence's results while the
+    // GPU runs another's steps.
     void run(int M, std::vector<double>& ah, std::vector<double>& bh, bool graph = false) {
+        prepare(M, graph);
+        launch();
+        collect(ah, bh, st);
+    }
+
+    // alpha/beta cleared; with `graph`, the steps captured (the stream idle)
+    void prepare(int M, bool graph) {
+        steps_ = M;
         const size_t tot = size_t(std::max(M, 1)) * std::max(nsamp, 1);
         alpha.resize(sizeof(double) * tot);
         beta.resize(sizeof(double) * tot);
         QK_HIP(hipMemsetAsync(alpha.get(), 0, sizeof(double) * tot, st));
         QK_HIP(hipMemsetAsync(beta.get(), 0, sizeof(double) * tot, st));
-        auto steps = [&]() {
-            cplx* U = bufU.get<cplx>();
-            cplx* Y = bufY.get<cplx>();
-            cplx* P = bufP.get<cplx>();
-            for (int step = 0; step < M; ++step) {
-                double* a = alpha.get<double>() + size_t(step) * nsamp;
-                double* b = beta.get<double>() + size_t(step) * nsamp;
-                if (dstride) {
-                    if (mparts) {
-                        multi.launch(U, Y, st, dparts.get<double>());
-                    } else {
-                        size_t j = 0;
-                        for (auto& g : grps)
-                            for (int r = 0; r < g.nvec; ++r, ++j) {
-                                const u64 o = g.off + u64(r) * g.h->dim();
-                                double* part = dparts.get<double>() + j * dstride;
-                                if (g.h->dim() && !g.h->apply_dot(U + o, Y + o, part, st))
-                                    fail("block recurrence: fused apply not available");
-                            }
-                    }
-                    vec::alpha_from_parts(dparts.get<double>(), dstride, dcount.get<u32>(), nsamp,
-                                          c.get<double>(), a, alive.get<int>(), st);
-                } else {
-                    if (multi.ok()) multi.launch(U, Y, st);
-                    else
-                        for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
-                    vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);
-                }
-                vec::seg_update(seg, Y, U, P, c.get<double>(), cp.get<double>(), bp.get<double>(), a,
-                                b, alive.get<int>(), step == 0 ? 1 : 0, st);
-                cplx* oldP = P;
-                P = U;
-                U = Y;
-                Y = oldP;
-            }
-        };
+        release_graph();
         if (graph && M > 0) {
             QK_HIP(hipStreamSynchronize(st));
-            hipGraph_t g = nullptr;
-            hipGraphExec_t ge = nullptr;
             QK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             try {
-                steps();
+                steps(M);
             } catch (...) {
-                (void)hipStreamEndCapture(st, &g);
-                if (g) (void)hipGraphDestroy(g);
+                (void)hipStreamEndCapture(st, &graph_);
+                release_graph();
                 throw;
             }
-            QK_HIP(hipStreamEndCapture(st, &g));
-            const hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            QK_HIP(hipStreamEndCapture(st, &graph_));
+            const hipError_t e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
             if (e != hipSuccess) {
-                (void)hipGraphDestroy(g);
+                exec_ = nullptr;
+                release_graph();
                 QK_HIP(e);
             }
-            QK_HIP(hipGraphLaunch(ge, st));
-            QK_HIP(hipStreamSynchronize(st));
-            (void)hipGraphExecDestroy(ge);
-            (void)hipGraphDestroy(g);
-        } else {
-            steps();
         }
+    }
+
+    // the steps queued on the stream (the graph, or direct launches)
+    void launch() {
+        if (exec_) QK_HIP(hipGraphLaunch(exec_, st));
+        else steps(steps_);
+        if (!done_) QK_HIP(hipEventCreateWithFlags(&done_, hipEventDisableTiming));
+        QK_HIP(hipEventRecord(done_, st));
+    }
+
+    // waits for the steps and copies alpha/beta on stream cs (which may be
+    // another stream than the one the steps run on)
+    void collect(std::vector<double>& ah, std::vector<double>& bh, hipStream_t cs) {
+        const int M = steps_;
         ah.resize(size_t(M) * nsamp);
         bh.resize(size_t(M) * nsamp);
+        QK_HIP(hipStreamWaitEvent(cs, done_, 0));
         if (M > 0 && nsamp > 0) {
-            QK_HIP(hipMemcpyAsync(ah.data(), alpha.get(), sizeof(double) * M * nsamp, hipMemcpyDeviceToHost, st));
-            QK_HIP(hipMemcpyAsync(bh.data(), beta.get(), sizeof(double) * M * nsamp, hipMemcpyDeviceToHost, st));
+            QK_HIP(hipMemcpyAsync(ah.data(), alpha.get(), sizeof(double) * M * nsamp, hipMemcpyDeviceToHost, cs));
+            QK_HIP(hipMemcpyAsync(bh.data(), beta.get(), sizeof(double) * M * nsamp, hipMemcpyDeviceToHost, cs));
         }
-        QK_HIP(hipStreamSynchronize(st));
+        QK_HIP(hipStreamSynchronize(cs));
+        release_graph();
+    }
+
+    BlockRec() = default;
+    BlockRec(const BlockRec&) = delete;
+    BlockRec& operator=(const BlockRec&) = delete;
+    ~BlockRec() {
+        if (st && (exec_ || done_)) (void)hipStreamSynchronize(st);
+        release_graph();
+        if (done_) (void)hipEventDestroy(done_);
+    }
+
+private:
+    int steps_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    hipEvent_t done_ = nullptr;
+
+    void release_graph() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        exec_ = nullptr;
+        graph_ = nullptr;
+    }
+
+    void steps(int M) {
+        cplx* U = bufU.get<cplx>();
+        cplx* Y = bufY.get<cplx>();
+        cplx* P = bufP.get<cplx>();
+        for (int step = 0; step < M; ++step) {
+            double* a = alpha.get<double>() + size_t(step) * nsamp;
+            double* b = beta.get<double>() + size_t(step) * nsamp;
+            if (dstride) {
+                if (mparts) {
+                    multi.launch(U, Y, st, dparts.get<double>());
+                } else {
+                    size_t j = 0;
+                    for (auto& g : grps)
+                        for (int r = 0; r < g.nvec; ++r, ++j) {
+                            const u64 o = g.off + u64(r) * g.h->dim();
+                            double* part = dparts.get<double>() + j * dstride;
+                            if (g.h->dim() && !g.h->apply_dot(U + o, Y + o, part, st))
+                                fail("block recurrence: fused apply not available");
+                        }
+                }
+                vec::alpha_from_parts(dparts.get<double>(), dstride, dcount.get<u32>(), nsamp,
+                                      c.get<double>(), a, alive.get<int>(), st);
+            } else {
+                if (multi.ok()) multi.launch(U, Y, st);
+                else
+                    for (auto& g : grps) g.h->apply(U + g.off, Y + g.off, g.nvec, g.h->dim(), st);
+                vec::seg_alpha(seg, U, Y, c.get<double>(), a, alive.get<int>(), st);
+            }
+            vec::seg_update(seg, Y, U, P, c.get<double>(), cp.get<double>(), bp.get<double>(), a,
+                            b, alive.get<int>(), step == 0 ? 1 : 0, st);
+            cplx* oldP = P;
+            P = U;
+            U = Y;
+            Y = oldP;
+        }
     }
 };
 
@@ -1416,13 +1459,26 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
     if (dstream) stream = dstream;
     tm.mark("random stream");
 
-    // per (sector, vector) tridiagonal results
+    // per (sector, vector) tridiagonal results and their eigensolves
     struct Sample {
         int sec;
         double nrm;
-        std::vector<double> a, b;
+        std::vector<double> a, b, eig, z0;
     };
     std::vector<Sample> samples;
+    size_t solved = 0;
+    // small eigensolves of the samples not solved yet (:151-152)
+    auto solve_new = [&]() {
+        const size_t from = solved;
+        parallel_for(int(samples.size() - from), [&](int k) {
+            Sample& sm = samples[from + size_t(k)];
+            const int n = int(sm.a.size());
+            sm.eig.resize(n);
+            sm.z0.resize(n);
+            tridiag_ql(sm.a.data(), sm.b.data(), n, sm.eig.data(), sm.z0.data(), nullptr);
+        });
+        solved = samples.size();
+    };
     for (auto* h : hs) QK_HIP(hipStreamSynchronize(h->stream));
 
     if (mode == 0) {
@@ -1434,23 +1490,67 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
                 secs.push_back(s);
             }
         if (!g.empty()) {
-            BlockRec rec;
-            rec.setup(g, st);
-            for (size_t gi = 0; gi < g.size(); ++gi)
-                QK_HIP(hipMemcpyAsync(rec.bufU.get<cplx>() + rec.grps[gi].off, stream,
-                                      sizeof(cplx) * g[gi].first->dim() * R, kind, st));
-            const std::vector<double> nrm = rec.start(0.0);
-            tm.mark("stage vectors");
-            std::vector<double> ah, bh;
-            rec.run(n_steps, ah, bh, /*graph=*/true);
-            tm.mark("block lanczos");
-            int j = 0;
-            for (size_t gi = 0; gi < g.size(); ++gi)
-                for (int r = 0; r < R; ++r, ++j) {
-                    Sample smp{secs[gi], nrm[j], {}, {}};
-                    extract(ah, bh, n_steps, rec.nsamp, j, smp.a, smp.b);
-                    samples.push_back(std::move(smp));
+            // Two recurrences over halves of the sectors (by work, largest
+            // first to the lighter half): the host solves the first half's
+            // tridiagonals while the GPU runs the second half's steps.  Each
+            // sample's arithmetic is its own, whichever half it is in.
+            std::vector<int> half(g.size(), 0);
+            if (g.size() >= 2) {
+                std::vector<size_t> order(g.size());
+                for (size_t i = 0; i < g.size(); ++i) order[i] = i;
+                std::stable_sort(order.begin(), order.end(),
+                                 [&](size_t a, size_t b) { return g[a].first->dim() > g[b].first->dim(); });
+                u64 load[2] = {0, 0};
+                for (size_t i : order) {
+                    const int hh = load[1] < load[0] ? 1 : 0;
+                    half[i] = hh;
+                    load[hh] += g[i].first->dim();
                 }
+            }
+            BlockRec rec[2];
+            std::vector<size_t> members[2];
+            std::vector<double> nrm[2];
+            for (int hh = 0; hh < 2; ++hh) {
+                std::vector<std::pair<const Hamiltonian*, int>> gh;
+                for (size_t gi = 0; gi < g.size(); ++gi)
+                    if (half[gi] == hh) {
+                        gh.push_back(g[gi]);
+                        members[hh].push_back(gi);
+                    }
+                if (gh.empty()) continue;
+                rec[hh].setup(gh, st);
+                for (size_t k = 0; k < gh.size(); ++k)
+                    QK_HIP(hipMemcpyAsync(rec[hh].bufU.get<cplx>() + rec[hh].grps[k].off, stream,
+                                          sizeof(cplx) * gh[k].first->dim() * R, kind, st));
+                nrm[hh] = rec[hh].start(0.0);
+                rec[hh].prepare(n_steps, /*graph=*/true);
+            }
+            tm.mark("stage vectors");
+            for (int hh = 0; hh < 2; ++hh)
+                if (!members[hh].empty()) rec[hh].launch();
+            hipStream_t cs = nullptr;
+            QK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            try {
+                for (int hh = 0; hh < 2; ++hh) {
+                    if (members[hh].empty()) continue;
+                    std::vector<double> ah, bh;
+                    rec[hh].collect(ah, bh, cs);
+                    int j = 0;
+                    for (size_t gi : members[hh])
+                        for (int r = 0; r < R; ++r, ++j) {
+                            Sample smp{secs[gi], nrm[hh][j], {}, {}, {}, {}};
+                            extract(ah, bh, n_steps, rec[hh].nsamp, j, smp.a, smp.b);
+                            samples.push_back(std::move(smp));
+                        }
+                    solve_new();   // overlaps the other half's steps on the GPU
+                }
+            } catch (...) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(cs);
+                throw;
+            }
+            QK_HIP(hipStreamDestroy(cs));
+            tm.mark("block lanczos + eigensolves");
         }
     } else {
         for (int s = 0; s < nsec; ++s) {
@@ -1464,7 +1564,7 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
                 const std::vector<double> nrm = rec.start(0.0);
                 std::vector<double> ah, bh;
                 rec.run(n_steps, ah, bh);
-                Sample smp{s, nrm[0], {}, {}};
+                Sample smp{s, nrm[0], {}, {}, {}, {}};
                 extract(ah, bh, n_steps, 1, 0, smp.a, smp.b);
                 samples.push_back(std::move(smp));
             }
@@ -1472,14 +1572,9 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
     }
 
     tm.mark("lanczos (all)");
-    // small eigensolves, then Boltzmann weights for every beta (:151-167)
-    std::vector<std::vector<double>> eig(samples.size()), z0(samples.size());
-    parallel_for(int(samples.size()), [&](int i) {
-        const int n = int(samples[i].a.size());
-        eig[i].resize(n);
-        z0[i].resize(n);
-        tridiag_ql(samples[i].a.data(), samples[i].b.data(), n, eig[i].data(), z0[i].data(), nullptr);
-    });
+    // small eigensolves (those not done yet), then Boltzmann weights for
+    // every beta (:151-167)
+    solve_new();
     tm.mark("tridiagonal eigensolves");
     std::vector<double> Zs(size_t(nsec) * nb, 0.0), Es(Zs.size(), 0.0), E2s(Zs.size(), 0.0);
     // The sums of one (sector, beta) run in sample order in registers and are
@@ -1507,11 +1602,13 @@ std::vector<FTLMPoint> ftlm_sweep(const std::vector<const Hamiltonian*>& hs,
                 E2 = E2s[o];
             }
             const double n2 = samples[i].nrm * samples[i].nrm;
-            for (size_t k = 0; k < eig[i].size(); ++k) {
-                const double w = n2 * z0[i][k] * z0[i][k] * std::exp(-beta * eig[i][k]);
+            const std::vector<double>& eig = samples[i].eig;
+            const std::vector<double>& z0 = samples[i].z0;
+            for (size_t k = 0; k < eig.size(); ++k) {
+                const double w = n2 * z0[k] * z0[k] * std::exp(-beta * eig[k]);
                 Z += w;
-                E += eig[i][k] * w;
-                E2 += eig[i][k] * eig[i][k] * w;
+                E += eig[k] * w;
+                E2 += eig[k] * eig[k] * w;
             }
         }
         flush();
